@@ -1,0 +1,106 @@
+/*
+ * redi_oracle.h -- CPU restatement of the reference Redi step (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * this library, and only as the checker / the CPU baseline.  The product
+ * (the HIP library under more_benchmarks_dc_ocean_amd/) never links or calls
+ * it.
+ *
+ * Every function follows one routine of the reference, cited per function in
+ * redi_oracle.c.  Array arguments use the reference layout: Fortran
+ * column-major, i fastest, then j, then k (or the neutral-surface index s).
+ * Layer indices inside a column are 1-based in the reference; the column
+ * helpers here take 0-based C arrays whose element [m] is the reference's
+ * element (m+1).  Ko values (neutral-surface layer indices) are stored exactly
+ * as the reference stores them: 1-based, top-down.
+ */
+#ifndef REDI_ORACLE_H
+#define REDI_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ocean_eos_t (redi/ocean_eos.F90:239-278): variant 1 linear, 2 Wright-97,
+ * 4 Roquet SpV. */
+typedef struct {
+    int variant;
+    int pad_;
+    double rho0, T_ref, S_ref, alpha_T, beta_S, p_ref;
+} oracle_eos_t;
+
+/* ---- column helpers (point routines) ---------------------------------- */
+double oracle_fv_diff(double hkm1, double hk, double hkp1,
+                      double skm1, double sk, double skp1);
+void oracle_plm_diff(int nk, const double *h, const double *s, double *diff);
+double oracle_ppm_edge(double hkm1, double hk, double hkp1, double hkp2,
+                       double ak, double akp1, double pk, double pkp1,
+                       double h_neglect);
+void oracle_interface_scalar(int nk, const double *h, const double *tr,
+                             double *edge, double h_neglect);
+double oracle_interpolate_position(double dRhoNeg, double Pneg,
+                                   double dRhoPos, double Ppos);
+void oracle_neutral_positions_continuous(
+    int nk, const double *Pl, const double *Tl, const double *Sl,
+    const double *dRdTl, const double *dRdSl, const double *Pr,
+    const double *Tr, const double *Sr, const double *dRdTr,
+    const double *dRdSr, double *PoL, double *PoR, int *KoL, int *KoR,
+    double *hEff);
+void oracle_roquet_spv_point(double T_pt, double S_sp, double p, double *sv,
+                             double *dsv_dt, double *dsv_ds);
+void oracle_eos_specvol_derivs(const oracle_eos_t *eos, double T, double S,
+                               double p, double *dsv_dt, double *dsv_ds);
+double oracle_eos_density_point(const oracle_eos_t *eos, double T, double S,
+                                double p);
+void oracle_build_column(int nz, const double *h_col, const double *thtr_col,
+                         const double *shtr_col, const oracle_eos_t *eos,
+                         double *Pint, double *Tint, double *Sint,
+                         double *dRdT, double *dRdS);
+void oracle_face_coeffs(int nz, const double *hL, const double *tL,
+                        const double *sL, const double *hR, const double *tR,
+                        const double *sR, const oracle_eos_t *eos,
+                        double *PoL, double *PoR, int *KoL, int *KoR,
+                        double *hEff);
+double oracle_ppm_ave(double xL, double xR, double aL, double aR, double aMean);
+void oracle_tracer_column(int nz, const double *h_col, const double *htr_col,
+                          double *Tlay, double *Tint, double *aLe,
+                          double *aRe);
+/* klt..krb are the reference's 1-based top-down layer indices. */
+double oracle_sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
+                          double PoLb, double PoRt, double PoRb,
+                          const double *TlL, const double *TiL,
+                          const double *aLL, const double *aRL,
+                          const double *TlR, const double *TiR,
+                          const double *aLR, const double *aRR);
+
+/* ---- whole-field phases ------------------------------------------------ */
+/* redi_calc_coeffs_x + _y for all faces (nx, ny are total sizes incl.
+ * ghosts).  Faces outside the reference's loop ranges are not touched. */
+void oracle_calc_coeffs(int nx, int ny, int nz, const oracle_eos_t *eos,
+                        const double *h_layer, const double *t_htr,
+                        const double *s_htr, const double *wet_u,
+                        const double *wet_v, double *uPoL, double *uPoR,
+                        int *uKoL, int *uKoR, double *uhEff, double *vPoL,
+                        double *vPoR, int *vKoL, int *vKoR, double *vhEff);
+
+/* redi_apply_flux_impl for ONE tracer: hTr += dTr/areaT over all cells,
+ * reading the neighbour columns from hTr_in (the snapshot). */
+void oracle_apply_flux_tracer(
+    int nx, int ny, int nz, double dt, int nghost, int nxp, int nyp,
+    int wall_w, int wall_e, int wall_s, int wall_n, const double *khtr_u,
+    const double *khtr_v, const double *dy_cu, const double *dx_cv,
+    const double *idxCu, const double *idyCv, const double *areaT,
+    const double *h_layer, const double *hTr_in, double *hTr,
+    const double *uPoL, const double *uPoR, const int *uKoL, const int *uKoR,
+    const double *uhEff, const double *vPoL, const double *vPoR,
+    const int *vKoL, const int *vKoR, const double *vhEff);
+
+/* OpenMP thread count used by the two whole-field phases (<=0: runtime
+ * default).  Returns the count that will be used. */
+int oracle_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
