@@ -1,0 +1,220 @@
+"""Redi step benchmark on one MI355X.
+
+Workload (default): the 1440x1080x75 domain, 4 tracers, Roquet SpV EOS,
+~30% land, two fronts plus mesoscale noise, dt = 900 s, kappa = 1000 m^2/s,
+all-wall edges (nghost = 3, so arrays are 1446x1086x75).  One "step" is the
+reference's timed region: redi_calc_coeffs (Phase A) followed by
+redi_apply_flux over all four tracers (Phase B), every tracer updated in
+place.  Inputs are generated on the host, copied to HBM, and resident there
+before the timed region starts.
+
+Printed JSON line: metric "Redi Mcell-steps/s" (interior cells x steps per
+second), plus
+  roofline      the dominant kernel's algorithmic bytes per launch divided by
+                its average launch time (HIP events around every launch on
+                the launch stream, recorded inside the timed region), against
+                the 8 TB/s HBM peak; ``traffic`` is the rocprofv3-measured
+                HBM bytes per launch when a committed measurement for this
+                kernel and workload exists under profiles/;
+  cpu_baseline  the reference's own Fortran Redi (oracle/_ref/redi_ref, built
+                from the reference sources) timed on this host's cores over
+                a bounded sample of the same workload, or the oracle's C
+                restatement when that binary is absent.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (builder, description)
+    "cfg2": ("case_cfg2", "redi_1440x1080x75_roquet_4tr"),
+    "cfg1": ("case_cfg1", "redi_473x297x75_roquet_4tr"),
+    "cfg0": ("case_cfg0", "redi_64x64x25_linear_2tr"),
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(case, kernel: str) -> float:
+    """Compulsory HBM bytes of one launch of ``kernel`` under the byte model
+    of the public two-call contract: h read once per step; each tracer read
+    and written once; the Phase-A coefficients written once and read back
+    once; the 2-D metrics read once.  A per-tracer Phase-B launch carries one
+    tracer's read+write and a 1/nt share of the coefficient read-back and of
+    the metrics.  FP64: 8-byte values, 4-byte Ko."""
+    nx, ny, nz, nt = case.nx, case.ny, case.nz, case.nt
+    ns = 2 * nz + 2
+    n3 = nx * ny * nz
+    nu, nv, n2 = (nx + 1) * ny, nx * (ny + 1), nx * ny
+    coeff = (nu + nv) * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
+    metrics = 8 * (4 * nu + 4 * nv + n2)
+    if kernel == "k_apply_flux":
+        return 2 * 8 * n3 + (coeff + metrics) / nt
+    if kernel == "k_tracer_recon":
+        return 0.0
+    if kernel == "k_column_state":
+        return 3 * 8 * n3
+    if kernel in ("k_neutral_sweep_u", "k_neutral_sweep_v"):
+        nf = nu if kernel.endswith("_u") else nv
+        return nf * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
+    return 0.0
+
+
+def step_algorithmic_bytes(case) -> float:
+    nx, ny, nz, nt = case.nx, case.ny, case.nz, case.nt
+    ns = 2 * nz + 2
+    n3 = nx * ny * nz
+    nu, nv, n2 = (nx + 1) * ny, nx * (ny + 1), nx * ny
+    return (8 * n3 + 2 * nt * 8 * n3 + 2 * (nu + nv) * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
+            + 8 * (4 * nu + 4 * nv + n2))
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def cpu_baseline(case, sample_steps: int = 2):
+    """Time the reference build (or the C restatement) on a cropped sample of
+    the workload: a 480x360 interior window of the same domain, all layers and
+    tracers, 1 warm-up + ``sample_steps`` timed steps, on every host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import redi_oracle
+    from more_benchmarks_dc_ocean_amd.inputs import crop
+    nxp = min(480, case.nx_phys)
+    nyp = min(360, case.ny_phys)
+    i0 = (case.nx_phys - nxp) // 2
+    j0 = (case.ny_phys - nyp) // 2
+    sub = crop(case, i0, j0, nxp, nyp)
+    threads = min(os.cpu_count() or 1, 64)
+    cells = sub.nx_phys * sub.ny_phys * sub.nz
+    sample = (f"{nxp}x{nyp}x{sub.nz} interior window of the workload domain, {sub.nt} tracers, "
+              f"{sample_steps} timed steps after 1 warm-up")
+    if redi_oracle.have_reference():
+        try:
+            _, out = redi_oracle.run_reference(sub, nsteps=sample_steps, threads=threads, nwarm=1,
+                                               want_output=False)
+            ms = float(out.split("ms_per_step=")[1].split()[0])
+            return {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+                    "kind": "reference", "sample": sample, "ms_per_step": ms}
+        except Exception as e:  # the binary exists but cannot run here
+            log(f"reference CPU build failed ({e}); timing the C restatement instead")
+    redi_oracle.set_threads(threads)
+    o = redi_oracle.OracleRedi(sub)
+    import numpy as np
+    h = np.ascontiguousarray(sub.h)
+    hTr = sub.hTr.copy()
+    o.step(h, hTr)
+    t0 = time.perf_counter()
+    for _ in range(sample_steps):
+        o.step(h, hTr)
+    ms = (time.perf_counter() - t0) * 1e3 / sample_steps
+    return {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+            "kind": "port", "sample": sample, "ms_per_step": ms}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the Redi step of one "
+                                   "domain is not split across GPUs (--gpus must be 1)"}))
+        return 2
+
+    import torch
+    from more_benchmarks_dc_ocean_amd import inputs
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+
+    builder, wname = WORKLOADS[args.workload]
+    t0 = time.time()
+    case = getattr(inputs, builder)(nsteps=args.steps)
+    log(f"generated {wname} ({case.nx}x{case.ny}x{case.nz}, nt={case.nt}, "
+        f"land={case.land_fraction:.3f}) in {time.time() - t0:.1f}s")
+    dev = "cuda:0"
+    r, dom = make_redi(case, dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        r.step(dom, 1)
+    torch.cuda.synchronize()
+    log(f"warm-up {args.warmup} steps done")
+
+    r.timing(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        r.step(dom, 1)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_total = ev0.elapsed_time(ev1)
+    kt = r.collect_timing()
+    r.timing(False)
+    ms_per_step = ms_total / args.steps
+    value = case.interior_cells * args.steps / (ms_total * 1e-3) / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall * 1e3 / args.steps:.3f})")
+    for k, (ms, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+        log(f"  {k:20s} {ms / n:9.4f} ms/launch x{n:4d}  ({100 * ms / ms_total:5.1f}% of step time)")
+
+    dom_k = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    avg_ms = kt[dom_k][0] / kt[dom_k][1]
+    alg = algorithmic_bytes(case, dom_k)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": measured_traffic(dom_k, args.workload),
+            "alg_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 4),
+            "step_alg_GBps": round(step_algorithmic_bytes(case) * args.steps / (ms_total * 1e-3) / 1e9, 2)}
+    r.close()
+    del dom
+    torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(case)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:
+            log(f"cpu baseline failed: {e}")
+
+    line = {"metric": "Redi Mcell-steps/s", "value": round(value, 3), "unit": "Mcell-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": wname, "nx_phys": case.nx_phys, "ny_phys": case.ny_phys,
+                       "nz": case.nz, "ntracers": case.nt, "eos": "roquet_spv",
+                       "land_fraction": round(case.land_fraction, 4), "form": "fp64",
+                       "parallelism": "single_gpu"},
+            "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,233 @@
+/*
+ * redi_hip.h -- C ABI of the MI355X (gfx950) Redi neutral-diffusion step.
+ *
+ * Drop-in boundary for the reference's ocean_redi call surface.  Every entry
+ * point takes plain pointers and sizes; array pointers are DEVICE pointers
+ * (HBM) laid out exactly like the reference's Fortran arrays (column-major:
+ * i fastest, then j, then k or the neutral-surface index s).  Streams are
+ * passed as `void*` holding a hipStream_t (NULL = the null stream).  Unless
+ * stated otherwise a call only enqueues work on its stream and returns.
+ *
+ * Return codes: REDI_OK (0) = work enqueued; REDI_SKIPPED (1) = one of the
+ * reference's silent no-op guards fired and nothing was done; negative =
+ * error, with a message from redi_last_error().
+ */
+#ifndef REDI_HIP_H
+#define REDI_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REDI_OK 0
+#define REDI_SKIPPED 1
+#define REDI_ERR_ARG (-1)
+#define REDI_ERR_NZ (-2)
+#define REDI_ERR_HIP (-3)
+#define REDI_ERR_STATE (-4)
+#define REDI_ERR_CONFIG (-5)
+
+/* Largest supported layer count (the reference caps 2*nz+2 at its
+ * compile-time NZ_STACK_MAX, redi/ocean_redi.F90:483-486). */
+#define REDI_NZ_MAX 256
+
+#define REDI_EOS_LINEAR 1
+#define REDI_EOS_WRIGHT97 2
+#define REDI_EOS_ROQUET_SPV 4
+
+#define REDI_OBC_WALL 1
+
+/* hgrid_t (redi/grid.F90:17-25) */
+typedef struct {
+    int nx_total, ny_total; /* incl. ghosts: n_phys + 2*nghost */
+    int nx_phys, ny_phys;
+    int nghost;
+    int pad_;
+    double dx, dy;
+} redi_hgrid_t;
+
+/* ocean_metrics_t (redi/ocean_metrics.F90:14-22); device pointers */
+typedef struct {
+    const double *dy_cu; /* (nx+1, ny) */
+    const double *dx_cv; /* (nx, ny+1) */
+    const double *idxCu; /* (nx+1, ny) */
+    const double *idyCv; /* (nx, ny+1) */
+    const double *areaT; /* (nx, ny)   */
+    const double *wet_u; /* (nx+1, ny) */
+    const double *wet_v; /* (nx, ny+1) */
+} redi_metrics_t;
+
+/* ocean_eos_t (redi/ocean_eos.F90:239-278) */
+typedef struct {
+    int variant; /* REDI_EOS_* */
+    int ts_convention;
+    double rho0, T_ref, S_ref, alpha_T, beta_S, p_ref;
+} redi_eos_t;
+
+/* multilayer_cgrid_state_t (redi/multilayer_cgrid_state.F90:18-28).
+ * `tracers` is a HOST array of ntracers device pointers to hTr (nx,ny,nz). */
+typedef struct {
+    int nz_ml;
+    int idx_temperature; /* 1-based; 0 = not registered */
+    int idx_salinity;
+    int ntracers;
+    const double *h_layer; /* (nx, ny, nz), device */
+    double *const *tracers;
+} redi_state_t;
+
+/* ocean_bc_state_t (redi/ocean_boundary_types.F90:14-22): per-edge tags,
+ * REDI_OBC_WALL = no-normal-flow wall. */
+typedef struct {
+    int west, east, south, north;
+} redi_bc_t;
+
+/* Phase-A coefficient arrays, the public components uPoL..vhEff of
+ * ocean_redi_t (redi/ocean_redi.F90:105-110), plus khtr_u/khtr_v
+ * (:115-116).  Device pointers; Ko are int32; layouts (nx+1,ny,ns),
+ * (nx+1,ny,ns-1), (nx,ny+1,ns), (nx,ny+1,ns-1), (nx+1,ny), (nx,ny+1). */
+typedef struct {
+    double *uPoL, *uPoR;
+    int *uKoL, *uKoR;
+    double *uhEff;
+    double *vPoL, *vPoR;
+    int *vKoL, *vKoR;
+    double *vhEff;
+    double *khtr_u, *khtr_v;
+} redi_coeffs_t;
+
+/* Opaque ocean_redi_t. */
+typedef struct redi_s redi_t;
+
+/* ocean_redi_init (redi/ocean_redi.F90:466-507).  nz_ml < 1 is taken as 1;
+ * 2*nz+2 > 2*REDI_NZ_MAX+2 returns REDI_ERR_NZ (the reference's error stop).
+ * `bufs` may be NULL (the library allocates the coefficient arrays) or give
+ * caller-owned device arrays, which are then filled like the reference's
+ * `allocate(..., source=0 / 1)`.  Synchronous. */
+int redi_init(redi_t **self, const redi_hgrid_t *grid, int nz_ml,
+              const redi_coeffs_t *bufs);
+
+/* ocean_redi_destroy (redi/ocean_redi.F90:509-529).  Synchronous. */
+int redi_destroy(redi_t *self);
+
+/* The physics knobs of ocean_redi_t (:78-90): enable, khtr (m^2/s),
+ * continuous (0 is rejected, as the reference rejects it at configure),
+ * ref_pres (held, never read, as in the reference). */
+int redi_configure(redi_t *self, int enable, double khtr, int continuous,
+                   double ref_pres);
+
+/* Device pointers of the coefficient arrays (owned or bound). */
+int redi_get_coeffs(const redi_t *self, redi_coeffs_t *out);
+
+/* redi_calc_coeffs (redi/ocean_redi.F90:572-607): Phase A. */
+int redi_calc_coeffs(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+                     const redi_eos_t *eos, redi_t *self,
+                     const redi_state_t *ms, void *stream);
+
+/* redi_apply_flux (redi/ocean_redi.F90:927-1002): Phase B for every tracer.
+ * khtr_u_ext/khtr_v_ext: both non-NULL = the VarMix (use_ext) path, else the
+ * scalar khtr.  bc NULL = all four edges are walls. */
+int redi_apply_flux(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+                    redi_t *self, const redi_state_t *ms, double dt,
+                    const double *khtr_u_ext, const double *khtr_v_ext,
+                    const redi_bc_t *bc, void *stream);
+
+/* redi_apply_flux_hoist (redi/ocean_redi.F90:1201-1262): same contract and
+ * same bits as redi_apply_flux; here both run the same kernels, since the
+ * column reconstruction is always done once per column. */
+int redi_apply_flux_hoist(const redi_hgrid_t *grid,
+                          const redi_metrics_t *metrics, redi_t *self,
+                          const redi_state_t *ms, double dt,
+                          const double *khtr_u_ext, const double *khtr_v_ext,
+                          const redi_bc_t *bc, void *stream);
+
+/* One full step: redi_calc_coeffs then redi_apply_flux, repeated nsteps
+ * times (Phase B is not idempotent; hTr accumulates). */
+int redi_step(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+              const redi_eos_t *eos, redi_t *self, const redi_state_t *ms,
+              double dt, const double *khtr_u_ext, const double *khtr_v_ext,
+              const redi_bc_t *bc, int nsteps, void *stream);
+
+/* Bytes of device scratch an ocean_redi_t of this size holds beyond the
+ * reference's own arrays (the column reconstruction buffers). */
+long long redi_workspace_bytes(int nx_total, int ny_total, int nz);
+
+/* ---- per-kernel timing ---------------------------------------------------
+ * When enabled, every kernel launch made for this ocean_redi_t is bracketed
+ * by HIP events recorded on the launch's own stream.  redi_timing_collect
+ * waits for those events, adds their elapsed times per kernel kind into
+ * `out` and forgets them. */
+#define REDI_NKERNELS 8
+#define REDI_K_COLUMN_STATE 0
+#define REDI_K_SWEEP_U 1
+#define REDI_K_SWEEP_V 2
+#define REDI_K_FACE_KHTR 3
+#define REDI_K_TRACER_RECON 4
+#define REDI_K_APPLY_FLUX 5
+typedef struct {
+    double ms[REDI_NKERNELS];           /* summed kernel time */
+    long long launches[REDI_NKERNELS];  /* launches timed */
+} redi_kernel_times_t;
+int redi_timing_enable(redi_t *self, int on);
+int redi_timing_collect(redi_t *self, redi_kernel_times_t *out);
+const char *redi_kernel_name(int kind);
+
+/* Message of the last error on this host thread ("" if none). */
+const char *redi_last_error(void);
+
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int redi_version(void);
+
+/* ---- flat compatibility entries ------------------------------------------
+ * Same symbol, argument list and semantics as the reference's hand-written
+ * GPU launchers redi_opt_launch (redi/opt_kernel.cu:534-574) and
+ * redi_cuda_launch_ (redi/redi_kernel.cu:537-580): Fortran by-reference
+ * scalars, caller-owned device arrays, exactly two tracers (t_htr, s_htr,
+ * updated in place), linear EOS, use_ext path, walls ww/we/wsz/wn (1 =
+ * wall), default stream, synchronous (ends with a device synchronise),
+ * no return code.  tr_snap is accepted and not used (the reconstruction
+ * buffers are the snapshot).  Scratch beyond the caller's arrays comes from
+ * a per-device grow-only cache. */
+void redi_opt_launch(
+    const int *nx, const int *ny, const int *nz, const int *ns,
+    const double *dt, const int *nghost, const int *nxp, const int *nyp,
+    const int *ww, const int *we, const int *wsz, const int *wn,
+    const double *rho0, const double *T_ref, const double *S_ref,
+    const double *alpha_T, const double *beta_S, const double *h_layer,
+    double *t_htr, double *s_htr, const double *wet_u, const double *wet_v,
+    double *uPoL, double *uPoR, int *uKoL, int *uKoR, double *uhEff,
+    double *vPoL, double *vPoR, int *vKoL, int *vKoR, double *vhEff,
+    const double *khtr_u_ext, const double *khtr_v_ext, double *khtr_u,
+    double *khtr_v, const double *dy_cu, const double *dx_cv,
+    const double *idxCu, const double *idyCv, const double *areaT,
+    double *tr_snap);
+void redi_cuda_launch_(
+    const int *nx, const int *ny, const int *nz, const int *ns,
+    const double *dt, const int *nghost, const int *nxp, const int *nyp,
+    const int *ww, const int *we, const int *wsz, const int *wn,
+    const double *rho0, const double *T_ref, const double *S_ref,
+    const double *alpha_T, const double *beta_S, const double *h_layer,
+    double *t_htr, double *s_htr, const double *wet_u, const double *wet_v,
+    double *uPoL, double *uPoR, int *uKoL, int *uKoR, double *uhEff,
+    double *vPoL, double *vPoR, int *vKoL, int *vKoR, double *vhEff,
+    const double *khtr_u_ext, const double *khtr_v_ext, double *khtr_u,
+    double *khtr_v, const double *dy_cu, const double *dx_cv,
+    const double *idxCu, const double *idyCv, const double *areaT,
+    double *tr_snap);
+
+/* Extension of the flat entry to any tracer count and EOS variant, with a
+ * return code and a stream (asynchronous): the nt tracer device pointers are
+ * passed in the host array `tracers`; idx_T/idx_S are 1-based. */
+int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp,
+                   int nyp, int ww, int we, int wsz, int wn,
+                   const redi_eos_t *eos, const double *h_layer, int nt,
+                   double *const *tracers, int idx_T, int idx_S,
+                   const double *wet_u, const double *wet_v,
+                   const redi_coeffs_t *coeffs, const double *khtr_u_ext,
+                   const double *khtr_v_ext, const double *dy_cu,
+                   const double *dx_cv, const double *idxCu,
+                   const double *idyCv, const double *areaT, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REDI_HIP_H */

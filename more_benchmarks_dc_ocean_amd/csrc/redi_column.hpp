@@ -1,0 +1,164 @@
+// redi_column.hpp -- per-column reconstruction and neutral-sweep device code.
+//
+// The reference reconstructs columns into NZ_STACK_MAX-sized per-thread
+// arrays (redi/ocean_redi.F90:231-254, 609-654, 852-891), which on CDNA
+// becomes scratch spill traffic.  Here a column is streamed top-down once
+// with a four-layer register window: the PPM edge at interface k only needs
+// layers k-2..k+1 and the PLM slopes of layers k-1 and k, so each layer is
+// loaded exactly once (coalesced along i, since lanes are adjacent columns)
+// and every interface value is emitted as soon as it is known.
+//
+// The arithmetic of every helper below is the reference's, operation by
+// operation (see the cited lines), so the FP64 instantiation compiled without
+// FP contraction reproduces the reference bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace redi {
+
+constexpr double kGravity = 9.80665;   // redi/constants.F90:19
+constexpr double kHDivEps = 1.0e-20;   // redi/constants.F90:21
+constexpr double kHNeglect = 1.0e-30;  // redi/ocean_redi.F90:242
+
+template <class R>
+__device__ __forceinline__ R f_sign(R a, R b) {  // Fortran SIGN(a, b)
+  return copysign(fabs(a), b);
+}
+template <class R>
+__device__ __forceinline__ R signum1(R x) {  // redi_signum(1,x) / redi_signum1
+  return x == R(0) ? R(0) : copysign(R(1), x);
+}
+template <class R>
+__device__ __forceinline__ R fmin_sel(R a, R b) { return a < b ? a : b; }
+template <class R>
+__device__ __forceinline__ R fmax_sel(R a, R b) { return a > b ? a : b; }
+
+// redi_fv_diff (redi/ocean_redi.F90:135-150)
+template <class R>
+__device__ __forceinline__ R fv_diff(R hkm1, R hk, R hkp1, R skm1, R sk, R skp1) {
+  R h_sum = (hkm1 + hkp1) + hk;
+  if (h_sum != R(0)) h_sum = R(1) / h_sum;
+  R hm = hkm1 + hk;
+  if (hm != R(0)) hm = R(1) / hm;
+  R hp = hkp1 + hk;
+  if (hp != R(0)) hp = R(1) / hp;
+  const R a = ((R(2) * hkm1 + hk) * hp) * (skp1 - sk);
+  const R b = ((R(2) * hkp1 + hk) * hm) * (sk - skm1);
+  return (hk * h_sum) * (a + b);
+}
+
+// one interior entry of redi_plm_diff (redi/ocean_redi.F90:174-187)
+template <class R>
+__device__ __forceinline__ R plm_slope(R hm1, R h0, R hp1, R sm1, R s0, R sp1) {
+  if (!((hp1 + h0) * (hm1 + h0) > R(0))) return R(0);
+  const R dc = fv_diff<R>(hm1, h0, hp1, sm1, s0, sp1);
+  const R dl = R(2) * (s0 - sm1);
+  const R dr = R(2) * (sp1 - s0);
+  if (signum1<R>(dl) * signum1<R>(dr) <= R(0)) return R(0);
+  const R m = fmin_sel<R>(fmin_sel<R>(fabs(dl), fabs(dc)), fabs(dr));
+  return f_sign<R>(m, dc);
+}
+
+// redi_ppm_edge (redi/ocean_redi.F90:194-225)
+template <class R>
+__device__ __forceinline__ R ppm_edge(R hkm1, R hk, R hkp1, R hkp2, R ak, R akp1, R pk, R pkp1) {
+  R r = hk + hkp1;
+  if (r <= R(0)) return R(0.5) * (ak + akp1);
+  r = R(1) / r;
+  R e;
+  if (hk < hkp1)
+    e = ak + (hk * r) * (akp1 - ak);
+  else
+    e = akp1 + (hkp1 * r) * (ak - akp1);
+  const R r2a = R(1) / ((R(2) * hk + hkp1) + R(kHNeglect));
+  const R r2b = R(1) / ((hk + R(2) * hkp1) + R(kHNeglect));
+  const R f1 = R(1) / ((hk + hkp1) + (hkm1 + hkp2));
+  const R f2 = ((R(2) * (hkp1 * hk)) * r) * (((hkm1 + hk) * r2a) - ((hkp2 + hkp1) * r2b));
+  const R f3 = (hk * (hkm1 + hk)) * r2a;
+  const R f4 = (hkp1 * (hkp1 + hkp2)) * r2b;
+  return e + f1 * (f2 * (akp1 - ak) - (f3 * pkp1 - f4 * pk));
+}
+
+// Streaming form of redi_interface_scalar (redi/ocean_redi.F90:231-254) for
+// NS scalars sharing one thickness column.  `load(kt, h, s[])` returns the
+// top-down layer kt (0-based; the caller flips the native bottom-up index);
+// `emit(k, e[], s_above[])` receives interface k = 0..nz (0 = surface) with
+// the edge values and, for k >= 1, the layer-average of layer k-1 above it.
+template <class R, int NS, class Load, class Emit>
+__device__ __forceinline__ void stream_edges(int nz, Load load, Emit emit) {
+  R h0, h1, h2, s0[NS], s1[NS], s2[NS];
+  load(0, h0, s0);
+  load(nz > 1 ? 1 : 0, h1, s1);
+  load(nz > 2 ? 2 : (nz - 1), h2, s2);
+  // window at interface k: hw0..hw3 = h[k-2], h[k-1], h[k], h[k+1] (clamped)
+  R hw0 = h0, hw1 = h0, hw2 = h1, hw3 = h2;
+  R sw0[NS], sw1[NS], sw2[NS], dprev[NS], e[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    sw0[q] = s0[q]; sw1[q] = s1[q]; sw2[q] = s2[q]; dprev[q] = R(0);
+    e[q] = s0[q] - R(0.5) * R(0);  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
+  }
+  emit(0, e, sw0);
+  for (int k = 1; k < nz; ++k) {
+    R dcur[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      dcur[q] = (k <= nz - 2) ? plm_slope<R>(hw1, hw2, hw3, sw0[q], sw1[q], sw2[q]) : R(0);
+      e[q] = ppm_edge<R>(hw0, hw1, hw2, hw3, sw0[q], sw1[q], dprev[q], dcur[q]);
+    }
+    emit(k, e, sw0);
+    R hn, sn[NS];
+    const int kn = (k + 2 < nz) ? (k + 2) : (nz - 1);
+    load(kn, hn, sn);
+    hw0 = hw1; hw1 = hw2; hw2 = hw3; hw3 = hn;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      sw0[q] = sw1[q]; sw1[q] = sw2[q]; sw2[q] = sn[q]; dprev[q] = dcur[q];
+    }
+  }
+  // after the loop sw0 holds layer nz-1 for every nz >= 1
+#pragma unroll
+  for (int q = 0; q < NS; ++q) e[q] = sw0[q] + R(0.5) * R(0);  // diff(nk) = 0
+  emit(nz, e, sw0);
+}
+
+// PPM limiter of redi_tracer_column (redi/ocean_redi.F90:876-890)
+template <class R>
+__device__ __forceinline__ void ppm_limit(R tl, R& al, R& ar) {
+  if (signum1<R>(ar - tl) * signum1<R>(tl - al) <= R(0)) {
+    al = tl;
+    ar = tl;
+  } else if (f_sign<R>(R(3), ar - al) * ((tl - al) + (tl - ar)) > fabs(ar - al)) {
+    al = tl + R(2) * (tl - ar);
+  } else if (f_sign<R>(R(3), ar - al) * ((tl - al) + (tl - ar)) < -fabs(ar - al)) {
+    ar = tl + R(2) * (tl - al);
+  }
+}
+
+// redi_interpolate_position (redi/ocean_redi.F90:260-285)
+template <class R>
+__device__ __forceinline__ R interpolate_position(R dRhoNeg, R Pneg, R dRhoPos, R Ppos) {
+  if ((Ppos > Pneg) && (dRhoPos - dRhoNeg >= R(0))) {
+    if (dRhoPos - dRhoNeg > R(0))
+      return fmin_sel<R>(R(1), fmax_sel<R>(R(0), -dRhoNeg / (dRhoPos - dRhoNeg)));
+    if (dRhoNeg > R(0)) return R(0);
+    if (dRhoNeg < R(0)) return R(1);
+    return R(0.5);
+  }
+  return R(0.5);
+}
+
+// redi_ppm_ave (redi/ocean_redi.F90:833-850)
+template <class R>
+__device__ __forceinline__ R ppm_ave(R xL, R xR, R aL, R aR, R aMean) {
+  const R dx = xR - xL;
+  const R xave = R(0.5) * (xR + xL);
+  const R a6o3 = R(2) * aMean - (aL + aR);
+  const R a6 = R(3) * a6o3;
+  if (dx > R(0) && dx <= R(1))
+    return (aL + xave * ((aR - aL) + a6)) - a6o3 * ((xR * xR + xR * xL) + xL * xL);
+  return (aL + (aR - aL) * xR) + (a6 * xR) * (R(1) - xR);
+}
+
+}  // namespace redi

@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 import struct
 from typing import Optional, Sequence
 
@@ -142,15 +143,18 @@ def _rng(config: int, domain: int, field: int) -> np.random.Generator:
     return np.random.default_rng([int(config), int(domain), int(field)])
 
 
-def _smooth_noise(rng: np.random.Generator, shape, sigma: float) -> np.ndarray:
+_FFT_WORKERS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _smooth_noise(rng: np.random.Generator, shape, sigma: float, workers: int = 1) -> np.ndarray:
     """Unit-std Gaussian-filtered white noise (periodic FFT filter)."""
-    from numpy.fft import rfft2, irfft2
+    from scipy.fft import rfft2, irfft2
     ny, nx = shape
     w = rng.standard_normal((ny, nx))
     ky = np.fft.fftfreq(ny)[:, None]
     kx = np.fft.rfftfreq(nx)[None, :]
     filt = np.exp(-2.0 * (math.pi * sigma) ** 2 * (kx * kx + ky * ky))
-    f = irfft2(rfft2(w) * filt, s=(ny, nx))
+    f = irfft2(rfft2(w, workers=workers) * filt, s=(ny, nx), workers=workers)
     f -= f.mean()
     sd = f.std()
     return f / sd if sd > 0 else f
@@ -158,7 +162,7 @@ def _smooth_noise(rng: np.random.Generator, shape, sigma: float) -> np.ndarray:
 
 def _fractal_field(rng: np.random.Generator, shape) -> np.ndarray:
     """2-D field with power spectrum ~ |k|^-2 (spectral slope -2)."""
-    from numpy.fft import rfft2, irfft2
+    from scipy.fft import rfft2, irfft2
     ny, nx = shape
     w = rng.standard_normal((ny, nx))
     ky = np.fft.fftfreq(ny)[:, None]
@@ -166,7 +170,7 @@ def _fractal_field(rng: np.random.Generator, shape) -> np.ndarray:
     k = np.sqrt(kx * kx + ky * ky)
     amp = np.zeros_like(k)
     amp[k > 0] = 1.0 / k[k > 0]  # |F|^2 ~ k^-2
-    f = irfft2(rfft2(w) * amp, s=(ny, nx))
+    f = irfft2(rfft2(w, workers=_FFT_WORKERS) * amp, s=(ny, nx), workers=_FFT_WORKERS)
     return (f - f.min()) / (f.max() - f.min())
 
 
@@ -188,13 +192,62 @@ def _metrics(nx: int, ny: int, dx: float, dy: float, ocean: np.ndarray, kappa: f
                 wet_u=wet_u, wet_v=wet_v, khtr_u_ext=ku, khtr_v_ext=kv)
 
 
-def _columns(depth: np.ndarray, nz: int, h_ref_td: np.ndarray, total: float):
-    """z*-scaled native (bottom-up) thickness and layer-centre depth fields."""
+def _columns(depth: np.ndarray, h_ref_td: np.ndarray, total: float, vanished: bool = False):
+    """Native (bottom-up) thickness and layer-centre depth fields.
+
+    z*-scaled columns: h_k = h_ref_k * depth / total.  With ``vanished``
+    the columns are z-coordinate instead: layers wholly below the local
+    depth have zero thickness and the layer holding the bed is cut."""
+    nz = h_ref_td.shape[0]
     ny, nx = depth.shape
+    h = np.empty((nz, ny, nx))
+    zc = np.empty((nz, ny, nx))
     scale = depth / total  # 0 on land
-    h_td = h_ref_td[:, None, None] * scale[None]  # (nz, ny, nx) top-down
-    zc_td = (np.cumsum(h_td, axis=0) - 0.5 * h_td)
-    return h_td[::-1].copy(), zc_td[::-1].copy()
+    top = np.zeros((ny, nx))
+    for kt in range(nz):  # top-down
+        if vanished:
+            hk = np.clip(depth - top, 0.0, h_ref_td[kt])
+        else:
+            hk = h_ref_td[kt] * scale
+        k = nz - 1 - kt
+        h[k] = hk
+        zc[k] = top + 0.5 * hk
+        top = top + hk
+    return h, zc
+
+
+def _layer_tracers(cfg: int, seed: int, h: np.ndarray, zc: np.ndarray, sigma: float,
+                   noise_T: float, noise_S: float, fronts2d, nt: int) -> np.ndarray:
+    """Thickness-weighted tracers, one native layer per pool task.
+
+    T = 2 + 23 exp(-z/800) + noise_T xi_T - sum(front) exp(-z/1000),
+    S = 34.5 + 0.8 exp(-z/500) + noise_S xi_S, tracers 3.. are U[0,1];
+    xi are per-layer unit-std smoothed noise fields seeded by
+    (cfg, seed, 100+k) and (cfg, seed, 300+k); U[0,1] by (cfg, seed,
+    10000*t+k)."""
+    from concurrent.futures import ThreadPoolExecutor
+    nz, ny, nx = h.shape
+    hTr = np.empty((nt, nz, ny, nx))
+
+    def one(k):
+        z, hk = zc[k], h[k]
+        xiT = _smooth_noise(_rng(cfg, seed, 100 + k), (ny, nx), sigma)
+        xiS = _smooth_noise(_rng(cfg, seed, 300 + k), (ny, nx), sigma)
+        T = 2.0 + 23.0 * np.exp(-z / 800.0) + noise_T * xiT
+        S = 34.5 + 0.8 * np.exp(-z / 500.0) + noise_S * xiS
+        if fronts2d:
+            e = np.exp(-z / 1000.0)
+            for p in fronts2d:
+                T = T - p * e
+        hTr[0, k] = T * hk
+        if nt > 1:
+            hTr[1, k] = S * hk
+        for t in range(2, nt):
+            hTr[t, k] = _rng(cfg, seed, 10000 * t + k).uniform(0.0, 1.0, size=(ny, nx)) * hk
+
+    with ThreadPoolExecutor(_FFT_WORKERS) as ex:
+        list(ex.map(one, range(nz)))
+    return hTr
 
 
 # --------------------------------------------------------------------------
@@ -228,17 +281,11 @@ def case_cfg0(nx_phys: int = 64, ny_phys: int = 64, nz: int = 25, nt: int = 2,
     ocean = ~(height > 0.6 * D)
     depth = np.where(ocean, D, 0.0)
     h_ref = stretched_thickness(nz, D)
-    h, zc = _columns(depth, nz, h_ref, D)
-    xiT = np.stack([_smooth_noise(_rng(cfg, seed, 100 + k), (ny, nx), 3.0) for k in range(nz)])
-    xiS = np.stack([_smooth_noise(_rng(cfg, seed, 300 + k), (ny, nx), 3.0) for k in range(nz)])
-    T = 2.0 + 23.0 * np.exp(-zc / 800.0) + 0.5 * xiT
-    S = 34.5 + 0.8 * np.exp(-zc / 500.0) + 0.05 * xiS
-    tr = [T * h, S * h]
-    for t in range(2, nt):
-        tr.append(_rng(cfg, seed, 500 + t).uniform(0.0, 1.0, size=h.shape) * h)
+    h, zc = _columns(depth, h_ref, D)
+    hTr = _layer_tracers(cfg, seed, h, zc, 3.0, 0.5, 0.05, [], nt)
     m = _metrics(nx, ny, dx, dx, ocean, 1000.0)
     return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost,
-                    h=h, hTr=np.stack(tr[:nt]), eos_variant=EOS_LINEAR,
+                    h=h, hTr=hTr, eos_variant=EOS_LINEAR,
                     alpha_T=2.0e-4, beta_S=7.6e-4, khtr=1000.0, nsteps=nsteps,
                     name=f"cfg0_{nx_phys}x{ny_phys}x{nz}_s{seed}",
                     land_fraction=float(1.0 - ocean[nghost:-nghost, nghost:-nghost].mean()),
@@ -269,22 +316,10 @@ def case_fractal(cfg: int, nx_phys: int, ny_phys: int, nz: int, nt: int,
     ocean = f > q
     depth = np.where(ocean, np.maximum(50.0, DMAX * (f - q) / (f.max() - q)), 0.0)
     h_ref = stretched_thickness(nz, DMAX)
-    if vanished:
-        # z-coordinate columns: layers fully below the local depth vanish,
-        # the layer containing the bed is cut.
-        top = np.concatenate([[0.0], np.cumsum(h_ref)[:-1]])
-        h_td = np.clip(depth[None] - top[:, None, None], 0.0, h_ref[:, None, None])
-        h_td = np.where(ocean[None], h_td, 0.0)
-        zc_td = np.cumsum(h_td, axis=0) - 0.5 * h_td
-        h, zc = h_td[::-1].copy(), zc_td[::-1].copy()
-    else:
-        h, zc = _columns(depth, nz, h_ref, DMAX)
-    xiT = np.stack([_smooth_noise(_rng(cfg, seed, 100 + k), (ny, nx), noise_sigma) for k in range(nz)])
-    xiS = np.stack([_smooth_noise(_rng(cfg, seed, 300 + k), (ny, nx), noise_sigma) for k in range(nz)])
-    T = 2.0 + 23.0 * np.exp(-zc / 800.0) + noise_T * xiT
-    S = 34.5 + 0.8 * np.exp(-zc / 500.0) + noise_S * xiS
+    h, zc = _columns(depth, h_ref, DMAX, vanished=vanished)
     yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
     rf = _rng(cfg, seed, 2)
+    fronts2d = []
     for axis, pos, amp, width in fronts:
         jit = rf.uniform(-0.05, 0.05)
         if axis == "y":
@@ -293,13 +328,11 @@ def case_fractal(cfg: int, nx_phys: int, ny_phys: int, nz: int, nt: int,
         else:
             c = (pos + jit) * nx * dx
             prof = np.tanh((xx * dx - c) / width)
-        T = T - amp * prof[None] * np.exp(-zc / 1000.0)
-    tr = [T * h, S * h]
-    for t in range(2, nt):
-        tr.append(_rng(cfg, seed, 500 + t).uniform(0.0, 1.0, size=h.shape) * h)
+        fronts2d.append(amp * prof)
+    hTr = _layer_tracers(cfg, seed, h, zc, noise_sigma, noise_T, noise_S, fronts2d, nt)
     m = _metrics(nx, ny, dx, dy, ocean, 1000.0)
     return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost, h=h,
-                    hTr=np.stack(tr[:nt]), eos_variant=eos_variant,
+                    hTr=hTr, eos_variant=eos_variant,
                     alpha_T=2.0e-4, beta_S=7.6e-4, khtr=1000.0, nsteps=nsteps,
                     name=f"cfg{cfg}_{nx_phys}x{ny_phys}x{nz}_s{seed}",
                     land_fraction=float(1.0 - ocean[nghost:-nghost, nghost:-nghost].mean()),

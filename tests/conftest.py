@@ -1,0 +1,90 @@
+"""Shared test setup: markers, import paths and golden-fixture loading.
+
+Tests marked ``gpu`` need an MI355X (HIP device) and call the product
+through the C ABI of libredi_hip.so; everything else runs on the CPU.
+The oracle (``oracle/``) is used here only as the checker.
+"""
+from __future__ import annotations
+
+import glob
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from more_benchmarks_dc_ocean_amd.inputs import RediCase  # noqa: E402
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (HIP device)")
+    config.addinivalue_line("markers", "slow: long-running CPU test")
+
+
+SCALAR_INT = ("nx_phys", "ny_phys", "nz", "nghost", "idx_T", "idx_S", "eos_variant", "nsteps",
+              "use_ext", "enable")
+SCALAR_F = ("dt", "khtr", "rho0", "T_ref", "S_ref", "alpha_T", "beta_S", "p_ref")
+ARRAYS = ("h", "hTr", "dy_cu", "dx_cv", "idxCu", "idyCv", "areaT", "wet_u", "wet_v",
+          "khtr_u_ext", "khtr_v_ext")
+OUTS = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", "vKoR", "vhEff",
+        "hTr1", "hTrN")
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name: str):
+    """(RediCase, reference outputs) of one committed fixture."""
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    kw = {a: np.ascontiguousarray(z[a]) for a in ARRAYS}
+    for s in SCALAR_INT:
+        kw[s] = int(z["scalar_" + s])
+    for s in SCALAR_F:
+        kw[s] = float(z["scalar_" + s])
+    kw["walls"] = tuple(int(w) for w in z["walls"])
+    case = RediCase(name=name, **kw)
+    outs = {k: z["out_" + k] for k in OUTS}
+    return case, outs
+
+
+def input_digest(case) -> str:
+    """SHA-256 of a case's input arrays (as tests/golden/make_golden.py)."""
+    import hashlib
+    h = hashlib.sha256()
+    for a in ARRAYS:
+        h.update(np.ascontiguousarray(getattr(case, a), dtype="<f8").tobytes())
+    return h.hexdigest()
+
+
+def load_digests() -> dict:
+    with open(os.path.join(GOLDEN, "digests.json")) as f:
+        return json.load(f)
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> tuple:
+    """(max per-cell |a-b|/max(|a|,|b|), field-relative max|a-b|/max|b|) --
+    the two error measures of the reference's cross-checks
+    (redi/drivers/dc_main.F90:224-235, redi/drivers/cpp_main.cu:132-140)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    sc = np.maximum(np.abs(a), np.abs(b))
+    per_cell = float(np.max(np.where(sc > 1e-30, d / np.where(sc > 1e-30, sc, 1.0), 0.0))) if d.size else 0.0
+    mb = float(np.max(np.abs(b))) if b.size else 0.0
+    field = float(d.max() / mb) if mb > 0 else float(d.max() if d.size else 0.0)
+    return per_cell, field
+
+
+@pytest.fixture(scope="session")
+def gpu_device():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return "cuda:0"

@@ -1,0 +1,106 @@
+"""Write the golden fixtures from the compiled reference (oracle/_ref/redi_ref).
+
+Run in the build container (where the reference tree exists):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each small case stores its inputs AND the reference outputs in full:
+``<name>.npz`` with the input arrays (h, hTr, metrics, wet masks, face
+diffusivities), the scalar header, the ten Phase-A coefficient arrays after
+step 1, hTr after step 1 and hTr after the last step.  The full-size
+workloads (64x64x25 over 10 steps; 473x297x75 over 1 step) are too large to
+commit, so ``digests.json`` keeps for every output array its SHA-256 over
+the little-endian bytes plus min/max/sum, together with the SHA-256 of the
+generated inputs (which pins the generator).
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from more_benchmarks_dc_ocean_amd import inputs  # noqa: E402
+import redi_oracle  # noqa: E402
+
+SCALARS = ("nx_phys", "ny_phys", "nz", "nghost", "idx_T", "idx_S", "eos_variant", "nsteps",
+           "use_ext", "enable", "dt", "khtr", "rho0", "T_ref", "S_ref", "alpha_T", "beta_S", "p_ref")
+ARRAYS = ("h", "hTr", "dy_cu", "dx_cv", "idxCu", "idyCv", "areaT", "wet_u", "wet_v",
+          "khtr_u_ext", "khtr_v_ext")
+OUTS = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", "vKoR", "vhEff",
+        "hTr1", "hTrN")
+
+
+def small_cases():
+    c = []
+    c.append(dataclasses.replace(inputs.case_cfg0(20, 16, 25, nt=2, nsteps=10, seed=7),
+                                 name="cfg0_mini"))
+    c.append(dataclasses.replace(inputs.case_cfg1(14, 12, 75, nt=4, nsteps=3, seed=3),
+                                 name="roquet_mini"))
+    c.append(dataclasses.replace(inputs.case_cfg2(14, 10, 75, nt=4, nsteps=2, seed=5),
+                                 name="roquet_two_fronts_mini"))
+    w = inputs.case_fractal(9, 18, 14, 30, 3, 3, seed=11, land=0.3,
+                            fronts=[("y", 0.5, 1.0, 60_000.0)], noise_sigma=2.0,
+                            noise_T=0.5, noise_S=0.05, eos_variant=inputs.EOS_WRIGHT97,
+                            vanished=True)
+    c.append(dataclasses.replace(w, name="wright_vanished_mini"))
+    o = inputs.case_cfg0(16, 12, 12, nt=3, nsteps=3, seed=2)
+    c.append(dataclasses.replace(o, walls=(0, 1, 0, 1), use_ext=0, khtr=500.0,
+                                 name="open_scalar_mini"))
+    l1 = inputs.case_cfg0(12, 10, 1, nt=2, nsteps=3, seed=4)
+    c.append(dataclasses.replace(l1, name="nz1_mini"))
+    return c
+
+
+def input_digest(case) -> str:
+    h = hashlib.sha256()
+    for a in ARRAYS:
+        h.update(np.ascontiguousarray(getattr(case, a), dtype="<f8").tobytes())
+    return h.hexdigest()
+
+
+def out_digest(a: np.ndarray) -> dict:
+    b = np.ascontiguousarray(a)
+    return {"sha256": hashlib.sha256(b.tobytes()).hexdigest(), "shape": list(b.shape),
+            "dtype": str(b.dtype), "min": float(b.min()), "max": float(b.max()),
+            "sum": float(b.astype(np.float64).sum())}
+
+
+def main():
+    if not redi_oracle.have_reference():
+        redi_oracle.build_reference()
+    for case in small_cases():
+        out, _ = redi_oracle.run_reference(case, threads=8)
+        payload = {a: getattr(case, a) for a in ARRAYS}
+        payload["walls"] = np.array(case.walls, dtype=np.int32)
+        for s in SCALARS:
+            payload["scalar_" + s] = np.array(getattr(case, s))
+        for k in OUTS:
+            payload["out_" + k] = out[k]
+        path = os.path.join(HERE, case.name + ".npz")
+        np.savez_compressed(path, **payload)
+        print(f"{path}: {os.path.getsize(path) / 1024:.0f} KiB")
+
+    digests = {}
+    for name, case in (("cfg0_64x64x25_10steps", inputs.case_cfg0(nsteps=10)),
+                       ("cfg1_473x297x75_1step", inputs.case_cfg1(nsteps=1))):
+        out, so = redi_oracle.run_reference(case, threads=8)
+        d = {"generator": case.name, "nsteps": case.nsteps, "input_sha256": input_digest(case),
+             "land_fraction": case.land_fraction, "outputs": {k: out_digest(out[k]) for k in OUTS}}
+        digests[name] = d
+        print(name, "done")
+        del out
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,85 @@
+"""The oracle against the reference's own outputs (CPU).
+
+The fixtures under tests/golden/ were written by tests/golden/make_golden.py
+from the reference's Fortran sources compiled unmodified (oracle/_ref).  The
+C restatement must reproduce every output bit for bit: the ten Phase-A
+coefficient arrays after step 1 and every tracer after step 1 and after the
+last step.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import OUTS, golden_names, input_digest, load_digests, load_golden, rel_err
+
+import redi_oracle
+from more_benchmarks_dc_ocean_amd import inputs
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_matches_reference_fixture(name):
+    case, ref = load_golden(name)
+    got = redi_oracle.run_oracle(case)
+    for k in OUTS:
+        assert got[k].dtype == ref[k].dtype, k
+        assert np.array_equal(got[k], ref[k]), (name, k, rel_err(got[k], ref[k]))
+
+
+def test_fixture_set_covers_the_reference_variants():
+    """The committed fixtures span the three EOS variants, nz from 1 to 75,
+    vanished layers, open edges and the scalar-diffusivity path."""
+    seen = {"eos": set(), "nz": set(), "vanished": False, "open": False, "scalar": False}
+    for name in golden_names():
+        c, _ = load_golden(name)
+        seen["eos"].add(c.eos_variant)
+        seen["nz"].add(c.nz)
+        seen["vanished"] |= c.vanished_layers() > 0
+        seen["open"] |= min(c.walls) == 0
+        seen["scalar"] |= c.use_ext == 0
+    assert seen["eos"] == {1, 2, 4}
+    assert 1 in seen["nz"] and 75 in seen["nz"]
+    assert seen["vanished"] and seen["open"] and seen["scalar"]
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_generator_and_oracle_match_full_cfg0_digests():
+    """64x64x25, linear EOS, 2 tracers, 10 steps: generated inputs hash to
+    the digest recorded with the reference run, and every oracle output
+    hashes to the reference's."""
+    d = load_digests()["cfg0_64x64x25_10steps"]
+    case = inputs.case_cfg0(nsteps=10)
+    assert input_digest(case) == d["input_sha256"]
+    got = redi_oracle.run_oracle(case)
+    for k in OUTS:
+        assert _digest(got[k]) == d["outputs"][k]["sha256"], k
+
+
+@pytest.mark.slow
+def test_generator_and_oracle_match_full_cfg1_digests():
+    """473x297x75, Roquet EOS, 4 tracers, 1 step."""
+    d = load_digests()["cfg1_473x297x75_1step"]
+    case = inputs.case_cfg1(nsteps=1)
+    assert input_digest(case) == d["input_sha256"]
+    got = redi_oracle.run_oracle(case)
+    for k in OUTS:
+        assert _digest(got[k]) == d["outputs"][k]["sha256"], k
+
+
+def test_verifier_detects_one_ulp():
+    """Verifying the verifier (the reference's own practice): a single-ulp
+    perturbation of one oracle output cell must fail the comparison."""
+    name = golden_names()[0]
+    case, ref = load_golden(name)
+    got = redi_oracle.run_oracle(case)
+    bad = got["hTrN"].copy()
+    idx = np.unravel_index(np.argmax(np.abs(bad)), bad.shape)
+    bad[idx] = np.nextafter(bad[idx], np.inf)
+    assert not np.array_equal(bad, ref["hTrN"])
+    pc, fr = rel_err(bad, ref["hTrN"])
+    assert 0.0 < pc < 1e-15 and fr > 0.0
