@@ -21,7 +21,7 @@ second), plus
                 a bounded sample of the same workload, or the oracle's C
                 restatement when that binary is absent.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload two_fronts_1440x1080x75]
 """
 from __future__ import annotations
 
@@ -40,9 +40,9 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # name: (builder, description)
-    "cfg2": ("case_cfg2", "redi_1440x1080x75_roquet_4tr"),
-    "cfg1": ("case_cfg1", "redi_473x297x75_roquet_4tr"),
-    "cfg0": ("case_cfg0", "redi_64x64x25_linear_2tr"),
+    "two_fronts_1440x1080x75": ("case_two_fronts", "redi_1440x1080x75_roquet_4tr"),
+    "front_473x297x75": ("case_front", "redi_473x297x75_roquet_4tr"),
+    "seamount_64x64x25": ("case_seamount", "redi_64x64x25_linear_2tr"),
 }
 
 
@@ -138,7 +138,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="two_fronts_1440x1080x75", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

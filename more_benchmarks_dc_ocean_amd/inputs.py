@@ -10,10 +10,10 @@ cells on each side (redi/grid.F90:17-25).
 
 The generators build the domains of the benchmark workloads:
 
-* ``case_cfg0``  64x64x25, linear EOS, 2 tracers, 3 Gaussian seamounts;
-* ``case_cfg1``  473x297x75, Roquet SpV EOS, 4 tracers, fractal bathymetry,
+* ``case_seamount``  64x64x25, linear EOS, 2 tracers, 3 Gaussian seamounts;
+* ``case_front``  473x297x75, Roquet SpV EOS, 4 tracers, fractal bathymetry,
   one tanh front;
-* ``case_cfg2``  1440x1080x75, as cfg1 with two fronts and mesoscale noise;
+* ``case_two_fronts``  1440x1080x75, as case_front with two fronts and mesoscale noise;
 * ``case_domain`` one batched/served domain of arbitrary size.
 
 All randomness comes from ``numpy.random.default_rng`` seeded with
@@ -254,7 +254,7 @@ def _layer_tracers(cfg: int, seed: int, h: np.ndarray, zc: np.ndarray, sigma: fl
 # configurations
 # --------------------------------------------------------------------------
 
-def case_cfg0(nx_phys: int = 64, ny_phys: int = 64, nz: int = 25, nt: int = 2,
+def case_seamount(nx_phys: int = 64, ny_phys: int = 64, nz: int = 25, nt: int = 2,
               nsteps: int = 10, seed: int = 0, nghost: int = 3) -> RediCase:
     """Linear-EOS seamount domain (64x64x25, dx=dy=10 km, 4000 m deep).
 
@@ -287,7 +287,7 @@ def case_cfg0(nx_phys: int = 64, ny_phys: int = 64, nz: int = 25, nt: int = 2,
     return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost,
                     h=h, hTr=hTr, eos_variant=EOS_LINEAR,
                     alpha_T=2.0e-4, beta_S=7.6e-4, khtr=1000.0, nsteps=nsteps,
-                    name=f"cfg0_{nx_phys}x{ny_phys}x{nz}_s{seed}",
+                    name=f"seamount_{nx_phys}x{ny_phys}x{nz}_s{seed}",
                     land_fraction=float(1.0 - ocean[nghost:-nghost, nghost:-nghost].mean()),
                     **m)
 
@@ -298,14 +298,14 @@ def case_fractal(cfg: int, nx_phys: int, ny_phys: int, nz: int, nt: int,
                  eos_variant: int = EOS_ROQUET_SPV, nghost: int = 3,
                  dx: float = 8000.0, dy: float = 11100.0,
                  vanished: bool = False) -> RediCase:
-    """Fractal-bathymetry domain (cfg1/cfg2/batched/served family).
+    """Fractal-bathymetry domain (front, two-front, batched and served workloads).
 
     Bathymetry: seeded 2-D fractal (spectral slope -2) scaled to 0-5500 m,
     the lowest ``land`` fraction of cells is land; ocean depth is floored at
     50 m.  Layers are the 75-layer (or nz-layer) 10 m-top stretched grid over
     5500 m, z*-scaled per column (no vanished layers unless ``vanished``, in
     which case layers below the local depth collapse to zero thickness).
-    T/S as cfg0 with noise (sigma, amplitude) as given plus tanh fronts
+    T/S as case_seamount with noise (sigma, amplitude) as given plus tanh fronts
     ``(axis, position fraction, half-amplitude degC, half-width m)`` decaying
     as exp(-z/1000 m).  Tracers beyond T and S are i.i.d. U[0,1] concentrations.
     """
@@ -334,21 +334,21 @@ def case_fractal(cfg: int, nx_phys: int, ny_phys: int, nz: int, nt: int,
     return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost, h=h,
                     hTr=hTr, eos_variant=eos_variant,
                     alpha_T=2.0e-4, beta_S=7.6e-4, khtr=1000.0, nsteps=nsteps,
-                    name=f"cfg{cfg}_{nx_phys}x{ny_phys}x{nz}_s{seed}",
+                    name=f"fractal{cfg}_{nx_phys}x{ny_phys}x{nz}_s{seed}",
                     land_fraction=float(1.0 - ocean[nghost:-nghost, nghost:-nghost].mean()),
                     **m)
 
 
-def case_cfg1(nx_phys: int = 473, ny_phys: int = 297, nz: int = 75, nt: int = 4,
+def case_front(nx_phys: int = 473, ny_phys: int = 297, nz: int = 75, nt: int = 4,
               nsteps: int = 100, seed: int = 0, **kw) -> RediCase:
     """473x297x75 Roquet domain, ~35% land, one meridional 2 degC front over
-    200 km (tanh half-width 100 km), noise as cfg0."""
+    200 km (tanh half-width 100 km), noise as case_seamount."""
     return case_fractal(1, nx_phys, ny_phys, nz, nt, nsteps, seed, land=0.35,
                         fronts=[("y", 0.5, 1.0, 100_000.0)], noise_sigma=3.0,
                         noise_T=0.5, noise_S=0.05, **kw)
 
 
-def case_cfg2(nx_phys: int = 1440, ny_phys: int = 1080, nz: int = 75, nt: int = 4,
+def case_two_fronts(nx_phys: int = 1440, ny_phys: int = 1080, nz: int = 75, nt: int = 4,
               nsteps: int = 50, seed: int = 0, **kw) -> RediCase:
     """1440x1080x75 Roquet domain, ~30% land, two fronts (meridional at 1/3,
     zonal at 2/3) and mesoscale noise (sigma 5 cells, 0.3 degC)."""
@@ -359,7 +359,7 @@ def case_cfg2(nx_phys: int = 1440, ny_phys: int = 1080, nz: int = 75, nt: int = 
 
 def case_domain(cfg: int, domain: int, side_x: int, side_y: int, nz: int,
                 nt: int = 4, nsteps: int = 1) -> RediCase:
-    """One independent domain of the batched / served workloads (cfg1
+    """One independent domain of the batched / served workloads (case_front
     physics at its own size and seed)."""
     return case_fractal(cfg, side_x, side_y, nz, nt, nsteps, seed=domain, land=0.35,
                         fronts=[("y", 0.5, 1.0, 100_000.0)], noise_sigma=3.0,

@@ -8,9 +8,9 @@ Runs, in one process on cuda:0:
 
 Typical use (each counter in its own run):
     rocprofv3 --kernel-trace --stats -d gpurun_out/prof -o run -- \
-        python -m more_benchmarks_dc_ocean_amd.profiling --workload cfg2 --steps 3
+        python -m more_benchmarks_dc_ocean_amd.profiling --workload two_fronts_1440x1080x75 --steps 3
     rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- \
-        python -m more_benchmarks_dc_ocean_amd.profiling --workload cfg2 --steps 1
+        python -m more_benchmarks_dc_ocean_amd.profiling --workload two_fronts_1440x1080x75 --steps 1
 ``summarize`` turns the CSV outputs into the per-kernel figures kept under
 profiles/.
 """
@@ -39,7 +39,8 @@ def run(workload: str, steps: int, warmup: int, copy_mib: int, form: str = "fp64
     b.copy_(a)
     torch.cuda.synchronize()
     del a, b
-    builder = {"cfg0": inputs.case_cfg0, "cfg1": inputs.case_cfg1, "cfg2": inputs.case_cfg2}[workload]
+    builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
+               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
     case = builder(nsteps=steps)
     print(f"[profiling] {workload} {case.nx}x{case.ny}x{case.nz} nt={case.nt}", flush=True)
     r, dom = make_redi(case, dev)
@@ -84,7 +85,7 @@ def summarize_counters(fetch_glob: str, write_glob: str, copy_mib: int) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--workload", default="two_fronts_1440x1080x75")
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--copy-mib", type=int, default=2048)

@@ -41,21 +41,21 @@ OUTS = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", "vKoR",
 
 def small_cases():
     c = []
-    c.append(dataclasses.replace(inputs.case_cfg0(20, 16, 25, nt=2, nsteps=10, seed=7),
-                                 name="cfg0_mini"))
-    c.append(dataclasses.replace(inputs.case_cfg1(14, 12, 75, nt=4, nsteps=3, seed=3),
+    c.append(dataclasses.replace(inputs.case_seamount(20, 16, 25, nt=2, nsteps=10, seed=7),
+                                 name="seamount_mini"))
+    c.append(dataclasses.replace(inputs.case_front(14, 12, 75, nt=4, nsteps=3, seed=3),
                                  name="roquet_mini"))
-    c.append(dataclasses.replace(inputs.case_cfg2(14, 10, 75, nt=4, nsteps=2, seed=5),
+    c.append(dataclasses.replace(inputs.case_two_fronts(14, 10, 75, nt=4, nsteps=2, seed=5),
                                  name="roquet_two_fronts_mini"))
     w = inputs.case_fractal(9, 18, 14, 30, 3, 3, seed=11, land=0.3,
                             fronts=[("y", 0.5, 1.0, 60_000.0)], noise_sigma=2.0,
                             noise_T=0.5, noise_S=0.05, eos_variant=inputs.EOS_WRIGHT97,
                             vanished=True)
     c.append(dataclasses.replace(w, name="wright_vanished_mini"))
-    o = inputs.case_cfg0(16, 12, 12, nt=3, nsteps=3, seed=2)
+    o = inputs.case_seamount(16, 12, 12, nt=3, nsteps=3, seed=2)
     c.append(dataclasses.replace(o, walls=(0, 1, 0, 1), use_ext=0, khtr=500.0,
                                  name="open_scalar_mini"))
-    l1 = inputs.case_cfg0(12, 10, 1, nt=2, nsteps=3, seed=4)
+    l1 = inputs.case_seamount(12, 10, 1, nt=2, nsteps=3, seed=4)
     c.append(dataclasses.replace(l1, name="nz1_mini"))
     return c
 
@@ -90,8 +90,8 @@ def main():
         print(f"{path}: {os.path.getsize(path) / 1024:.0f} KiB")
 
     digests = {}
-    for name, case in (("cfg0_64x64x25_10steps", inputs.case_cfg0(nsteps=10)),
-                       ("cfg1_473x297x75_1step", inputs.case_cfg1(nsteps=1))):
+    for name, case in (("seamount_64x64x25_10steps", inputs.case_seamount(nsteps=10)),
+                       ("front_473x297x75_1step", inputs.case_front(nsteps=1))):
         out, so = redi_oracle.run_reference(case, threads=8)
         d = {"generator": case.name, "nsteps": case.nsteps, "input_sha256": input_digest(case),
              "land_fraction": case.land_fraction, "outputs": {k: out_digest(out[k]) for k in OUTS}}

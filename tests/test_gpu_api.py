@@ -24,7 +24,7 @@ def _mk(case, device):
 def test_guards_return_skipped_and_change_nothing(gpu_device):
     import torch
     from more_benchmarks_dc_ocean_amd import redi as R
-    case, _ = load_golden("cfg0_mini")
+    case, _ = load_golden("seamount_mini")
     r, dom = _mk(case, gpu_device)
     before = dom.hTr.clone()
     r.configure(enable=False, khtr=case.khtr)  # enable = .false. (:586, :952)
@@ -45,7 +45,7 @@ def test_guards_return_skipped_and_change_nothing(gpu_device):
 
 def test_error_codes(gpu_device):
     from more_benchmarks_dc_ocean_amd import redi as R
-    case, _ = load_golden("cfg0_mini")
+    case, _ = load_golden("seamount_mini")
     with pytest.raises(R.RediError, match="neutral-surface limit"):
         R.OceanRedi(case, nz=R.REDI_NZ_MAX + 1, device=gpu_device)
     r, dom = _mk(case, gpu_device)
@@ -61,7 +61,7 @@ def test_scalar_khtr_equals_uniform_ext(gpu_device):
     """redi_face_const with khtr = k gives the same bits as redi_face_copy of
     a field filled with k."""
     import torch
-    case, _ = load_golden("cfg0_mini")
+    case, _ = load_golden("seamount_mini")
     case = dataclasses.replace(case, khtr=1000.0)
     r1, d1 = _mk(case, gpu_device)
     r2, d2 = _mk(case, gpu_device)
@@ -73,7 +73,7 @@ def test_scalar_khtr_equals_uniform_ext(gpu_device):
 
 def test_bc_absent_means_all_walls(gpu_device):
     import torch
-    case, _ = load_golden("cfg0_mini")
+    case, _ = load_golden("seamount_mini")
     r1, d1 = _mk(case, gpu_device)
     r2, d2 = _mk(case, gpu_device)
     r1.calc_coeffs(d1); r1.apply_flux(d1)  # case walls = all WALL
@@ -87,7 +87,7 @@ def test_flat_reference_launcher_matches(gpu_device):
     2 tracers, linear EOS) against the oracle."""
     import torch
     from more_benchmarks_dc_ocean_amd import redi as R
-    case, _ = load_golden("cfg0_mini")
+    case, _ = load_golden("seamount_mini")
     dom = R.DeviceDomain(case, gpu_device)
     nx, ny, nz, ns = case.nx, case.ny, case.nz, case.ns
     d = torch.device(gpu_device)
@@ -129,7 +129,7 @@ def test_flat_reference_launcher_matches(gpu_device):
             assert np.array_equal(g, a) or rel_err(g, a)[0] <= 1e-13, n
 
 
-def test_full_size_invariants_cfg2(gpu_device):
+def test_full_size_invariants_two_fronts_1440x1080x75(gpu_device):
     """1440x1080x75, Roquet, 4 tracers: two steps at the headline workload's
     size.  Checks the properties that hold at any size: all values finite,
     the total of each tracer over the interior conserved (the walls stop
@@ -137,13 +137,13 @@ def test_full_size_invariants_cfg2(gpu_device):
     tracer left bit-identical, and run-to-run determinism."""
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
-    case = inputs.case_cfg2(nsteps=2)
+    case = inputs.case_two_fronts(nsteps=2)
     ocean = case.h.sum(axis=0) > 0
     case.hTr[3] = 0.5 * case.h  # uniform concentration 0.5 on every wet cell
     g = case.nghost
     area = case.areaT[None]
     tot0 = [(case.hTr[t] * area)[:, g:-g, g:-g].sum() for t in range(case.nt)]
-    print(f"[gpu] cfg2 full {case.nx}x{case.ny}x{case.nz} land={case.land_fraction:.2f}", flush=True)
+    print(f"[gpu] two-front 1440x1080x75 full {case.nx}x{case.ny}x{case.nz} land={case.land_fraction:.2f}", flush=True)
     r, dom = make_redi(case, gpu_device)
     r.step(dom, 2)
     torch.cuda.synchronize()

@@ -74,27 +74,27 @@ def _digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def test_hip_full_cfg0_against_reference_digests(gpu_device):
-    d = load_digests()["cfg0_64x64x25_10steps"]
-    case = inputs.case_cfg0(nsteps=10)
+def test_hip_full_seamount_64x64x25_against_reference_digests(gpu_device):
+    d = load_digests()["seamount_64x64x25_10steps"]
+    case = inputs.case_seamount(nsteps=10)
     got = _run_hip(case, 10, gpu_device)
     orc = redi_oracle.run_oracle(case)
-    _compare(got, orc, "cfg0 64x64x25 x10 vs oracle")
+    _compare(got, orc, "seamount 64x64x25 x10 vs oracle")
     same = [k for k in OUTS if _digest(got[k]) == d["outputs"][k]["sha256"]]
-    print(f"[gpu] cfg0 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
+    print(f"[gpu] seamount 64x64x25 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
 
 
-def test_hip_full_cfg1_one_step(gpu_device):
+def test_hip_full_front_473x297x75_one_step(gpu_device):
     """473x297x75, Roquet, 4 tracers: one full step at the workload's size
     against the oracle run live and the reference's digests."""
-    d = load_digests()["cfg1_473x297x75_1step"]
-    case = inputs.case_cfg1(nsteps=1)
+    d = load_digests()["front_473x297x75_1step"]
+    case = inputs.case_front(nsteps=1)
     got = _run_hip(case, 1, gpu_device)
     redi_oracle.set_threads(16)
     orc = redi_oracle.run_oracle(case, nsteps=1)
-    _compare(got, orc, "cfg1 473x297x75 x1 vs oracle")
+    _compare(got, orc, "front 473x297x75 x1 vs oracle")
     same = [k for k in OUTS if _digest(got[k]) == d["outputs"][k]["sha256"]]
-    print(f"[gpu] cfg1 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
+    print(f"[gpu] front 473x297x75 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
 
 
 @pytest.mark.parametrize("seed,eos,nz,walls,use_ext", [

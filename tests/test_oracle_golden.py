@@ -48,12 +48,12 @@ def _digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def test_generator_and_oracle_match_full_cfg0_digests():
+def test_generator_and_oracle_match_full_seamount_digests():
     """64x64x25, linear EOS, 2 tracers, 10 steps: generated inputs hash to
     the digest recorded with the reference run, and every oracle output
     hashes to the reference's."""
-    d = load_digests()["cfg0_64x64x25_10steps"]
-    case = inputs.case_cfg0(nsteps=10)
+    d = load_digests()["seamount_64x64x25_10steps"]
+    case = inputs.case_seamount(nsteps=10)
     assert input_digest(case) == d["input_sha256"]
     got = redi_oracle.run_oracle(case)
     for k in OUTS:
@@ -61,10 +61,10 @@ def test_generator_and_oracle_match_full_cfg0_digests():
 
 
 @pytest.mark.slow
-def test_generator_and_oracle_match_full_cfg1_digests():
+def test_generator_and_oracle_match_full_front_digests():
     """473x297x75, Roquet EOS, 4 tracers, 1 step."""
-    d = load_digests()["cfg1_473x297x75_1step"]
-    case = inputs.case_cfg1(nsteps=1)
+    d = load_digests()["front_473x297x75_1step"]
+    case = inputs.case_front(nsteps=1)
     assert input_digest(case) == d["input_sha256"]
     got = redi_oracle.run_oracle(case)
     for k in OUTS:

@@ -249,7 +249,7 @@ def test_uniform_tracer_is_bit_constant(small_case):
 
 
 def test_horizontally_uniform_state_has_no_flux():
-    c = inputs.case_cfg0(12, 10, 8, nt=2, seed=1)
+    c = inputs.case_seamount(12, 10, 8, nt=2, seed=1)
     c = c.copy()
     prof_h = c.h[:, c.ny // 2, c.nx // 2].copy()
     ocean = c.h.sum(0) > 0
@@ -278,7 +278,7 @@ def test_zero_scalar_kappa_is_a_noop(small_case):
 
 
 def test_x_mirror_symmetry():
-    c = inputs.case_cfg0(14, 10, 6, nt=2, seed=3)
+    c = inputs.case_seamount(14, 10, 6, nt=2, seed=3)
     m = c.copy()
     m.h = c.h[:, :, ::-1].copy()
     m.hTr = c.hTr[:, :, :, ::-1].copy()
@@ -291,7 +291,7 @@ def test_x_mirror_symmetry():
 
 def test_wall_faces_carry_no_flux():
     """With walls, the ghost ring and the interior never exchange tracer."""
-    c = inputs.case_cfg0(10, 8, 6, nt=2, seed=5)
+    c = inputs.case_seamount(10, 8, 6, nt=2, seed=5)
     g = c.nghost
     c = c.copy()
     c.hTr[0][:, :g, :] *= 3.0  # make the ghost rows very different
