@@ -100,11 +100,9 @@ typedef struct redi_s redi_t;
 
 /* ocean_redi_init (redi/ocean_redi.F90:466-507).  nz_ml < 1 is taken as 1;
  * 2*nz+2 > 2*REDI_NZ_MAX+2 returns REDI_ERR_NZ (the reference's error stop).
- * `bufs` may be NULL (the library allocates the coefficient arrays) or give
- * caller-owned device arrays, which are then filled like the reference's
+ * Allocates the Phase-A state in HBM and fills it like the reference's
  * `allocate(..., source=0 / 1)`.  Synchronous. */
-int redi_init(redi_t **self, const redi_hgrid_t *grid, int nz_ml,
-              const redi_coeffs_t *bufs);
+int redi_init(redi_t **self, const redi_hgrid_t *grid, int nz_ml);
 
 /* ocean_redi_destroy (redi/ocean_redi.F90:509-529).  Synchronous. */
 int redi_destroy(redi_t *self);
@@ -115,8 +113,34 @@ int redi_destroy(redi_t *self);
 int redi_configure(redi_t *self, int enable, double khtr, int continuous,
                    double ref_pres);
 
-/* Device pointers of the coefficient arrays (owned or bound). */
-int redi_get_coeffs(const redi_t *self, redi_coeffs_t *out);
+/* Write the Phase-A state into caller-owned device arrays in the
+ * reference layout (uPoL..vhEff as after the last redi_calc_coeffs, the
+ * reference's init values on faces Phase A never visits; khtr_u/khtr_v as
+ * after the last redi_apply_flux; either khtr pointer may be NULL).  The
+ * library keeps the state as one 32-byte record per face and neutral
+ * surface, so this is a layout conversion enqueued on `stream`. */
+int redi_export_coeffs(const redi_t *self, const redi_coeffs_t *dst,
+                       void *stream);
+
+/* Thread-tile shape (tx * ty lanes: 64, 128, 192 or 256) of the sweep and
+ * flux kernels: tiles of adjacent faces / cells share columns and v-faces
+ * in cache.  Default 16 x 4. */
+int redi_set_tiles(redi_t *self, int tx, int ty);
+
+/* Tracers handled per flux launch (1..4, default 4): more tracers share one
+ * pass over the face records, fewer need fewer registers. */
+int redi_set_tracer_group(redi_t *self, int n);
+
+/* The flux kernel accumulates each cell's layers top-down with one cursor
+ * per face, which needs the layer index of a face's contributing sublayers
+ * to be non-decreasing; cells next to a face where it is not take an
+ * order-general path with the same result.  general != 0 sends every cell
+ * down the general path (for testing). */
+int redi_set_flux_path(redi_t *self, int general);
+
+/* Faces of the last redi_calc_coeffs whose contributing sublayers are not
+ * monotone in layer index (u and v faces).  Synchronous. */
+int redi_nonmonotone_faces(const redi_t *self, long long *n_u, long long *n_v);
 
 /* redi_calc_coeffs (redi/ocean_redi.F90:572-607): Phase A. */
 int redi_calc_coeffs(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
@@ -147,9 +171,11 @@ int redi_step(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
               double dt, const double *khtr_u_ext, const double *khtr_v_ext,
               const redi_bc_t *bc, int nsteps, void *stream);
 
-/* Bytes of device scratch an ocean_redi_t of this size holds beyond the
- * reference's own arrays (the column reconstruction buffers). */
+/* Bytes of device scratch an ocean_redi_t of this size holds for the
+ * column state / reconstruction buffers, and the total device bytes it
+ * holds (scratch plus the Phase-A face state). */
 long long redi_workspace_bytes(int nx_total, int ny_total, int nz);
+long long redi_state_bytes(int nx_total, int ny_total, int nz);
 
 /* ---- per-kernel timing ---------------------------------------------------
  * When enabled, every kernel launch made for this ocean_redi_t is bracketed
@@ -163,6 +189,7 @@ long long redi_workspace_bytes(int nx_total, int ny_total, int nz);
 #define REDI_K_FACE_KHTR 3
 #define REDI_K_TRACER_RECON 4
 #define REDI_K_APPLY_FLUX 5
+#define REDI_K_APPLY_FLUX_GENERAL 6
 typedef struct {
     double ms[REDI_NKERNELS];           /* summed kernel time */
     long long launches[REDI_NKERNELS];  /* launches timed */

@@ -1,23 +1,27 @@
-// redi_kernels_fp64.hpp -- the plain FP64 Redi step as four gfx950 kernels.
+// redi_kernels_fp64.hpp -- the plain FP64 Redi step as gfx950 kernels.
 //
 //   k_column_state   Phase A, once per column: top-down interface pressure,
 //                    PPM T/S edges and dR/dT, dR/dS from the EOS.
 //   k_neutral_sweep  Phase A, once per wet face: the continuous neutral
-//                    surface sweep over the two adjacent columns, writing
-//                    PoL/PoR/KoL/KoR/hEff in the reference layout.
+//                    surface sweep over the two adjacent columns.
 //   k_tracer_recon   Phase B, once per column and tracer: layer averages,
 //                    PPM edges and limited left/right edge values.
-//   k_apply_flux     Phase B, once per cell and tracer: the four bounding
-//                    faces' sublayer fluxes accumulated into the cell's own
-//                    layers (W, E, S, N order), then hTr += dTr / areaT.
+//   k_apply_flux     Phase B, once per cell for up to NT tracers: the four
+//                    bounding faces' sublayer fluxes accumulated into the
+//                    cell's layers in the reference's order, then
+//                    hTr += dTr / areaT.
+//   k_export_coeffs  face records -> the reference's Po/Ko/hEff arrays.
 //
-// Memory layout (all device, FP64 unless noted):
-//   3-D cell fields (h, hTr):           [k][j][i], k native bottom-up
-//   column interface scratch (K1/K3):   [kt][j][i], kt top-down interface
-//   face coefficients (Po/Ko/hEff):     [s][j][i] of the face grid (Ko int32)
-// i.e. exactly the reference's Fortran column-major arrays, so lanes of a
-// wavefront (consecutive i) always touch consecutive addresses for a given
-// layer / interface / surface.
+// Device memory layout:
+//   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
+//   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
+//                               kt = top-down interface 0..nz
+//   tracer recon (K3 -> K4)     per column contiguous: ReconRec[t][col][kt]
+//   face coefficients (K2->K4)  per face contiguous: FaceRec[face][s], s = 0..ns-1,
+//                               plus one monotonicity flag byte per face
+// Per-column / per-face contiguity lets each lane stream its own columns and
+// faces with full use of every 128-byte line it touches, which matters
+// because lanes of one wavefront sit at different depths of their sweeps.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,27 +32,59 @@
 
 namespace redi {
 
+struct __align__(16) ColRec {  // one interface of a column (K1 output)
+  double T, S, dT, dS;         // interface T, S, dR/dT, dR/dS
+};
+
+struct __align__(16) FaceRec {  // one neutral surface s of a face (K2 output)
+  double PoL, PoR;              // positions in the left / right column
+  double hEff;                  // effective thickness of sublayer s (m); 0 for s = ns-1
+  int32_t KoL, KoR;             // 1-based top-down layer indices
+};
+
+struct __align__(16) ReconRec {  // one layer kt of a tracer column (K3 output)
+  double Tl, aL, aR, Ti;         // layer mean, limited top/bottom edges, top interface
+};                               // (record nz holds the bottom interface in Ti)
+
+// Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
+// non-decreasing in s on the left (bit 0) / right (bit 1) column.
+constexpr uint8_t kMonoL = 1, kMonoR = 2;
+
+// Thread -> cell/face mapping over TX x TY tiles of a 2-D index space, with
+// tiles handed to the 8 XCDs in contiguous bands: workgroups are dispatched
+// round-robin over XCDs, so workgroup b runs on XCD b % 8; giving each XCD a
+// contiguous run of tiles keeps vertically adjacent tiles (which share
+// v-faces and neighbour columns) in the same L2.
+struct TileMap {
+  int n0, n1;    // index space extent (fast, slow)
+  int tx, ty;    // tile shape (tx * ty == blockDim.x)
+  int tiles0;    // tiles along the fast axis
+  __device__ __forceinline__ bool map(int& a, int& b) const {
+    const int nb = gridDim.x, bid = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = bid & 7, local = bid >> 3;
+    const int tile = x * q + (x < r ? x : r) + local;
+    const int t0 = tile % tiles0, t1 = tile / tiles0;
+    a = t0 * tx + (int)(threadIdx.x % (unsigned)tx);
+    b = t1 * ty + (int)(threadIdx.x / (unsigned)tx);
+    return a < n0 && b < n1;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // K1: column interface state (redi_build_column, redi/ocean_redi.F90:609-654)
 // ---------------------------------------------------------------------------
-struct ColumnState {
-  double* P;     // interface pressure, [nz+1][N2]
-  double* T;     // interface T
-  double* S;     // interface S
-  double* dRdT;  // -rho^2 dSV/dT
-  double* dRdS;  // -rho^2 dSV/dS
-};
-
-__global__ void __launch_bounds__(256) k_column_state(int N2, int nz, const double* __restrict__ h,
+__global__ void __launch_bounds__(128) k_column_state(int N2, int nz, const double* __restrict__ h,
                                                       const double* __restrict__ thtr,
-                                                      const double* __restrict__ shtr, EosParams eos,
-                                                      ColumnState cs) {
+                                                      const double* __restrict__ shtr,
+                                                      EosParams eos, ColRec* __restrict__ rec,
+                                                      double* __restrict__ Pout) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
   const double g_rho0 = kGravity * eos.rho0;
-  double P = 0.0;     // Pint(1) = 0
-  double h_above = 0.0;
+  ColRec* r = rec + (size_t)c * (nz + 1);
+  double* pc = Pout + (size_t)c * (nz + 1);
+  double P = 0.0;  // Pint(1) = 0
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
     hh = h[o];
@@ -56,201 +92,222 @@ __global__ void __launch_bounds__(256) k_column_state(int N2, int nz, const doub
     s[0] = thtr[o] / he;
     s[1] = shtr[o] / he;
   };
-  // The thickness of the layer above interface k is needed for Pint(k+1);
-  // it is re-read (cached) rather than carried through the edge window.
-  auto emit = [&](int k, const double* e, const double* /*s_above*/) {
-    if (k > 0) {
-      h_above = h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
-      P = P + g_rho0 * h_above;
-    }
+  auto emit = [&](int k, const double* e, const double*) {
+    if (k > 0) P = P + g_rho0 * h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
     double dRdT, dRdS;
     interface_drho<double>(eos, e[0], e[1], P, dRdT, dRdS);
-    const size_t o = (size_t)k * n2 + c;
-    cs.P[o] = P;
-    cs.T[o] = e[0];
-    cs.S[o] = e[1];
-    cs.dRdT[o] = dRdT;
-    cs.dRdS[o] = dRdS;
+    r[k] = ColRec{e[0], e[1], dRdT, dRdS};
+    pc[k] = P;
   };
   stream_edges<double, 2>(nz, load, emit);
 }
 
 // ---------------------------------------------------------------------------
 // K2: continuous neutral-surface sweep
-// (redi_neutral_positions_continuous, redi/ocean_redi.F90:307-464, run by
-//  redi_face_coeffs / redi_calc_coeffs_x/_y, :656-812)
+// (redi_neutral_positions_continuous, redi/ocean_redi.F90:307-464, run for
+//  every wet face by redi_face_coeffs / redi_calc_coeffs_x/_y, :656-812)
 // ---------------------------------------------------------------------------
-struct FaceCoeffs {
-  double* PoL;
-  double* PoR;
-  int32_t* KoL;
-  int32_t* KoR;
-  double* hEff;
-};
 
-struct ColView {  // one column of the K1 scratch; k is the 1-based interface
-  const double *P, *T, *S, *dT, *dS;
-  size_t c, n2;
-  __device__ __forceinline__ size_t at(int k) const { return (size_t)(k - 1) * n2 + c; }
-};
-
-// secant density difference rho(X, a) - rho(Y, b)
-__device__ __forceinline__ double sec_drho(const ColView& X, int a, const ColView& Y, int b) {
-  const size_t xa = X.at(a), yb = Y.at(b);
-  return 0.5 * ((X.dT[xa] + Y.dT[yb]) * (X.T[xa] - Y.T[yb]) +
-                (X.dS[xa] + Y.dS[yb]) * (X.S[xa] - Y.S[yb]));
+// secant density difference rho(a) - rho(b) of two interface records
+// (ocean_redi.F90:344-345, 367-371, 406-410)
+__device__ __forceinline__ double sec_drho(const ColRec& a, const ColRec& b) {
+  return 0.5 * ((a.dT + b.dT) * (a.T - b.T) + (a.dS + b.dS) * (a.S - b.S));
 }
 
-// Face grid: faces (i, j) for i in [i0, i0+nfi), j in [j0, j0+nfj) of an
-// (nfa x nfb) face array; the left column is (i-di, j-dj) and the right
-// column (i, j) of the (nx x ny) cell grid.
-__global__ void __launch_bounds__(256) k_neutral_sweep(
-    int nx, int N2, int nz, int nfa, int nfb, int i0, int j0, int nfi, int nfj, int di, int dj,
-    const double* __restrict__ wet, ColumnState cs, FaceCoeffs fc, double pa_to_h) {
-  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= (long)nfi * nfj) return;
-  const int i = i0 + (int)(f % nfi), j = j0 + (int)(f / nfi);
-  const size_t fo = (size_t)i + (size_t)nfa * j;
-  const size_t fs = (size_t)nfa * nfb;
-  const int nk = nz, ns = 2 * nz + 2;
-  if (!(wet[fo] > 0.0)) {
-    for (int s = 0; s < ns; ++s) {
-      fc.PoL[fo + s * fs] = 0.0;
-      fc.PoR[fo + s * fs] = 0.0;
-      fc.KoL[fo + s * fs] = 1;
-      fc.KoR[fo + s * fs] = 1;
+// Sliding window over one column of the sweep.  `k` is the reference's
+// kl / kr (1-based, 1..nk+1); the window holds the records at m = max(k-1,1)
+// and m+1 and the interface pressures at m, m+1, min(m+2, nk+1), which is
+// every index the sweep reads while the pointer sits at k.
+struct SweepSide {
+  const ColRec* rec;
+  const double* P;
+  int k, m;
+  ColRec w0, w1;
+  double p0, p1, p2;
+  __device__ __forceinline__ void init(const ColRec* r, const double* p, int nk) {
+    rec = r; P = p; k = 1; m = 1;
+    w0 = r[0]; w1 = r[1];
+    p0 = p[0]; p1 = p[1]; p2 = p[nk < 2 ? nk : 2];
+  }
+  __device__ __forceinline__ ColRec cur() const { return k == 1 ? w0 : w1; }  // record at k
+  __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }   // P(k)
+  __device__ __forceinline__ double pk1() const { return k == 1 ? p1 : p2; }  // P(k+1)
+  __device__ __forceinline__ void advance(int nk) {  // k -> k+1, k <= nk
+    if (k >= 2) {
+      m += 1;
+      w0 = w1; w1 = rec[m];
+      p0 = p1; p1 = p2; p2 = P[m + 1 < nk ? m + 1 : nk];
     }
-    for (int s = 0; s < ns - 1; ++s) fc.hEff[fo + s * fs] = 0.0;
+    k += 1;
+  }
+};
+
+struct SweepArgs {
+  int nx, nz;
+  int nfa;                 // faces per row of the face array
+  int i0, j0;              // first computed face
+  int di, dj;              // left column = (i - di, j - dj)
+  const double* wet;
+  const ColRec* rec;
+  const double* P;
+  FaceRec* out;            // [face][s]
+  uint8_t* mono;           // [face]
+  double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
+};
+
+__global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) {
+  int fi, fj;
+  if (!tm.map(fi, fj)) return;
+  const int i = a.i0 + fi, j = a.j0 + fj;
+  const size_t fo = (size_t)i + (size_t)a.nfa * j;
+  const int nk = a.nz, ns = 2 * nk + 2;
+  FaceRec* o = a.out + fo * ns;
+  if (!(a.wet[fo] > 0.0)) {  // dry face: the inert values of :722-730
+    for (int s = 0; s < ns; ++s) o[s] = FaceRec{0.0, 0.0, 0.0, 1, 1};
+    a.mono[fo] = kMonoL | kMonoR;
     return;
   }
-  const size_t n2 = (size_t)N2;
-  const ColView L{cs.P, cs.T, cs.S, cs.dRdT, cs.dRdS, (size_t)(i - di) + (size_t)nx * (j - dj), n2};
-  const ColView Rc{cs.P, cs.T, cs.S, cs.dRdT, cs.dRdS, (size_t)i + (size_t)nx * j, n2};
+  const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
+  const size_t cr = (size_t)i + (size_t)a.nx * j;
+  SweepSide L, R;
+  L.init(a.rec + cl * (nk + 1), a.P + cl * (nk + 1), nk);
+  R.init(a.rec + cr * (nk + 1), a.P + cr * (nk + 1), nk);
 
-  int kr = 1, kl = 1, lastK_r = 1, lastK_l = 1;
-  double lastP_r = 0.0, lastP_l = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;
-  bool reached_bottom = false, search_l = false;
+  int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
+  double lastP_l = 0.0, lastP_r = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;
+  bool reached_bottom = false, search_l = false, mono_l = true, mono_r = true;
+  FaceRec pend{};
   for (int ks = 1; ks <= ns; ++ks) {
-    int klm1 = kl - 1 > 1 ? kl - 1 : 1;
-    int krm1 = kr - 1 > 1 ? kr - 1 : 1;
     if (!reached_bottom) {
-      const double dRho = sec_drho(Rc, kr, L, kl);
+      const double dRho = sec_drho(R.cur(), L.cur());
       if (dRho < 0.0)
         search_l = true;
       else if (dRho > 0.0)
         search_l = false;
-      else if (kl + kr == 2)
+      else if (L.k + R.k == 2)
         search_l = true;
       else
         search_l = !search_l;
     }
-    double PoL, PoR;
+    double PoL, PoR, posL, posR;
     int KoL, KoR;
     if (search_l) {
-      const double top = sec_drho(L, klm1, Rc, kr);
-      const double bot = sec_drho(L, klm1 + 1, Rc, kr);
-      if (top > 0.0 || kr + kl == 2)
-        PoL = 0.0;
+      const ColRec cr_ = R.cur();
+      const double top = sec_drho(L.w0, cr_);
+      const double bot = sec_drho(L.w1, cr_);
+      double Po;
+      if (top > 0.0 || R.k + L.k == 2)
+        Po = 0.0;
       else if (top >= bot)
-        PoL = 1.0;
+        Po = 1.0;
       else
-        PoL = interpolate_position<double>(top, L.P[L.at(klm1)], bot, L.P[L.at(klm1 + 1)]);
-      if (PoL >= 1.0 && klm1 < nk) {
-        klm1 += 1;
-        PoL = PoL - 1.0;
+        Po = interpolate_position<double>(top, L.p0, bot, L.p1);
+      int K = L.m;
+      double pa = L.p0, pb = L.p1;
+      if (Po >= 1.0 && K < nk) {
+        K += 1;
+        Po = Po - 1.0;
+        pa = L.p1;
+        pb = L.p2;
       }
-      if ((double)(klm1 - lastK_l) + (PoL - lastP_l) < 0.0) {
-        PoL = lastP_l;
-        klm1 = lastK_l;
+      if ((double)(K - lastK_l) + (Po - lastP_l) < 0.0) {  // monotonic backstop
+        PoL = lastP_l; KoL = lastK_l; posL = lastPos_l;
+      } else {
+        PoL = Po; KoL = K; posL = pa + Po * (pb - pa);
       }
-      KoL = klm1;
-      if (kr <= nk) {
-        PoR = 0.0; KoR = kr; kr += 1;
+      if (R.k <= nk) {
+        PoR = 0.0; KoR = R.k;
+        const double q = R.pk();
+        posR = q + 0.0 * (R.pk1() - q);
+        R.advance(nk);
       } else {
         PoR = 1.0; KoR = nk;
+        posR = R.p0 + 1.0 * (R.p1 - R.p0);
         reached_bottom = true; search_l = false;
       }
     } else {
-      const double top = sec_drho(Rc, krm1, L, kl);
-      const double bot = sec_drho(Rc, krm1 + 1, L, kl);
-      if (top >= 0.0 || kr + kl == 2)
-        PoR = 0.0;
+      const ColRec cl_ = L.cur();
+      const double top = sec_drho(R.w0, cl_);
+      const double bot = sec_drho(R.w1, cl_);
+      double Po;
+      if (top >= 0.0 || R.k + L.k == 2)
+        Po = 0.0;
       else if (top >= bot)
-        PoR = 1.0;
+        Po = 1.0;
       else
-        PoR = interpolate_position<double>(top, Rc.P[Rc.at(krm1)], bot, Rc.P[Rc.at(krm1 + 1)]);
-      if (PoR >= 1.0 && krm1 < nk) {
-        krm1 += 1;
-        PoR = PoR - 1.0;
+        Po = interpolate_position<double>(top, R.p0, bot, R.p1);
+      int K = R.m;
+      double pa = R.p0, pb = R.p1;
+      if (Po >= 1.0 && K < nk) {
+        K += 1;
+        Po = Po - 1.0;
+        pa = R.p1;
+        pb = R.p2;
       }
-      if ((double)(krm1 - lastK_r) + (PoR - lastP_r) < 0.0) {
-        PoR = lastP_r;
-        krm1 = lastK_r;
+      if ((double)(K - lastK_r) + (Po - lastP_r) < 0.0) {
+        PoR = lastP_r; KoR = lastK_r; posR = lastPos_r;
+      } else {
+        PoR = Po; KoR = K; posR = pa + Po * (pb - pa);
       }
-      KoR = krm1;
-      if (kl <= nk) {
-        PoL = 0.0; KoL = kl; kl += 1;
+      if (L.k <= nk) {
+        PoL = 0.0; KoL = L.k;
+        const double q = L.pk();
+        posL = q + 0.0 * (L.pk1() - q);
+        L.advance(nk);
       } else {
         PoL = 1.0; KoL = nk;
+        posL = L.p0 + 1.0 * (L.p1 - L.p0);
         reached_bottom = true; search_l = true;
       }
     }
     lastK_l = KoL; lastP_l = PoL; lastK_r = KoR; lastP_r = PoR;
-    const size_t so = fo + (size_t)(ks - 1) * fs;
-    fc.PoL[so] = PoL;
-    fc.PoR[so] = PoR;
-    fc.KoL[so] = KoL;
-    fc.KoR[so] = KoR;
-    // redi_absolute_position (:289-298) of this surface in both columns
-    const double pl0 = L.P[L.at(KoL)], pr0 = Rc.P[Rc.at(KoR)];
-    const double posL = pl0 + PoL * (L.P[L.at(KoL + 1)] - pl0);
-    const double posR = pr0 + PoR * (Rc.P[Rc.at(KoR + 1)] - pr0);
-    if (ks > 1) {
+    if (ks > 1) {  // sublayer between surfaces ks-1 and ks (:452-462)
       const double hL = posL - lastPos_l;
       const double hR = posR - lastPos_r;
       double he = 0.0;
       if (hL + hR > 0.0) he = ((2.0 * hL) * hR) / (hL + hR);
-      fc.hEff[fo + (size_t)(ks - 2) * fs] = he * pa_to_h;  // Pa -> m (:688-691)
+      pend.hEff = he * a.pa_to_h;
+      if (pend.hEff != 0.0) {
+        mono_l = mono_l && pend.KoL >= contK_l;
+        mono_r = mono_r && pend.KoR >= contK_r;
+        contK_l = pend.KoL;
+        contK_r = pend.KoR;
+      }
+      o[ks - 2] = pend;
     }
+    pend = FaceRec{PoL, PoR, 0.0, KoL, KoR};
     lastPos_l = posL;
     lastPos_r = posR;
   }
+  o[ns - 1] = pend;
+  a.mono[fo] = (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0));
 }
 
 // ---------------------------------------------------------------------------
 // K3: tracer column reconstruction (redi_tracer_column, :852-891)
 // ---------------------------------------------------------------------------
-struct TracerRecon {
-  double* Tl;  // layer average, [nz][N2] top-down
-  double* Ti;  // interface edge, [nz+1][N2]
-  double* aL;  // limited upper edge, [nz][N2]
-  double* aR;  // limited lower edge, [nz][N2]
-};
-
-__global__ void __launch_bounds__(256) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
-                                                      const double* __restrict__ htr,
-                                                      TracerRecon tr) {
+__global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
+                                                      const double* __restrict__ q,
+                                                      ReconRec* __restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
+  ReconRec* r = out + (size_t)c * (nz + 1);
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
     hh = h[o];
     const double he = hh > kHDivEps ? hh : kHDivEps;
-    s[0] = htr[o] / he;
+    s[0] = q[o] / he;
   };
   double e_prev = 0.0;
   auto emit = [&](int k, const double* e, const double* s_above) {
-    tr.Ti[(size_t)k * n2 + c] = e[0];
     if (k > 0) {
       const double tl = s_above[0];
       double al = e_prev, ar = e[0];
       ppm_limit<double>(tl, al, ar);
-      const size_t o = (size_t)(k - 1) * n2 + c;
-      tr.Tl[o] = tl;
-      tr.aL[o] = al;
-      tr.aR[o] = ar;
+      r[k - 1] = ReconRec{tl, al, ar, e_prev};
     }
+    if (k == nz) r[nz] = ReconRec{0.0, 0.0, 0.0, e[0]};
     e_prev = e[0];
   };
   stream_edges<double, 1>(nz, load, emit);
@@ -260,32 +317,18 @@ __global__ void __launch_bounds__(256) k_tracer_recon(int N2, int nz, const doub
 // K4: sublayer fluxes and divergence (redi_apply_flux_impl + redi_face_flux
 // + redi_sublayer_dT, :893-925, :1037-1181)
 // ---------------------------------------------------------------------------
-struct FluxArgs {
-  int nx, ny, nz;
-  double dt;
-  int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
-  int wall_w, wall_e, wall_s, wall_n;
-  const double *khtr_u, *khtr_v, *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT;
-  FaceCoeffs u, v;
-};
 
-struct ReconView {
-  const double *Tl, *Ti, *aL, *aR;
-  size_t c, n2;
-};
-
-// redi_sublayer_dT with 1-based top-down layer indices
+// redi_sublayer_dT (1-based top-down layer indices) on recon records
 __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
                                               double PoLb, double PoRt, double PoRb,
-                                              const ReconView& L, const ReconView& R) {
-  const size_t n2 = L.n2;
-  const double tlt = (1.0 - PoLt) * L.Ti[(size_t)(klt - 1) * n2 + L.c] + PoLt * L.Ti[(size_t)klt * n2 + L.c];
-  const double tlb = (1.0 - PoLb) * L.Ti[(size_t)(klb - 1) * n2 + L.c] + PoLb * L.Ti[(size_t)klb * n2 + L.c];
-  const double trt = (1.0 - PoRt) * R.Ti[(size_t)(krt - 1) * n2 + R.c] + PoRt * R.Ti[(size_t)krt * n2 + R.c];
-  const double trb = (1.0 - PoRb) * R.Ti[(size_t)(krb - 1) * n2 + R.c] + PoRb * R.Ti[(size_t)krb * n2 + R.c];
-  const size_t ol = (size_t)(klt - 1) * n2 + L.c, orr = (size_t)(krt - 1) * n2 + R.c;
-  const double tlay = ppm_ave<double>(PoLt, PoLb + (double)(klb - klt), L.aL[ol], L.aR[ol], L.Tl[ol]);
-  const double trlay = ppm_ave<double>(PoRt, PoRb + (double)(krb - krt), R.aL[orr], R.aR[orr], R.Tl[orr]);
+                                              const ReconRec* L, const ReconRec* R) {
+  const ReconRec lt = L[klt - 1], rt = R[krt - 1];
+  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * L[klt].Ti;
+  const double tlb = (1.0 - PoLb) * L[klb - 1].Ti + PoLb * L[klb].Ti;
+  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * R[krt].Ti;
+  const double trb = (1.0 - PoRb) * R[krb - 1].Ti + PoRb * R[krb].Ti;
+  const double tlay = ppm_ave<double>(PoLt, PoLb + (double)(klb - klt), lt.aL, lt.aR, lt.Tl);
+  const double trlay = ppm_ave<double>(PoRt, PoRb + (double)(krb - krt), rt.aL, rt.aR, rt.Tl);
   const double dT_top = trt - tlt;
   const double dT_bot = trb - tlb;
   const double dT_ave = 0.5 * (dT_top + dT_bot);
@@ -296,106 +339,222 @@ __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb
   return dT_layer;
 }
 
-// One face's ns-1 sublayers into the LDS accumulator of this cell.
-__device__ __forceinline__ void face_flux(const FaceCoeffs& fc, size_t fo, size_t fs, int nz,
-                                          const ReconView& L, const ReconView& R, double coef,
-                                          bool centre_is_left, double* acc, int stride) {
-  const int ns = 2 * nz + 2;
-  for (int s = 0; s < ns - 1; ++s) {
-    const size_t a = fo + (size_t)s * fs;
-    const double he = fc.hEff[a];
-    if (he != 0.0) {
-      const size_t b = a + fs;
-      const int kla = fc.KoL[a], kra = fc.KoR[a];
-      const double d = sublayer_dT(kla, fc.KoL[b], kra, fc.KoR[b], fc.PoL[a], fc.PoL[b], fc.PoR[a],
-                                   fc.PoR[b], L, R);
-      const double flx = (d * he) * coef;
-      if (centre_is_left) {
-        double* p = acc + (size_t)(nz - kla) * stride;
-        *p = *p + flx;
-      } else {
-        double* p = acc + (size_t)(nz - kra) * stride;
-        *p = *p - flx;
+constexpr int kMaxNT = 4;
+
+struct FluxArgs {
+  int nx, ny, nz;
+  double dt;
+  int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
+  int wall_w, wall_e, wall_s, wall_n;
+  const double *khtr_u, *khtr_v, *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT;
+  const FaceRec *u, *v;
+  const uint8_t *mono_u, *mono_v;
+  int nt;                          // tracers in this launch (<= NT)
+  int force_general;               // take the order-general path for every cell
+  size_t tstride;                  // records between consecutive tracers' recon
+  const ReconRec* recon[kMaxNT];   // per tracer, [col][kt]
+  double* htr[kMaxNT];
+};
+
+struct FaceRef {
+  const FaceRec* rec;  // this face's ns records
+  const ReconRec* L0;  // left column recon of tracer 0 (tracer t: + t * tstride)
+  const ReconRec* R0;
+  double coef;
+  bool left;           // this cell is the face's left (west/south) column
+};
+
+// Accumulate the contributing sublayers of one face whose target layer is K,
+// starting at sublayer s and stopping at the first contributing sublayer
+// with another target; returns the new s.
+template <int NT>
+__device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, size_t tstride,
+                                        double* acc) {
+  while (s < ns - 1) {
+    const FaceRec top = f.rec[s];
+    if (top.hEff != 0.0) {
+      if ((f.left ? top.KoL : top.KoR) != K) break;
+      const FaceRec bot = f.rec[s + 1];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
+                                     bot.PoR, f.L0 + t * tstride, f.R0 + t * tstride);
+        const double flx = (d * top.hEff) * f.coef;
+        acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
       }
+    }
+    ++s;
+  }
+  return s;
+}
+
+// The same accumulation without the monotonicity assumption: for layer K,
+// scan every sublayer of the face and take those whose target is K.
+template <int NT>
+__device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, size_t tstride,
+                                          double* acc) {
+  for (int s = 0; s < ns - 1; ++s) {
+    const FaceRec top = f.rec[s];
+    if (top.hEff == 0.0 || (f.left ? top.KoL : top.KoR) != K) continue;
+    const FaceRec bot = f.rec[s + 1];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
+                                   bot.PoR, f.L0 + t * tstride, f.R0 + t * tstride);
+      const double flx = (d * top.hEff) * f.coef;
+      acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
     }
   }
 }
 
-// blockDim.x threads, dynamic LDS = nz * blockDim.x doubles (dTr per thread,
-// layer-major so simultaneous accesses to one layer hit distinct banks).
-__global__ void __launch_bounds__(64) k_apply_flux(FluxArgs a, TracerRecon rc,
-                                                   const double* __restrict__ /*h*/,
-                                                   double* __restrict__ htr) {
-  extern __shared__ double lds[];
-  const int N2 = a.nx * a.ny;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int stride = blockDim.x;
-  double* acc = lds + threadIdx.x;
-  const int nz = a.nz;
-  for (int k = 0; k < nz; ++k) acc[(size_t)k * stride] = 0.0;
-  if (c >= N2) return;
-  const int i = c % a.nx + 1, j = c / a.nx + 1;  // 1-based
-  const size_t n2 = (size_t)N2;
-  const ReconView C{rc.Tl, rc.Ti, rc.aL, rc.aR, (size_t)c, n2};
-  const size_t ufs = (size_t)(a.nx + 1) * a.ny, vfs = (size_t)a.nx * (a.ny + 1);
+// The four bounding faces of cell (ci, cj) (0-based): their records, the
+// left/right recon columns, Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
+// and which of them are open (:1136-1175).  Returns whether every open
+// face's contributing sublayers are monotone on this cell's side.
+__device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, FaceRef* F,
+                                           bool* act) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz, ns = 2 * nz + 2;
+  const size_t c = (size_t)ci + (size_t)nx * cj;
+  const int i = ci + 1, j = cj + 1;  // 1-based
+  const ReconRec* C0 = a.recon[0] + c * (nz + 1);
+  bool mono = true;
   // WEST u-face (face i): this cell is the right column
-  if (i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e))) {
-    const size_t fo = (size_t)(i - 1) + (size_t)(a.nx + 1) * (j - 1);
-    const ReconView N{rc.Tl, rc.Ti, rc.aL, rc.aR, (size_t)c - 1, n2};
-    const double coef = ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo];
-    face_flux(a.u, fo, ufs, nz, N, C, coef, false, acc, stride);
+  act[0] = i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e));
+  if (act[0]) {
+    const size_t fo = (size_t)(i - 1) + (size_t)(nx + 1) * (j - 1);
+    F[0] = FaceRef{a.u + fo * ns, C0 - (nz + 1), C0,
+                   ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], false};
+    mono = mono && (a.mono_u[fo] & kMonoR);
   }
   // EAST u-face (face i+1): this cell is the left column
-  if (i <= a.nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e))) {
-    const size_t fo = (size_t)i + (size_t)(a.nx + 1) * (j - 1);
-    const ReconView N{rc.Tl, rc.Ti, rc.aL, rc.aR, (size_t)c + 1, n2};
-    const double coef = ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo];
-    face_flux(a.u, fo, ufs, nz, C, N, coef, true, acc, stride);
+  act[1] = i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e));
+  if (act[1]) {
+    const size_t fo = (size_t)i + (size_t)(nx + 1) * (j - 1);
+    F[1] = FaceRef{a.u + fo * ns, C0, C0 + (nz + 1),
+                   ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], true};
+    mono = mono && (a.mono_u[fo] & kMonoL);
   }
   // SOUTH v-face (face j): this cell is the right (north) column
-  if (j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n))) {
-    const size_t fo = (size_t)(i - 1) + (size_t)a.nx * (j - 1);
-    const ReconView N{rc.Tl, rc.Ti, rc.aL, rc.aR, (size_t)c - a.nx, n2};
-    const double coef = ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo];
-    face_flux(a.v, fo, vfs, nz, N, C, coef, false, acc, stride);
+  act[2] = j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n));
+  if (act[2]) {
+    const size_t fo = (size_t)(i - 1) + (size_t)nx * (j - 1);
+    F[2] = FaceRef{a.v + fo * ns, C0 - (size_t)nx * (nz + 1), C0,
+                   ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], false};
+    mono = mono && (a.mono_v[fo] & kMonoR);
   }
   // NORTH v-face (face j+1): this cell is the left (south) column
-  if (j <= a.ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n))) {
-    const size_t fo = (size_t)(i - 1) + (size_t)a.nx * j;
-    const ReconView N{rc.Tl, rc.Ti, rc.aL, rc.aR, (size_t)c + a.nx, n2};
-    const double coef = ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo];
-    face_flux(a.v, fo, vfs, nz, C, N, coef, true, acc, stride);
+  act[3] = j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n));
+  if (act[3]) {
+    const size_t fo = (size_t)(i - 1) + (size_t)nx * j;
+    F[3] = FaceRef{a.v + fo * ns, C0, C0 + (size_t)nx * (nz + 1),
+                   ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], true};
+    mono = mono && (a.mono_v[fo] & kMonoL);
   }
+  return mono;
+}
+
+// Fast path: cells whose open faces are all monotone.  Layers are visited
+// top-down (K = 1..nz); at each K every face contributes its run of
+// sublayers targeting K, in W, E, S, N order, which is exactly the order in
+// which the reference adds them into dTr(nz+1-K); dTr lives in registers.
+template <int NT>
+__global__ void __launch_bounds__(256) k_apply_flux(FluxArgs a, TileMap tm) {
+  int ci, cj;
+  if (!tm.map(ci, cj)) return;
+  FaceRef F[4];
+  bool act[4];
+  const bool mono = cell_faces(a, ci, cj, F, act);
+  if (!mono || a.force_general) return;  // k_apply_flux_general's cell
+  const int nz = a.nz, ns = 2 * nz + 2;
+  const size_t n2 = (size_t)a.nx * a.ny;
+  const size_t c = (size_t)ci + (size_t)a.nx * cj;
   const double iaij = 1.0 / a.areaT[c];
-  for (int k = 0; k < nz; ++k) {
-    const size_t o = (size_t)k * n2 + c;
-    htr[o] = htr[o] + acc[(size_t)k * stride] * iaij;
+  const size_t tstride = a.tstride;
+  int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (int K = 1; K <= nz; ++K) {
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+    if (act[0]) s0 = face_run<NT>(F[0], s0, ns, K, tstride, acc);
+    if (act[1]) s1 = face_run<NT>(F[1], s1, ns, K, tstride, acc);
+    if (act[2]) s2 = face_run<NT>(F[2], s2, ns, K, tstride, acc);
+    if (act[3]) s3 = face_run<NT>(F[3], s3, ns, K, tstride, acc);
+    const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
   }
 }
 
-// khtr_u/v <- khtr_*_ext (redi_face_copy, :1015-1024) or the scalar khtr
-// (redi_face_const, :1026-1035); one launch covers both face grids.
+// Order-general path for the remaining cells: for every layer K, scan all
+// sublayers of faces W, E, S, N in order and take those targeting K.
+template <int NT>
+__global__ void __launch_bounds__(256) k_apply_flux_general(FluxArgs a, TileMap tm) {
+  int ci, cj;
+  if (!tm.map(ci, cj)) return;
+  FaceRef F[4];
+  bool act[4];
+  const bool mono = cell_faces(a, ci, cj, F, act);
+  if (mono && !a.force_general) return;  // done by k_apply_flux
+  const int nz = a.nz, ns = 2 * nz + 2;
+  const size_t n2 = (size_t)a.nx * a.ny;
+  const size_t c = (size_t)ci + (size_t)a.nx * cj;
+  const double iaij = 1.0 / a.areaT[c];
+  for (int K = 1; K <= nz; ++K) {
+    double acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+    if (act[0]) face_scan<NT>(F[0], ns, K, a.tstride, acc);
+    if (act[1]) face_scan<NT>(F[1], ns, K, a.tstride, acc);
+    if (act[2]) face_scan<NT>(F[2], ns, K, a.tstride, acc);
+    if (act[3]) face_scan<NT>(F[3], ns, K, a.tstride, acc);
+    const size_t o = (size_t)(nz - K) * n2 + c;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Face records <-> reference arrays, khtr faces, initial fill
+// ---------------------------------------------------------------------------
+struct CoeffArrays {  // the reference layout: [s][face] (Fortran (nfa,nfb,ns))
+  double *PoL, *PoR, *hEff;
+  int32_t *KoL, *KoR;
+};
+
+// Faces (i, j), i in [i0, i1), j in [j0, j1) of an nfa-wide face array.
+__global__ void k_export_coeffs(const FaceRec* __restrict__ rec, int nfa, int nfb, int ns, int i0,
+                                int i1, int j0, int j1, CoeffArrays out) {
+  const size_t nf = (size_t)nfa * nfb;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf * ns) return;
+  const size_t f = t % nf, s = t / nf;  // writes coalesced along the face index
+  const int i = (int)(f % nfa), j = (int)(f / nfa);
+  if (i < i0 || i >= i1 || j < j0 || j >= j1) return;
+  const FaceRec r = rec[f * ns + s];
+  out.PoL[t] = r.PoL;
+  out.PoR[t] = r.PoR;
+  out.KoL[t] = r.KoL;
+  out.KoR[t] = r.KoR;
+  if ((int)s < ns - 1) out.hEff[t] = r.hEff;
+}
+
 __global__ void k_face_khtr(int nu, int nv, const double* __restrict__ ext_u,
                             const double* __restrict__ ext_v, double khtr, double* __restrict__ ku,
                             double* __restrict__ kv) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nu) {
-    ku[t] = ext_u ? ext_u[t] : khtr;
+    ku[t] = ext_u ? ext_u[t] : khtr;  // redi_face_copy / redi_face_const (:1015-1035)
   } else if (t < nu + nv) {
     kv[t - nu] = ext_v ? ext_v[t - nu] : khtr;
   }
 }
 
 // ocean_redi_init fill (:493-502): Po = hEff = 0, Ko = 1
-__global__ void k_fill_coeffs(size_t n_po, size_t n_he, FaceCoeffs fc) {
+__global__ void k_fill_faces(size_t nrec, FaceRec* __restrict__ rec, size_t nface,
+                             uint8_t* __restrict__ mono) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n_po) {
-    fc.PoL[t] = 0.0;
-    fc.PoR[t] = 0.0;
-    fc.KoL[t] = 1;
-    fc.KoR[t] = 1;
-  }
-  if (t < n_he) fc.hEff[t] = 0.0;
+  if (t < nrec) rec[t] = FaceRec{0.0, 0.0, 0.0, 1, 1};
+  if (t < nface) mono[t] = kMonoL | kMonoR;
 }
 
 }  // namespace redi

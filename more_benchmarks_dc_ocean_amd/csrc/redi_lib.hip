@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -56,7 +57,6 @@ struct Timer {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
   }
-  // Records the start event; returns the slot to close with end().
   int begin(int kind, hipStream_t st) {
     if (!on) return -1;
     Rec r{kind, get(), get()};
@@ -74,47 +74,103 @@ struct Timer {
 };
 Timer g_no_timer;
 
-// Column scratch of one step: Phase A holds 5 interface arrays, Phase B 4
-// reconstruction arrays (they are live at different times and share it).
-size_t workspace_elems(int nx, int ny, int nz) { return (size_t)5 * (nz + 1) * nx * ny; }
+constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 
-struct Workspace {
-  double* base = nullptr;
-  size_t elems = 0;
-  ColumnState column(int N2, int nz) const {
-    const size_t s = (size_t)(nz + 1) * N2;
-    return ColumnState{base, base + s, base + 2 * s, base + 3 * s, base + 4 * s};
-  }
-  TracerRecon recon(int N2, int nz) const {
-    const size_t s = (size_t)(nz + 1) * N2;
-    return TracerRecon{base, base + s, base + 2 * s, base + 3 * s};
-  }
+// Device scratch of one step: Phase A holds ColRec + P per column interface
+// (40 B), Phase B one ReconRec (32 B) per interface and tracer of a group;
+// they are live at different times and share one buffer.
+size_t workspace_bytes(int nx, int ny, int nz, int nt) {
+  const size_t per = std::max<size_t>(40, (size_t)32 * std::min(std::max(nt, 1), kTracerGroup));
+  return per * (size_t)(nz + 1) * nx * ny;
+}
+
+// The per-face Phase-A state: FaceRec[face][s] for u and v faces plus the
+// monotonicity flags and the resolved face diffusivities.
+struct FaceState {
+  FaceRec* u = nullptr;
+  FaceRec* v = nullptr;
+  uint8_t* mono_u = nullptr;
+  uint8_t* mono_v = nullptr;
+  double* khtr_u = nullptr;
+  double* khtr_v = nullptr;
+  size_t nu = 0, nv = 0;  // faces
 };
+
+size_t face_state_bytes(int nx, int ny, int nz) {
+  const size_t ns = 2 * (size_t)nz + 2, nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  return (nu + nv) * (ns * sizeof(FaceRec) + 1 + sizeof(double));
+}
+
+int alloc_face_state(FaceState* fs, int nx, int ny, int nz, bool own_khtr) {
+  const size_t ns = 2 * (size_t)nz + 2;
+  fs->nu = (size_t)(nx + 1) * ny;
+  fs->nv = (size_t)nx * (ny + 1);
+  HIP_TRY(hipMalloc(&fs->u, fs->nu * ns * sizeof(FaceRec)));
+  HIP_TRY(hipMalloc(&fs->v, fs->nv * ns * sizeof(FaceRec)));
+  HIP_TRY(hipMalloc(&fs->mono_u, fs->nu));
+  HIP_TRY(hipMalloc(&fs->mono_v, fs->nv));
+  if (own_khtr) {
+    HIP_TRY(hipMalloc(&fs->khtr_u, fs->nu * sizeof(double)));
+    HIP_TRY(hipMalloc(&fs->khtr_v, fs->nv * sizeof(double)));
+  }
+  return REDI_OK;
+}
+
+void free_face_state(FaceState* fs, bool own_khtr) {
+  (void)hipFree(fs->u); (void)hipFree(fs->v);
+  (void)hipFree(fs->mono_u); (void)hipFree(fs->mono_v);
+  if (own_khtr) { (void)hipFree(fs->khtr_u); (void)hipFree(fs->khtr_v); }
+  *fs = FaceState{};
+}
+
+int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
+  const size_t ns = 2 * (size_t)nz + 2;
+  k_fill_faces<<<blocks_for(fs.nu * ns, 256), 256, 0, st>>>(fs.nu * ns, fs.u, fs.nu, fs.mono_u);
+  k_fill_faces<<<blocks_for(fs.nv * ns, 256), 256, 0, st>>>(fs.nv * ns, fs.v, fs.nv, fs.mono_v);
+  HIP_TRY(hipGetLastError());
+  return REDI_OK;
+}
+
+struct Tiles {
+  int tx = 16, ty = 4;  // one 64-lane wavefront per tile
+  int force_general = 0;
+  int group = kTracerGroup;  // tracers per flux launch (1..4)
+};
+
+TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
+  TileMap m;
+  m.n0 = n0; m.n1 = n1; m.tx = t.tx; m.ty = t.ty;
+  m.tiles0 = (n0 + t.tx - 1) / t.tx;
+  *nblocks = (unsigned)m.tiles0 * (unsigned)((n1 + t.ty - 1) / t.ty);
+  return m;
+}
 
 // Phase A over both face grids (redi_calc_coeffs_x / _y).
 int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h, const double* t,
-                   const double* s, const double* wet_u, const double* wet_v, const FaceCoeffs& u,
-                   const FaceCoeffs& v, const Workspace& ws, hipStream_t st, Timer& tm) {
+                   const double* s, const double* wet_u, const double* wet_v, const FaceState& fs,
+                   void* ws, const Tiles& tiles, hipStream_t st, Timer& tm) {
   const int N2 = nx * ny;
-  const ColumnState cs = ws.column(N2, nz);
+  ColRec* rec = (ColRec*)ws;
+  double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
-  k_column_state<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, t, s, eos, cs);
+  k_column_state<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, t, s, eos, rec, P);
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
-  // u faces i = 2..nx, j = 1..ny (1-based) -> 0-based i = 1..nx-1
-  if (nx > 1) {
-    const long nf = (long)(nx - 1) * ny;
+  const int bs = tiles.tx * tiles.ty;
+  if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
+    SweepArgs a{nx, nz, nx + 1, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
+    unsigned nb;
+    const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
-    k_neutral_sweep<<<blocks_for(nf, 64), 64, 0, st>>>(nx, N2, nz, nx + 1, ny, 1, 0, nx - 1, ny, 1, 0,
-                                                       wet_u, cs, u, pa_to_h);
+    k_neutral_sweep<<<nb, bs, 0, st>>>(a, m);
     tm.end(slot, st);
   }
-  // v faces j = 2..ny, i = 1..nx
-  if (ny > 1) {
-    const long nf = (long)nx * (ny - 1);
+  if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
+    SweepArgs a{nx, nz, nx, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
+    unsigned nb;
+    const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
-    k_neutral_sweep<<<blocks_for(nf, 64), 64, 0, st>>>(nx, N2, nz, nx, ny + 1, 0, 1, nx, ny - 1, 0, 1,
-                                                       wet_v, cs, v, pa_to_h);
+    k_neutral_sweep<<<nb, bs, 0, st>>>(a, m);
     tm.end(slot, st);
   }
   HIP_TRY(hipGetLastError());
@@ -127,40 +183,86 @@ struct PhaseB {
   int ww, we, ws, wn;
   const double *khtr_u_ext, *khtr_v_ext;
   double khtr;
-  double *khtr_u, *khtr_v;
   const double *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT, *h;
-  FaceCoeffs u, v;
 };
 
-int flux_block(int nz) { return nz <= 128 ? 64 : 32; }
-
-int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const Workspace& ws,
-                   hipStream_t st, Timer& tm) {
+int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceState& fs, void* ws,
+                   const Tiles& tiles, hipStream_t st, Timer& tm) {
   const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   int slot = tm.begin(REDI_K_FACE_KHTR, st);
   k_face_khtr<<<blocks_for((size_t)nu + nv, 256), 256, 0, st>>>(
-      nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, p.khtr_u, p.khtr_v);
+      nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v);
   tm.end(slot, st);
-  FluxArgs fa;
+  FluxArgs fa{};
   fa.nx = nx; fa.ny = ny; fa.nz = nz; fa.dt = p.dt;
   fa.wuf_w = p.nghost + 1; fa.wuf_e = p.nghost + p.nxp + 1;
   fa.wvf_s = p.nghost + 1; fa.wvf_n = p.nghost + p.nyp + 1;
   fa.wall_w = p.ww; fa.wall_e = p.we; fa.wall_s = p.ws; fa.wall_n = p.wn;
-  fa.khtr_u = p.khtr_u; fa.khtr_v = p.khtr_v;
+  fa.khtr_u = fs.khtr_u; fa.khtr_v = fs.khtr_v;
   fa.dy_cu = p.dy_cu; fa.dx_cv = p.dx_cv; fa.idxCu = p.idxCu; fa.idyCv = p.idyCv;
   fa.areaT = p.areaT;
-  fa.u = p.u; fa.v = p.v;
-  const TracerRecon rc = ws.recon(N2, nz);
-  const int bs = flux_block(nz);
-  const size_t lds = (size_t)nz * bs * sizeof(double);
-  for (int t = 0; t < nt; ++t) {
+  fa.u = fs.u; fa.v = fs.v; fa.mono_u = fs.mono_u; fa.mono_v = fs.mono_v;
+  ReconRec* rc = (ReconRec*)ws;
+  const size_t tstride = (size_t)N2 * (nz + 1);
+  const int bs = tiles.tx * tiles.ty;
+  unsigned nb;
+  const TileMap m = make_tiles(nx, ny, tiles, &nb);
+  const int G = std::max(1, std::min(tiles.group, kTracerGroup));
+  for (int g0 = 0; g0 < nt; g0 += G) {
+    const int gn = std::min(G, nt - g0);
     slot = tm.begin(REDI_K_TRACER_RECON, st);
-    k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[t], rc);
+    for (int t = 0; t < gn; ++t)
+      k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t], rc + t * tstride);
     tm.end(slot, st);
+    fa.nt = gn;
+    for (int t = 0; t < kMaxNT; ++t) {
+      const int tt = std::min(t, gn - 1);
+      fa.recon[t] = rc + (size_t)tt * tstride;
+      fa.htr[t] = tracers[g0 + tt];
+    }
+    fa.tstride = tstride;
+    fa.force_general = tiles.force_general;
     slot = tm.begin(REDI_K_APPLY_FLUX, st);
-    k_apply_flux<<<blocks_for(N2, bs), bs, lds, st>>>(fa, rc, p.h, tracers[t]);
+    switch (gn) {
+      case 1: k_apply_flux<1><<<nb, bs, 0, st>>>(fa, m); break;
+      case 2: k_apply_flux<2><<<nb, bs, 0, st>>>(fa, m); break;
+      case 3: k_apply_flux<3><<<nb, bs, 0, st>>>(fa, m); break;
+      default: k_apply_flux<4><<<nb, bs, 0, st>>>(fa, m); break;
+    }
     tm.end(slot, st);
+    slot = tm.begin(REDI_K_APPLY_FLUX_GENERAL, st);
+    switch (gn) {
+      case 1: k_apply_flux_general<1><<<nb, bs, 0, st>>>(fa, m); break;
+      case 2: k_apply_flux_general<2><<<nb, bs, 0, st>>>(fa, m); break;
+      case 3: k_apply_flux_general<3><<<nb, bs, 0, st>>>(fa, m); break;
+      default: k_apply_flux_general<4><<<nb, bs, 0, st>>>(fa, m); break;
+    }
+    tm.end(slot, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return REDI_OK;
+}
+
+// Write the reference-layout coefficient arrays (and khtr) into `d`.
+// all_faces: every face (the reference's init values where Phase A never
+// writes); otherwise only the faces redi_calc_coeffs_x/_y compute.
+int export_coeffs(const FaceState& fs, int nx, int ny, int nz, const redi_coeffs_t& d,
+                  bool all_faces, bool with_khtr, hipStream_t st) {
+  const int ns = 2 * nz + 2;
+  const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  CoeffArrays cu{d.uPoL, d.uPoR, d.uhEff, d.uKoL, d.uKoR};
+  CoeffArrays cv{d.vPoL, d.vPoR, d.vhEff, d.vKoL, d.vKoR};
+  if (all_faces) {
+    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, nx + 1, ny, ns, 0, nx + 1, 0, ny, cu);
+    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, nx, ny + 1, ns, 0, nx, 0, ny + 1, cv);
+  } else {
+    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, nx + 1, ny, ns, 1, nx, 0, ny, cu);
+    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, nx, ny + 1, ns, 0, nx, 1, ny, cv);
+  }
+  if (with_khtr) {
+    if (d.khtr_u) HIP_TRY(hipMemcpyAsync(d.khtr_u, fs.khtr_u, nu * 8, hipMemcpyDeviceToDevice, st));
+    if (d.khtr_v) HIP_TRY(hipMemcpyAsync(d.khtr_v, fs.khtr_v, nv * 8, hipMemcpyDeviceToDevice, st));
   }
   HIP_TRY(hipGetLastError());
   return REDI_OK;
@@ -174,28 +276,41 @@ bool valid_eos(int v) {
   return v == REDI_EOS_LINEAR || v == REDI_EOS_WRIGHT97 || v == REDI_EOS_ROQUET_SPV;
 }
 
-// Per-device grow-only scratch for the flat entries (which, like the
+// Per-device grow-only state for the flat entries (which, like the
 // reference launchers, receive no state object).
-struct FlatCache {
-  std::mutex mu;
-  std::vector<std::pair<double*, size_t>> per_dev;
+struct FlatSlot {
+  int nx = 0, ny = 0, nz = 0;
+  FaceState fs;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
 };
-FlatCache g_flat;
+std::mutex g_flat_mu;
+std::vector<FlatSlot> g_flat;
 
-int flat_workspace(size_t elems, Workspace* out) {
+int flat_state(int nx, int ny, int nz, int nt, FlatSlot** out) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_flat.mu);
-  if ((int)g_flat.per_dev.size() <= dev) g_flat.per_dev.resize(dev + 1, {nullptr, 0});
-  auto& slot = g_flat.per_dev[dev];
-  if (slot.second < elems) {
-    if (slot.first) HIP_TRY(hipFree(slot.first));
-    slot = {nullptr, 0};
-    HIP_TRY(hipMalloc(&slot.first, elems * sizeof(double)));
-    slot.second = elems;
+  std::lock_guard<std::mutex> lk(g_flat_mu);
+  if ((int)g_flat.size() <= dev) g_flat.resize(dev + 1);
+  FlatSlot& s = g_flat[dev];
+  if (s.nx != nx || s.ny != ny || s.nz != nz) {
+    if (s.fs.u) free_face_state(&s.fs, false);
+    s.nx = s.ny = s.nz = 0;
+    int rc = alloc_face_state(&s.fs, nx, ny, nz, false);
+    if (rc) return rc;
+    rc = fill_face_state(s.fs, nz, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    s.nx = nx; s.ny = ny; s.nz = nz;
   }
-  out->base = slot.first;
-  out->elems = slot.second;
+  const size_t wb = workspace_bytes(nx, ny, nz, nt);
+  if (s.ws_bytes < wb) {
+    if (s.ws) HIP_TRY(hipFree(s.ws));
+    s.ws = nullptr; s.ws_bytes = 0;
+    HIP_TRY(hipMalloc(&s.ws, wb));
+    s.ws_bytes = wb;
+  }
+  *out = &s;
   return REDI_OK;
 }
 
@@ -206,10 +321,10 @@ struct redi_s {
   int nx = 0, ny = 0, nz = 0, ns = 0;
   int enable = 0, continuous = 1;
   double khtr = 0.0, ref_pres = -1.0;
-  bool owns = false;
-  FaceCoeffs u{}, v{};
-  double *khtr_u = nullptr, *khtr_v = nullptr;
-  Workspace ws;
+  FaceState fs;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  Tiles tiles;
   Timer timer;
   bool is_init = false;
 };
@@ -218,72 +333,45 @@ extern "C" {
 
 const char* redi_last_error(void) { return g_err.c_str(); }
 
-int redi_version(void) { return 100; }
+int redi_version(void) { return 200; }
 
 long long redi_workspace_bytes(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return 0;
-  return (long long)(workspace_elems(nx, ny, nz) * sizeof(double));
+  return (long long)workspace_bytes(nx, ny, nz, kTracerGroup);
 }
 
-int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml, const redi_coeffs_t* bufs) {
+long long redi_state_bytes(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return 0;
+  return (long long)(face_state_bytes(nx, ny, nz) + workspace_bytes(nx, ny, nz, kTracerGroup));
+}
+
+int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
   if (!out || !grid) return fail(REDI_ERR_ARG, "redi_init: null argument");
   *out = nullptr;
   const int nx = grid->nx_total, ny = grid->ny_total;
   if (nx < 1 || ny < 1) return fail(REDI_ERR_ARG, "redi_init: bad grid %d x %d", nx, ny);
-  int nz = nz_ml < 1 ? 1 : nz_ml;
+  const int nz = nz_ml < 1 ? 1 : nz_ml;
   if (nz > REDI_NZ_MAX)
     return fail(REDI_ERR_NZ, "redi_init: 2*nz_ml+2 = %d exceeds the neutral-surface limit %d",
                 2 * nz + 2, 2 * REDI_NZ_MAX + 2);
   redi_t* r = new redi_t();
-  HIP_TRY(hipGetDevice(&r->device));
-  r->nx = nx; r->ny = ny; r->nz = nz; r->ns = 2 * nz + 2;
-  const size_t npu = (size_t)(nx + 1) * ny * r->ns, nhu = (size_t)(nx + 1) * ny * (r->ns - 1);
-  const size_t npv = (size_t)nx * (ny + 1) * r->ns, nhv = (size_t)nx * (ny + 1) * (r->ns - 1);
   auto cleanup = [&](int code) {
-    if (r->owns) {
-      hipFree(r->u.PoL); hipFree(r->u.PoR); hipFree(r->u.KoL); hipFree(r->u.KoR); hipFree(r->u.hEff);
-      hipFree(r->v.PoL); hipFree(r->v.PoR); hipFree(r->v.KoL); hipFree(r->v.KoR); hipFree(r->v.hEff);
-      hipFree(r->khtr_u); hipFree(r->khtr_v);
-    }
-    hipFree(r->ws.base);
+    free_face_state(&r->fs, true);
+    (void)hipFree(r->ws);
     delete r;
     return code;
   };
-  if (bufs) {
-    if (!bufs->uPoL || !bufs->uPoR || !bufs->uKoL || !bufs->uKoR || !bufs->uhEff || !bufs->vPoL ||
-        !bufs->vPoR || !bufs->vKoL || !bufs->vKoR || !bufs->vhEff || !bufs->khtr_u || !bufs->khtr_v)
-      return cleanup(fail(REDI_ERR_ARG, "redi_init: incomplete coefficient buffers"));
-    r->u = FaceCoeffs{bufs->uPoL, bufs->uPoR, bufs->uKoL, bufs->uKoR, bufs->uhEff};
-    r->v = FaceCoeffs{bufs->vPoL, bufs->vPoR, bufs->vKoL, bufs->vKoR, bufs->vhEff};
-    r->khtr_u = bufs->khtr_u;
-    r->khtr_v = bufs->khtr_v;
-  } else {
-    r->owns = true;
-    hipError_t e = hipSuccess;
-    auto am = [&](void** p, size_t b) { if (e == hipSuccess) e = hipMalloc(p, b); };
-    am((void**)&r->u.PoL, npu * 8); am((void**)&r->u.PoR, npu * 8);
-    am((void**)&r->u.KoL, npu * 4); am((void**)&r->u.KoR, npu * 4); am((void**)&r->u.hEff, nhu * 8);
-    am((void**)&r->v.PoL, npv * 8); am((void**)&r->v.PoR, npv * 8);
-    am((void**)&r->v.KoL, npv * 4); am((void**)&r->v.KoR, npv * 4); am((void**)&r->v.hEff, nhv * 8);
-    am((void**)&r->khtr_u, (size_t)(nx + 1) * ny * 8);
-    am((void**)&r->khtr_v, (size_t)nx * (ny + 1) * 8);
-    if (e != hipSuccess)
-      return cleanup(fail(REDI_ERR_HIP, "redi_init: hipMalloc: %s", hipGetErrorString(e)));
-  }
-  {
-    const size_t we = workspace_elems(nx, ny, nz);
-    hipError_t e = hipMalloc(&r->ws.base, we * sizeof(double));
-    if (e != hipSuccess)
-      return cleanup(fail(REDI_ERR_HIP, "redi_init: workspace hipMalloc(%zu B): %s",
-                          we * sizeof(double), hipGetErrorString(e)));
-    r->ws.elems = we;
-  }
-  // allocate(..., source=0 / 1) (:493-505)
-  k_fill_coeffs<<<blocks_for(npu, 256), 256>>>(npu, nhu, r->u);
-  k_fill_coeffs<<<blocks_for(npv, 256), 256>>>(npv, nhv, r->v);
-  if (hipMemset(r->khtr_u, 0, (size_t)(nx + 1) * ny * 8) != hipSuccess ||
-      hipMemset(r->khtr_v, 0, (size_t)nx * (ny + 1) * 8) != hipSuccess ||
-      hipDeviceSynchronize() != hipSuccess)
+  if (hipGetDevice(&r->device) != hipSuccess) return cleanup(fail(REDI_ERR_HIP, "redi_init: no device"));
+  r->nx = nx; r->ny = ny; r->nz = nz; r->ns = 2 * nz + 2;
+  int rc = alloc_face_state(&r->fs, nx, ny, nz, true);
+  if (rc) return cleanup(rc);
+  r->ws_bytes = workspace_bytes(nx, ny, nz, kTracerGroup);
+  if (hipMalloc(&r->ws, r->ws_bytes) != hipSuccess)
+    return cleanup(fail(REDI_ERR_HIP, "redi_init: workspace hipMalloc(%zu B) failed", r->ws_bytes));
+  rc = fill_face_state(r->fs, nz, nullptr);
+  if (rc) return cleanup(rc);
+  if (hipMemset(r->fs.khtr_u, 0, r->fs.nu * 8) != hipSuccess ||
+      hipMemset(r->fs.khtr_v, 0, r->fs.nv * 8) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
     return cleanup(fail(REDI_ERR_HIP, "redi_init: initial fill failed"));
   r->is_init = true;
   *out = r;
@@ -292,13 +380,9 @@ int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml, const redi_coef
 
 int redi_destroy(redi_t* r) {
   if (!r) return REDI_OK;
-  hipDeviceSynchronize();
-  if (r->owns) {
-    hipFree(r->u.PoL); hipFree(r->u.PoR); hipFree(r->u.KoL); hipFree(r->u.KoR); hipFree(r->u.hEff);
-    hipFree(r->v.PoL); hipFree(r->v.PoR); hipFree(r->v.KoL); hipFree(r->v.KoR); hipFree(r->v.hEff);
-    hipFree(r->khtr_u); hipFree(r->khtr_v);
-  }
-  hipFree(r->ws.base);
+  (void)hipDeviceSynchronize();
+  free_face_state(&r->fs, true);
+  (void)hipFree(r->ws);
   delete r;
   return REDI_OK;
 }
@@ -314,12 +398,47 @@ int redi_configure(redi_t* r, int enable, double khtr, int continuous, double re
   return REDI_OK;
 }
 
-int redi_get_coeffs(const redi_t* r, redi_coeffs_t* o) {
-  if (!r || !o) return fail(REDI_ERR_ARG, "redi_get_coeffs: null argument");
-  *o = redi_coeffs_t{r->u.PoL, r->u.PoR, r->u.KoL, r->u.KoR, r->u.hEff,
-                     r->v.PoL, r->v.PoR, r->v.KoL, r->v.KoR, r->v.hEff,
-                     r->khtr_u, r->khtr_v};
+int redi_set_tiles(redi_t* r, int tx, int ty) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_tiles: null handle");
+  if (tx < 1 || ty < 1 || tx * ty > 256 || (tx * ty) % 64 != 0)
+    return fail(REDI_ERR_ARG, "redi_set_tiles: tile %dx%d must hold 64, 128, 192 or 256 lanes", tx, ty);
+  r->tiles.tx = tx;
+  r->tiles.ty = ty;
   return REDI_OK;
+}
+
+int redi_set_tracer_group(redi_t* r, int n) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_tracer_group: null handle");
+  if (n < 1 || n > kTracerGroup)
+    return fail(REDI_ERR_ARG, "redi_set_tracer_group: %d not in 1..%d", n, kTracerGroup);
+  r->tiles.group = n;
+  return REDI_OK;
+}
+
+int redi_set_flux_path(redi_t* r, int general) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_flux_path: null handle");
+  r->tiles.force_general = general ? 1 : 0;
+  return REDI_OK;
+}
+
+int redi_nonmonotone_faces(const redi_t* r, long long* nu, long long* nv) {
+  if (!r || !nu || !nv) return fail(REDI_ERR_ARG, "redi_nonmonotone_faces: null argument");
+  std::vector<uint8_t> hu(r->fs.nu), hv(r->fs.nv);
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(hu.data(), r->fs.mono_u, r->fs.nu, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hv.data(), r->fs.mono_v, r->fs.nv, hipMemcpyDeviceToHost));
+  *nu = 0; *nv = 0;
+  for (uint8_t f : hu) *nu += (f != (kMonoL | kMonoR));
+  for (uint8_t f : hv) *nv += (f != (kMonoL | kMonoR));
+  return REDI_OK;
+}
+
+int redi_export_coeffs(const redi_t* r, const redi_coeffs_t* dst, void* stream) {
+  if (!r || !dst) return fail(REDI_ERR_ARG, "redi_export_coeffs: null argument");
+  if (!dst->uPoL || !dst->uPoR || !dst->uKoL || !dst->uKoR || !dst->uhEff || !dst->vPoL ||
+      !dst->vPoR || !dst->vKoL || !dst->vKoR || !dst->vhEff)
+    return fail(REDI_ERR_ARG, "redi_export_coeffs: incomplete destination arrays");
+  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
 }
 
 int redi_calc_coeffs(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_t* eos,
@@ -339,7 +458,7 @@ int redi_calc_coeffs(const redi_hgrid_t* grid, const redi_metrics_t* m, const re
   if (!m->wet_u || !m->wet_v) return fail(REDI_ERR_ARG, "redi_calc_coeffs: null wet mask");
   return launch_phase_a(r->nx, r->ny, r->nz, to_eos(*eos), ms->h_layer,
                         ms->tracers[ms->idx_temperature - 1], ms->tracers[ms->idx_salinity - 1],
-                        m->wet_u, m->wet_v, r->u, r->v, r->ws, (hipStream_t)stream, r->timer);
+                        m->wet_u, m->wet_v, r->fs, r->ws, r->tiles, (hipStream_t)stream, r->timer);
 }
 
 int redi_apply_flux(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
@@ -370,11 +489,10 @@ int redi_apply_flux(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r
   p.khtr_u_ext = use_ext ? kue : nullptr;
   p.khtr_v_ext = use_ext ? kve : nullptr;
   p.khtr = r->khtr;
-  p.khtr_u = r->khtr_u; p.khtr_v = r->khtr_v;
   p.dy_cu = m->dy_cu; p.dx_cv = m->dx_cv; p.idxCu = m->idxCu; p.idyCv = m->idyCv;
   p.areaT = m->areaT; p.h = ms->h_layer;
-  p.u = r->u; p.v = r->v;
-  return launch_phase_b(p, ms->ntracers, ms->tracers, r->ws, (hipStream_t)stream, r->timer);
+  return launch_phase_b(p, ms->ntracers, ms->tracers, r->fs, r->ws, r->tiles, (hipStream_t)stream,
+                        r->timer);
 }
 
 int redi_apply_flux_hoist(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
@@ -409,22 +527,27 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   if (idx_T < 1 || idx_T > nt || idx_S < 1 || idx_S > nt)
     return fail(REDI_ERR_ARG, "redi_launch_nt: tracer index out of range");
   if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "redi_launch_nt: unknown EOS variant");
-  Workspace ws;
-  int rc = flat_workspace(workspace_elems(nx, ny, nz), &ws);
+  if (!cf->khtr_u || !cf->khtr_v) return fail(REDI_ERR_ARG, "redi_launch_nt: null khtr_u/khtr_v");
+  FlatSlot* fsl = nullptr;
+  int rc = flat_state(nx, ny, nz, nt, &fsl);
   if (rc) return rc;
-  const FaceCoeffs u{cf->uPoL, cf->uPoR, cf->uKoL, cf->uKoR, cf->uhEff};
-  const FaceCoeffs v{cf->vPoL, cf->vPoR, cf->vKoL, cf->vKoR, cf->vhEff};
+  FaceState fs = fsl->fs;
+  fs.khtr_u = cf->khtr_u;  // the caller's khtr arrays are the resolved-diffusivity slots
+  fs.khtr_v = cf->khtr_v;
+  const hipStream_t st = (hipStream_t)stream;
+  const Tiles tiles;
   rc = launch_phase_a(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1], tracers[idx_S - 1],
-                      wet_u, wet_v, u, v, ws, (hipStream_t)stream, g_no_timer);
+                      wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
+  if (rc) return rc;
+  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, st);
   if (rc) return rc;
   PhaseB p;
   p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;
   p.ww = ww; p.we = we; p.ws = wsz; p.wn = wn;
   p.khtr_u_ext = kue; p.khtr_v_ext = kve; p.khtr = 0.0;
-  p.khtr_u = cf->khtr_u; p.khtr_v = cf->khtr_v;
   p.dy_cu = dy_cu; p.dx_cv = dx_cv; p.idxCu = idxCu; p.idyCv = idyCv; p.areaT = areaT;
-  p.h = h_layer; p.u = u; p.v = v;
-  return launch_phase_b(p, nt, tracers, ws, (hipStream_t)stream, g_no_timer);
+  p.h = h_layer;
+  return launch_phase_b(p, nt, tracers, fs, fsl->ws, tiles, st, g_no_timer);
 }
 
 int redi_timing_enable(redi_t* r, int on) {
@@ -452,7 +575,8 @@ int redi_timing_collect(redi_t* r, redi_kernel_times_t* out) {
 const char* redi_kernel_name(int kind) {
   static const char* names[REDI_NKERNELS] = {"k_column_state", "k_neutral_sweep_u",
                                              "k_neutral_sweep_v", "k_face_khtr",
-                                             "k_tracer_recon", "k_apply_flux", "", ""};
+                                             "k_tracer_recon", "k_apply_flux",
+                                             "k_apply_flux_general", ""};
   return (kind >= 0 && kind < REDI_NKERNELS) ? names[kind] : "";
 }
 
@@ -474,7 +598,7 @@ static void flat_launch(const int* nx, const int* ny, const int* nz, const doubl
                                 h_layer, 2, tr, 1, 2, wet_u, wet_v, &cf, khtr_u_ext, khtr_v_ext,
                                 dy_cu, dx_cv, idxCu, idyCv, areaT, nullptr);
   if (rc < 0) fprintf(stderr, "redi flat launch: %s\n", g_err.c_str());
-  hipDeviceSynchronize();
+  (void)hipDeviceSynchronize();
 }
 
 void redi_opt_launch(const int* nx, const int* ny, const int* nz, const int* /*ns*/,

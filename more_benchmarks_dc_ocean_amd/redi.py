@@ -71,7 +71,9 @@ class Coeffs(ctypes.Structure):
 COEFF_NAMES = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", "vKoR", "vhEff")
 
 # every extern "C" symbol declared in include/redi_hip.h
-EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_get_coeffs",
+EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs", "redi_set_tiles",
+            "redi_state_bytes", "redi_set_flux_path", "redi_nonmonotone_faces",
+            "redi_set_tracer_group",
             "redi_calc_coeffs", "redi_apply_flux", "redi_apply_flux_hoist", "redi_step",
             "redi_workspace_bytes", "redi_last_error", "redi_version", "redi_opt_launch",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
@@ -92,10 +94,17 @@ def lib() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RediError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
-        L.redi_init.argtypes = [ctypes.POINTER(_P), ctypes.POINTER(HGrid), _I, ctypes.POINTER(Coeffs)]
+        L.redi_init.argtypes = [ctypes.POINTER(_P), ctypes.POINTER(HGrid), _I]
         L.redi_destroy.argtypes = [_P]
         L.redi_configure.argtypes = [_P, _I, _D, _I, _D]
-        L.redi_get_coeffs.argtypes = [_P, ctypes.POINTER(Coeffs)]
+        L.redi_export_coeffs.argtypes = [_P, ctypes.POINTER(Coeffs), _P]
+        L.redi_set_tiles.argtypes = [_P, _I, _I]
+        L.redi_set_flux_path.argtypes = [_P, _I]
+        L.redi_set_tracer_group.argtypes = [_P, _I]
+        L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
+                                             ctypes.POINTER(ctypes.c_longlong)]
+        L.redi_state_bytes.restype = ctypes.c_longlong
+        L.redi_state_bytes.argtypes = [_I, _I, _I]
         L.redi_calc_coeffs.argtypes = [ctypes.POINTER(HGrid), ctypes.POINTER(Metrics),
                                        ctypes.POINTER(Eos), _P, ctypes.POINTER(State), _P]
         L.redi_apply_flux.argtypes = [ctypes.POINTER(HGrid), ctypes.POINTER(Metrics), _P,
@@ -180,12 +189,11 @@ class DeviceDomain:
 class OceanRedi:
     """``ocean_redi_t`` on one GPU.
 
-    ``coeff_tensors=True`` binds torch-owned coefficient arrays (so tests and
-    callers can read them as tensors); otherwise the library allocates them.
+    The Phase-A state lives in HBM inside the library; ``coeffs()`` exports
+    it in the reference layout.
     """
 
-    def __init__(self, case_or_grid, nz: Optional[int] = None, device: str = "cuda",
-                 coeff_tensors: bool = True):
+    def __init__(self, case_or_grid, nz: Optional[int] = None, device: str = "cuda"):
         import torch
         L = lib()
         if isinstance(case_or_grid, RediCase):
@@ -199,31 +207,8 @@ class OceanRedi:
             self.nx, self.ny = grid.nx_total, grid.ny_total
             self.nz = max(1, int(nz))
             self.ns = 2 * self.nz + 2
-            self.tensors = None
-            bufs = None
-            if coeff_tensors and self.nz <= REDI_NZ_MAX:
-                nx, ny, ns = self.nx, self.ny, self.ns
-                d = self.device
-                f64, i32 = torch.float64, torch.int32
-                self.tensors = {
-                    "uPoL": torch.empty((ns, ny, nx + 1), dtype=f64, device=d),
-                    "uPoR": torch.empty((ns, ny, nx + 1), dtype=f64, device=d),
-                    "uKoL": torch.empty((ns, ny, nx + 1), dtype=i32, device=d),
-                    "uKoR": torch.empty((ns, ny, nx + 1), dtype=i32, device=d),
-                    "uhEff": torch.empty((ns - 1, ny, nx + 1), dtype=f64, device=d),
-                    "vPoL": torch.empty((ns, ny + 1, nx), dtype=f64, device=d),
-                    "vPoR": torch.empty((ns, ny + 1, nx), dtype=f64, device=d),
-                    "vKoL": torch.empty((ns, ny + 1, nx), dtype=i32, device=d),
-                    "vKoR": torch.empty((ns, ny + 1, nx), dtype=i32, device=d),
-                    "vhEff": torch.empty((ns - 1, ny + 1, nx), dtype=f64, device=d),
-                    "khtr_u": torch.empty((ny, nx + 1), dtype=f64, device=d),
-                    "khtr_v": torch.empty((ny + 1, nx), dtype=f64, device=d),
-                }
-                torch.cuda.synchronize(self.device)
-                bufs = Coeffs(*(self.tensors[n].data_ptr() for n in COEFF_NAMES + ("khtr_u", "khtr_v")))
             h = _P()
-            _check(L.redi_init(ctypes.byref(h), ctypes.byref(grid), int(nz),
-                               ctypes.byref(bufs) if bufs is not None else None), "redi_init")
+            _check(L.redi_init(ctypes.byref(h), ctypes.byref(grid), int(nz)), "redi_init")
             self.handle = h
         self.enable = False
         self.khtr = 0.0
@@ -299,10 +284,48 @@ class OceanRedi:
                 out[lib().redi_kernel_name(k).decode()] = (kt.ms[k], int(kt.launches[k]))
         return out
 
-    def coeffs(self) -> dict:
-        if self.tensors is None:
-            raise RediError("coefficient arrays are library-owned; construct with coeff_tensors=True")
-        return {n: self.tensors[n] for n in COEFF_NAMES}
+    def set_flux_path(self, general: bool) -> None:
+        _check(lib().redi_set_flux_path(self.handle, int(bool(general))), "redi_set_flux_path")
+
+    def nonmonotone_faces(self) -> tuple:
+        nu, nv = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().redi_nonmonotone_faces(self.handle, ctypes.byref(nu), ctypes.byref(nv)),
+               "redi_nonmonotone_faces")
+        return nu.value, nv.value
+
+    def set_tracer_group(self, n: int) -> None:
+        _check(lib().redi_set_tracer_group(self.handle, int(n)), "redi_set_tracer_group")
+
+    def set_tiles(self, tx: int, ty: int) -> None:
+        _check(lib().redi_set_tiles(self.handle, int(tx), int(ty)), "redi_set_tiles")
+
+    def coeffs(self, with_khtr: bool = False) -> dict:
+        """The Phase-A arrays uPoL..vhEff (and khtr_u/v) in the reference
+        layout, as new tensors: (ns, ny, nx+1), (ns-1, ny, nx+1),
+        (ns, ny+1, nx), (ns-1, ny+1, nx); Ko int32."""
+        import torch
+        nx, ny, ns, d = self.nx, self.ny, self.ns, self.device
+        f64, i32 = torch.float64, torch.int32
+        t = {
+            "uPoL": torch.empty((ns, ny, nx + 1), dtype=f64, device=d),
+            "uPoR": torch.empty((ns, ny, nx + 1), dtype=f64, device=d),
+            "uKoL": torch.empty((ns, ny, nx + 1), dtype=i32, device=d),
+            "uKoR": torch.empty((ns, ny, nx + 1), dtype=i32, device=d),
+            "uhEff": torch.empty((ns - 1, ny, nx + 1), dtype=f64, device=d),
+            "vPoL": torch.empty((ns, ny + 1, nx), dtype=f64, device=d),
+            "vPoR": torch.empty((ns, ny + 1, nx), dtype=f64, device=d),
+            "vKoL": torch.empty((ns, ny + 1, nx), dtype=i32, device=d),
+            "vKoR": torch.empty((ns, ny + 1, nx), dtype=i32, device=d),
+            "vhEff": torch.empty((ns - 1, ny + 1, nx), dtype=f64, device=d),
+            "khtr_u": torch.empty((ny, nx + 1), dtype=f64, device=d),
+            "khtr_v": torch.empty((ny + 1, nx), dtype=f64, device=d),
+        }
+        cf = Coeffs(*(t[n].data_ptr() for n in COEFF_NAMES + ("khtr_u", "khtr_v")))
+        _check(lib().redi_export_coeffs(self.handle, ctypes.byref(cf), _stream(d)),
+               "redi_export_coeffs")
+        if not with_khtr:
+            t.pop("khtr_u"); t.pop("khtr_v")
+        return t
 
 
 def make_redi(case: RediCase, device: str = "cuda"):

@@ -24,11 +24,17 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
 
-def _run_hip(case, nsteps, device, hoist=False):
+def _run_hip(case, nsteps, device, hoist=False, general=False, tiles=None, group=None):
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     print(f"[gpu] {case.name} {case.nx}x{case.ny}x{case.nz} nt={case.nt} steps={nsteps}", flush=True)
     r, dom = make_redi(case, device)
+    if general:
+        r.set_flux_path(True)
+    if tiles:
+        r.set_tiles(*tiles)
+    if group:
+        r.set_tracer_group(group)
     out = {}
     for s in range(nsteps):
         assert r.calc_coeffs(dom) == 0
@@ -112,6 +118,33 @@ def test_hip_matches_oracle_random_cases(seed, eos, nz, walls, use_ext, gpu_devi
     got = _run_hip(c, 3, gpu_device)
     orc = redi_oracle.run_oracle(c)
     _compare(got, orc, f"random seed={seed}")
+
+
+@pytest.mark.parametrize("name", ["roquet_mini", "wright_vanished_mini", "seamount_mini"])
+def test_hip_general_flux_path_same_bits(name, gpu_device):
+    """Every cell through the order-general flux path gives the same result."""
+    case, ref = load_golden(name)
+    got = _run_hip(case, case.nsteps, gpu_device, general=True)
+    _compare(got, ref, name + "/general")
+
+
+@pytest.mark.parametrize("tiles,group", [((64, 1), 1), ((8, 8), 2), ((32, 4), 3), ((16, 16), 4)])
+def test_hip_launch_shapes_same_bits(tiles, group, gpu_device):
+    case, ref = load_golden("roquet_two_fronts_mini")
+    got = _run_hip(case, case.nsteps, gpu_device, tiles=tiles, group=group)
+    _compare(got, ref, f"roquet_two_fronts_mini tiles={tiles} group={group}")
+
+
+def test_nonmonotone_faces_reported(gpu_device):
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    case = inputs.case_front(nsteps=1)
+    r, dom = make_redi(case, gpu_device)
+    r.calc_coeffs(dom)
+    nu, nv = r.nonmonotone_faces()
+    print(f"[gpu] front 473x297x75: non-monotone faces u={nu} v={nv} of "
+          f"{(case.nx + 1) * case.ny + case.nx * (case.ny + 1)}", flush=True)
+    assert nu >= 0 and nv >= 0
+    r.close()
 
 
 def test_hip_deterministic(gpu_device):
