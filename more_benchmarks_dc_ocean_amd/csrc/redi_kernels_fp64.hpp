@@ -16,12 +16,16 @@
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
 //   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
 //                               kt = top-down interface 0..nz
-//   tracer recon (K3 -> K4)     per column contiguous: ReconRec[t][col][kt]
-//   face coefficients (K2->K4)  per face contiguous: FaceRec[face][s], s = 0..ns-1,
+//   tracer recon (K3 -> K4)     layer-major per tracer: ReconRec[kt][col] and
+//                               the interface edges Ti[kt][col]
+//   face coefficients (K2->K4)  surface-major: FaceRec[s][face], s = 0..ns-1,
 //                               plus one monotonicity flag byte per face
-// Per-column / per-face contiguity lets each lane stream its own columns and
-// faces with full use of every 128-byte line it touches, which matters
-// because lanes of one wavefront sit at different depths of their sweeps.
+// The sweep reads each of its two columns once, front to back, while lanes
+// sit at unrelated depths, so the column state is stored per column and each
+// lane streams whole 128-byte lines of its own columns.  The flux kernel's
+// lanes gather at similar depths of neighbouring columns, and every sweep
+// lane writes surface s at the same time, so those arrays are layer- and
+// surface-major and the lanes of a tile row share lines.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,7 +48,7 @@ struct __align__(16) FaceRec {  // one neutral surface s of a face (K2 output)
 
 struct __align__(16) ReconRec {  // one layer kt of a tracer column (K3 output)
   double Tl, aL, aR, Ti;         // layer mean, limited top/bottom edges, top interface
-};                               // (record nz holds the bottom interface in Ti)
+};
 
 // Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
 // non-decreasing in s on the left (bit 0) / right (bit 1) column.
@@ -145,12 +149,13 @@ struct SweepSide {
 struct SweepArgs {
   int nx, nz;
   int nfa;                 // faces per row of the face array
+  size_t nf;               // faces in the face array (record stride between surfaces)
   int i0, j0;              // first computed face
   int di, dj;              // left column = (i - di, j - dj)
   const double* wet;
   const ColRec* rec;
   const double* P;
-  FaceRec* out;            // [face][s]
+  FaceRec* out;            // [s][face]
   uint8_t* mono;           // [face]
   double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
 };
@@ -161,9 +166,10 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
   const int i = a.i0 + fi, j = a.j0 + fj;
   const size_t fo = (size_t)i + (size_t)a.nfa * j;
   const int nk = a.nz, ns = 2 * nk + 2;
-  FaceRec* o = a.out + fo * ns;
+  FaceRec* o = a.out + fo;
+  const size_t fst = a.nf;
   if (!(a.wet[fo] > 0.0)) {  // dry face: the inert values of :722-730
-    for (int s = 0; s < ns; ++s) o[s] = FaceRec{0.0, 0.0, 0.0, 1, 1};
+    for (int s = 0; s < ns; ++s) o[s * fst] = FaceRec{0.0, 0.0, 0.0, 1, 1};
     a.mono[fo] = kMonoL | kMonoR;
     return;
   }
@@ -273,13 +279,13 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
         contK_l = pend.KoL;
         contK_r = pend.KoR;
       }
-      o[ks - 2] = pend;
+      o[(size_t)(ks - 2) * fst] = pend;
     }
     pend = FaceRec{PoL, PoR, 0.0, KoL, KoR};
     lastPos_l = posL;
     lastPos_r = posR;
   }
-  o[ns - 1] = pend;
+  o[(size_t)(ns - 1) * fst] = pend;
   a.mono[fo] = (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0));
 }
 
@@ -288,11 +294,11 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
                                                       const double* __restrict__ q,
-                                                      ReconRec* __restrict__ out) {
+                                                      ReconRec* __restrict__ rec,
+                                                      double* __restrict__ ti) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
-  ReconRec* r = out + (size_t)c * (nz + 1);
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
     hh = h[o];
@@ -305,9 +311,9 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
       const double tl = s_above[0];
       double al = e_prev, ar = e[0];
       ppm_limit<double>(tl, al, ar);
-      r[k - 1] = ReconRec{tl, al, ar, e_prev};
+      rec[(size_t)(k - 1) * n2 + c] = ReconRec{tl, al, ar, e_prev};
     }
-    if (k == nz) r[nz] = ReconRec{0.0, 0.0, 0.0, e[0]};
+    ti[(size_t)k * n2 + c] = e[0];
     e_prev = e[0];
   };
   stream_edges<double, 1>(nz, load, emit);
@@ -318,15 +324,20 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
 // + redi_sublayer_dT, :893-925, :1037-1181)
 // ---------------------------------------------------------------------------
 
-// redi_sublayer_dT (1-based top-down layer indices) on recon records
+// redi_sublayer_dT (1-based top-down layer indices) on the recon arrays of
+// the left / right columns (layer-major with stride n2)
+// (32-bit element offsets from uniform bases: the host checks that every
+// layer-major scratch array holds fewer than 2^31 elements)
 __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
                                               double PoLb, double PoRt, double PoRb,
-                                              const ReconRec* L, const ReconRec* R) {
-  const ReconRec lt = L[klt - 1], rt = R[krt - 1];
-  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * L[klt].Ti;
-  const double tlb = (1.0 - PoLb) * L[klb - 1].Ti + PoLb * L[klb].Ti;
-  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * R[krt].Ti;
-  const double trb = (1.0 - PoRb) * R[krb - 1].Ti + PoRb * R[krb].Ti;
+                                              const ReconRec* __restrict__ rec,
+                                              const double* __restrict__ ti, uint32_t cl,
+                                              uint32_t cr, uint32_t n2) {
+  const ReconRec lt = rec[(uint32_t)(klt - 1) * n2 + cl], rt = rec[(uint32_t)(krt - 1) * n2 + cr];
+  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * ti[(uint32_t)klt * n2 + cl];
+  const double tlb = (1.0 - PoLb) * ti[(uint32_t)(klb - 1) * n2 + cl] + PoLb * ti[(uint32_t)klb * n2 + cl];
+  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * ti[(uint32_t)krt * n2 + cr];
+  const double trb = (1.0 - PoRb) * ti[(uint32_t)(krb - 1) * n2 + cr] + PoRb * ti[(uint32_t)krb * n2 + cr];
   const double tlay = ppm_ave<double>(PoLt, PoLb + (double)(klb - klt), lt.aL, lt.aR, lt.Tl);
   const double trlay = ppm_ave<double>(PoRt, PoRb + (double)(krb - krt), rt.aL, rt.aR, rt.Tl);
   const double dT_top = trt - tlt;
@@ -351,15 +362,16 @@ struct FluxArgs {
   const uint8_t *mono_u, *mono_v;
   int nt;                          // tracers in this launch (<= NT)
   int force_general;               // take the order-general path for every cell
-  size_t tstride;                  // records between consecutive tracers' recon
-  const ReconRec* recon[kMaxNT];   // per tracer, [col][kt]
+  size_t nfu, nfv;                 // u / v faces (record stride between surfaces)
+  const ReconRec* recon[kMaxNT];   // per tracer, [kt][col]
+  const double* ti[kMaxNT];        // per tracer, [kt][col], kt = 0..nz
   double* htr[kMaxNT];
 };
 
 struct FaceRef {
-  const FaceRec* rec;  // this face's ns records
-  const ReconRec* L0;  // left column recon of tracer 0 (tracer t: + t * tstride)
-  const ReconRec* R0;
+  const FaceRec* rec;  // surface 0 of this face; surface s at rec[s * fst]
+  size_t fst;
+  uint32_t cl, cr;     // left / right column indices
   double coef;
   bool left;           // this cell is the face's left (west/south) column
 };
@@ -368,17 +380,17 @@ struct FaceRef {
 // starting at sublayer s and stopping at the first contributing sublayer
 // with another target; returns the new s.
 template <int NT>
-__device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, size_t tstride,
-                                        double* acc) {
+__device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, const FluxArgs& a,
+                                        size_t n2, double* acc) {
   while (s < ns - 1) {
-    const FaceRec top = f.rec[s];
+    const FaceRec top = f.rec[(size_t)s * f.fst];
     if (top.hEff != 0.0) {
       if ((f.left ? top.KoL : top.KoR) != K) break;
-      const FaceRec bot = f.rec[s + 1];
+      const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                     bot.PoR, f.L0 + t * tstride, f.R0 + t * tstride);
+                                     bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, (uint32_t)n2);
         const double flx = (d * top.hEff) * f.coef;
         acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
       }
@@ -391,16 +403,16 @@ __device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, 
 // The same accumulation without the monotonicity assumption: for layer K,
 // scan every sublayer of the face and take those whose target is K.
 template <int NT>
-__device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, size_t tstride,
-                                          double* acc) {
+__device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, const FluxArgs& a,
+                                          size_t n2, double* acc) {
   for (int s = 0; s < ns - 1; ++s) {
-    const FaceRec top = f.rec[s];
+    const FaceRec top = f.rec[(size_t)s * f.fst];
     if (top.hEff == 0.0 || (f.left ? top.KoL : top.KoR) != K) continue;
-    const FaceRec bot = f.rec[s + 1];
+    const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                   bot.PoR, f.L0 + t * tstride, f.R0 + t * tstride);
+                                   bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, (uint32_t)n2);
       const double flx = (d * top.hEff) * f.coef;
       acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
     }
@@ -413,16 +425,15 @@ __device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, size_
 // face's contributing sublayers are monotone on this cell's side.
 __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, FaceRef* F,
                                            bool* act) {
-  const int nx = a.nx, ny = a.ny, nz = a.nz, ns = 2 * nz + 2;
+  const int nx = a.nx, ny = a.ny;
   const size_t c = (size_t)ci + (size_t)nx * cj;
   const int i = ci + 1, j = cj + 1;  // 1-based
-  const ReconRec* C0 = a.recon[0] + c * (nz + 1);
   bool mono = true;
   // WEST u-face (face i): this cell is the right column
   act[0] = i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e));
   if (act[0]) {
     const size_t fo = (size_t)(i - 1) + (size_t)(nx + 1) * (j - 1);
-    F[0] = FaceRef{a.u + fo * ns, C0 - (nz + 1), C0,
+    F[0] = FaceRef{a.u + fo, a.nfu, (uint32_t)(c - 1), (uint32_t)c,
                    ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], false};
     mono = mono && (a.mono_u[fo] & kMonoR);
   }
@@ -430,7 +441,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[1] = i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e));
   if (act[1]) {
     const size_t fo = (size_t)i + (size_t)(nx + 1) * (j - 1);
-    F[1] = FaceRef{a.u + fo * ns, C0, C0 + (nz + 1),
+    F[1] = FaceRef{a.u + fo, a.nfu, (uint32_t)c, (uint32_t)(c + 1),
                    ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], true};
     mono = mono && (a.mono_u[fo] & kMonoL);
   }
@@ -438,7 +449,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[2] = j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n));
   if (act[2]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * (j - 1);
-    F[2] = FaceRef{a.v + fo * ns, C0 - (size_t)nx * (nz + 1), C0,
+    F[2] = FaceRef{a.v + fo, a.nfv, (uint32_t)(c - nx), (uint32_t)c,
                    ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], false};
     mono = mono && (a.mono_v[fo] & kMonoR);
   }
@@ -446,7 +457,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[3] = j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n));
   if (act[3]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * j;
-    F[3] = FaceRef{a.v + fo * ns, C0, C0 + (size_t)nx * (nz + 1),
+    F[3] = FaceRef{a.v + fo, a.nfv, (uint32_t)c, (uint32_t)(c + nx),
                    ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], true};
     mono = mono && (a.mono_v[fo] & kMonoL);
   }
@@ -469,16 +480,15 @@ __global__ void __launch_bounds__(256) k_apply_flux(FluxArgs a, TileMap tm) {
   const size_t n2 = (size_t)a.nx * a.ny;
   const size_t c = (size_t)ci + (size_t)a.nx * cj;
   const double iaij = 1.0 / a.areaT[c];
-  const size_t tstride = a.tstride;
   int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (int K = 1; K <= nz; ++K) {
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) s0 = face_run<NT>(F[0], s0, ns, K, tstride, acc);
-    if (act[1]) s1 = face_run<NT>(F[1], s1, ns, K, tstride, acc);
-    if (act[2]) s2 = face_run<NT>(F[2], s2, ns, K, tstride, acc);
-    if (act[3]) s3 = face_run<NT>(F[3], s3, ns, K, tstride, acc);
+    if (act[0]) s0 = face_run<NT>(F[0], s0, ns, K, a, n2, acc);
+    if (act[1]) s1 = face_run<NT>(F[1], s1, ns, K, a, n2, acc);
+    if (act[2]) s2 = face_run<NT>(F[2], s2, ns, K, a, n2, acc);
+    if (act[3]) s3 = face_run<NT>(F[3], s3, ns, K, a, n2, acc);
     const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
 #pragma unroll
     for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
@@ -503,10 +513,10 @@ __global__ void __launch_bounds__(256) k_apply_flux_general(FluxArgs a, TileMap 
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) face_scan<NT>(F[0], ns, K, a.tstride, acc);
-    if (act[1]) face_scan<NT>(F[1], ns, K, a.tstride, acc);
-    if (act[2]) face_scan<NT>(F[2], ns, K, a.tstride, acc);
-    if (act[3]) face_scan<NT>(F[3], ns, K, a.tstride, acc);
+    if (act[0]) face_scan<NT>(F[0], ns, K, a, n2, acc);
+    if (act[1]) face_scan<NT>(F[1], ns, K, a, n2, acc);
+    if (act[2]) face_scan<NT>(F[2], ns, K, a, n2, acc);
+    if (act[3]) face_scan<NT>(F[3], ns, K, a, n2, acc);
     const size_t o = (size_t)(nz - K) * n2 + c;
 #pragma unroll
     for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
@@ -530,7 +540,7 @@ __global__ void k_export_coeffs(const FaceRec* __restrict__ rec, int nfa, int nf
   const size_t f = t % nf, s = t / nf;  // writes coalesced along the face index
   const int i = (int)(f % nfa), j = (int)(f / nfa);
   if (i < i0 || i >= i1 || j < j0 || j >= j1) return;
-  const FaceRec r = rec[f * ns + s];
+  const FaceRec r = rec[t];  // surface-major, same order as the output
   out.PoL[t] = r.PoL;
   out.PoR[t] = r.PoR;
   out.KoL[t] = r.KoL;

@@ -77,10 +77,11 @@ Timer g_no_timer;
 constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 
 // Device scratch of one step: Phase A holds ColRec + P per column interface
-// (40 B), Phase B one ReconRec (32 B) per interface and tracer of a group;
-// they are live at different times and share one buffer.
+// (40 B), Phase B a ReconRec (32 B) plus an edge value (8 B) per interface
+// and tracer of a group; they are live at different times and share one
+// buffer.
 size_t workspace_bytes(int nx, int ny, int nz, int nt) {
-  const size_t per = std::max<size_t>(40, (size_t)32 * std::min(std::max(nt, 1), kTracerGroup));
+  const size_t per = (size_t)40 * std::min(std::max(nt, 1), kTracerGroup);
   return per * (size_t)(nz + 1) * nx * ny;
 }
 
@@ -158,7 +159,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
-    SweepArgs a{nx, nz, nx + 1, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
+    SweepArgs a{nx, nz, nx + 1, fs.nu, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
@@ -166,7 +167,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
     tm.end(slot, st);
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
-    SweepArgs a{nx, nz, nx, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
+    SweepArgs a{nx, nz, nx, fs.nv, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
@@ -203,8 +204,11 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
   fa.dy_cu = p.dy_cu; fa.dx_cv = p.dx_cv; fa.idxCu = p.idxCu; fa.idyCv = p.idyCv;
   fa.areaT = p.areaT;
   fa.u = fs.u; fa.v = fs.v; fa.mono_u = fs.mono_u; fa.mono_v = fs.mono_v;
+  // tracer t of a group: ReconRec[nz][N2] at rc + t*rstride, Ti[nz+1][N2] after them
   ReconRec* rc = (ReconRec*)ws;
-  const size_t tstride = (size_t)N2 * (nz + 1);
+  const size_t rstride = (size_t)N2 * nz;
+  double* ti0 = (double*)(rc + (size_t)kTracerGroup * rstride);
+  const size_t tistride = (size_t)N2 * (nz + 1);
   const int bs = tiles.tx * tiles.ty;
   unsigned nb;
   const TileMap m = make_tiles(nx, ny, tiles, &nb);
@@ -213,15 +217,18 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
     const int gn = std::min(G, nt - g0);
     slot = tm.begin(REDI_K_TRACER_RECON, st);
     for (int t = 0; t < gn; ++t)
-      k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t], rc + t * tstride);
+      k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t],
+                                                          rc + t * rstride, ti0 + t * tistride);
     tm.end(slot, st);
     fa.nt = gn;
     for (int t = 0; t < kMaxNT; ++t) {
       const int tt = std::min(t, gn - 1);
-      fa.recon[t] = rc + (size_t)tt * tstride;
+      fa.recon[t] = rc + (size_t)tt * rstride;
+      fa.ti[t] = ti0 + (size_t)tt * tistride;
       fa.htr[t] = tracers[g0 + tt];
     }
-    fa.tstride = tstride;
+    fa.nfu = fs.nu;
+    fa.nfv = fs.nv;
     fa.force_general = tiles.force_general;
     slot = tm.begin(REDI_K_APPLY_FLUX, st);
     switch (gn) {
@@ -354,6 +361,9 @@ int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
   if (nz > REDI_NZ_MAX)
     return fail(REDI_ERR_NZ, "redi_init: 2*nz_ml+2 = %d exceeds the neutral-surface limit %d",
                 2 * nz + 2, 2 * REDI_NZ_MAX + 2);
+  if ((double)(nz + 1) * nx * ny >= 2147483648.0)
+    return fail(REDI_ERR_ARG, "redi_init: %d x %d x %d exceeds 2^31 cell interfaces per domain",
+                nx, ny, nz);
   redi_t* r = new redi_t();
   auto cleanup = [&](int code) {
     free_face_state(&r->fs, true);
@@ -524,6 +534,8 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   if (!eos || !cf || !tracers || nt < 1 || nx < 1 || ny < 1 || nz < 1)
     return fail(REDI_ERR_ARG, "redi_launch_nt: bad argument");
   if (nz > REDI_NZ_MAX) return fail(REDI_ERR_NZ, "redi_launch_nt: nz %d > %d", nz, REDI_NZ_MAX);
+  if ((double)(nz + 1) * nx * ny >= 2147483648.0)
+    return fail(REDI_ERR_ARG, "redi_launch_nt: domain exceeds 2^31 cell interfaces");
   if (idx_T < 1 || idx_T > nt || idx_S < 1 || idx_S > nt)
     return fail(REDI_ERR_ARG, "redi_launch_nt: tracer index out of range");
   if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "redi_launch_nt: unknown EOS variant");
