@@ -124,8 +124,15 @@ int redi_export_coeffs(const redi_t *self, const redi_coeffs_t *dst,
 
 /* Thread-tile shape (tx * ty lanes: 64, 128, 192 or 256) of the sweep and
  * flux kernels: tiles of adjacent faces / cells share columns and v-faces
- * in cache.  Default 16 x 4. */
+ * in cache.  Default 64 x 1. */
 int redi_set_tiles(redi_t *self, int tx, int ty);
+
+/* Storage order of the face records (face_major: each face's surfaces
+ * contiguous, else surface-major) and of the tracer reconstruction
+ * (column_major: each column's layers contiguous, else layer-major).
+ * Default 1, 1.  Changing the face order resets the Phase-A state to its
+ * initial values, so it must be followed by redi_calc_coeffs. */
+int redi_set_layout(redi_t *self, int face_major, int column_major);
 
 /* Tracers handled per flux launch (1..4, default 4): more tracers share one
  * pass over the face records, fewer need fewer registers. */

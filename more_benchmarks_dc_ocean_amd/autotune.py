@@ -26,19 +26,20 @@ from typing import Dict, Iterable, List, Optional, Tuple
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-DEFAULT = {"tx": 16, "ty": 4, "group": 4}
+DEFAULT = {"tx": 64, "ty": 1, "group": 4, "face_major": 1, "col_major": 1}
 
-TILES = [(64, 1), (32, 2), (16, 4), (8, 8), (128, 1), (32, 4), (16, 8), (64, 4), (32, 8),
-         (16, 16)]
-GROUPS = [1, 2, 4]
+TILES = [(64, 1), (16, 4), (8, 8), (128, 1), (64, 2), (32, 4), (256, 1)]
+GROUPS = [2, 4]
+LAYOUTS = [(1, 1), (0, 1), (1, 0), (0, 0)]  # (face_major, col_major)
 
 
 def bucket_of(nx: int, ny: int, nz: int) -> str:
     return f"nz{nz}_a{int(round(math.log2(max(1, nx * ny))))}"
 
 
-def _time_config(r, dom, tx: int, ty: int, group: int, steps: int) -> Tuple[float, Dict]:
+def _time_config(r, dom, tx: int, ty: int, group: int, layout, steps: int) -> Tuple[float, Dict]:
     import torch
+    r.set_layout(*layout)
     r.set_tiles(tx, ty)
     r.set_tracer_group(group)
     r.step(dom, 1)  # warm the new shape
@@ -52,24 +53,30 @@ def _time_config(r, dom, tx: int, ty: int, group: int, steps: int) -> Tuple[floa
     return total, {k: ms / steps for k, (ms, _) in kt.items()}
 
 
+KEYS = ("tx", "ty", "group", "face_major", "col_major")
+
+
 def tune(case, device: str = "cuda:0", tiles: Iterable = TILES, groups: Iterable = GROUPS,
-         steps: int = 2, log=None) -> Dict:
-    """Time every (tile, group) on ``case``; returns the winner and the table."""
+         layouts: Iterable = LAYOUTS, steps: int = 2, log=None) -> Dict:
+    """Time every (layout, tile, group) on ``case``; returns the winner and
+    the table."""
     from .redi import make_redi
     r, dom = make_redi(case, device)
     rows: List[Dict] = []
-    for (tx, ty) in tiles:
-        for g in groups:
-            if g > case.nt:
-                continue
-            ms, per = _time_config(r, dom, tx, ty, g, steps)
-            rows.append({"tx": tx, "ty": ty, "group": g, "ms_per_step": ms, "kernels": per})
-            if log:
-                log(f"tile {tx:3d}x{ty:<3d} group {g}: {ms:9.3f} ms/step  " +
-                    " ".join(f"{k}={v:.2f}" for k, v in sorted(per.items())))
+    for lay in layouts:
+        for (tx, ty) in tiles:
+            for g in groups:
+                if g > case.nt and g != min(groups):
+                    continue
+                ms, per = _time_config(r, dom, tx, ty, g, lay, steps)
+                rows.append({"tx": tx, "ty": ty, "group": g, "face_major": lay[0],
+                             "col_major": lay[1], "ms_per_step": ms, "kernels": per})
+                if log:
+                    log(f"layout f{lay[0]}c{lay[1]} tile {tx:3d}x{ty:<3d} group {g}: {ms:9.3f} "
+                        "ms/step  " + " ".join(f"{k}={v:.2f}" for k, v in sorted(per.items())))
     r.close()
     best = min(rows, key=lambda d: d["ms_per_step"])
-    return {"bucket": bucket_of(case.nx, case.ny, case.nz), "best": {k: best[k] for k in ("tx", "ty", "group")},
+    return {"bucket": bucket_of(case.nx, case.ny, case.nz), "best": {k: best[k] for k in KEYS},
             "best_ms_per_step": best["ms_per_step"], "table": rows,
             "domain": [case.nx, case.ny, case.nz, case.nt]}
 
@@ -106,7 +113,11 @@ class TuneCache:
         return dict(DEFAULT)
 
     def apply(self, r, nx: int, ny: int, nz: int) -> Dict:
-        cfg = self.lookup(nx, ny, nz)
+        """Set the bucket's launch shape on an OceanRedi (before its first
+        redi_calc_coeffs: a face-order change resets the Phase-A state)."""
+        cfg = dict(DEFAULT)
+        cfg.update(self.lookup(nx, ny, nz))
+        r.set_layout(cfg["face_major"], cfg["col_major"])
         r.set_tiles(cfg["tx"], cfg["ty"])
         r.set_tracer_group(cfg["group"])
         return cfg

@@ -16,16 +16,21 @@
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
 //   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
 //                               kt = top-down interface 0..nz
-//   tracer recon (K3 -> K4)     layer-major per tracer: ReconRec[kt][col] and
-//                               the interface edges Ti[kt][col]
-//   face coefficients (K2->K4)  surface-major: FaceRec[s][face], s = 0..ns-1,
-//                               plus one monotonicity flag byte per face
+//   tracer recon (K3 -> K4)     ReconRec (layer mean, limited edges, top edge)
+//                               per layer and column, and the interface edges
+//                               Ti per interface and column; element (k, col)
+//                               at col*ca + k*cb: per column (ca = depth,
+//                               cb = 1) or layer-major (ca = 1, cb = N2)
+//   face coefficients (K2->K4)  FaceRec per face and neutral surface s;
+//                               record (face, s) at face*fa + s*fb: per face
+//                               (fa = ns, fb = 1) or surface-major (fa = 1,
+//                               fb = faces); one monotonicity flag per face
 // The sweep reads each of its two columns once, front to back, while lanes
-// sit at unrelated depths, so the column state is stored per column and each
-// lane streams whole 128-byte lines of its own columns.  The flux kernel's
-// lanes gather at similar depths of neighbouring columns, and every sweep
-// lane writes surface s at the same time, so those arrays are layer- and
-// surface-major and the lanes of a tile row share lines.
+// sit at unrelated depths, so the column state is always stored per column
+// and each lane streams whole 128-byte lines of its own columns.  For the
+// face records and the reconstruction both orders are supported: per-lane
+// contiguity gives each lane reuse of its own lines, the major orders let
+// the lanes of a tile row share lines; the autotuner picks per domain size.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -149,7 +154,7 @@ struct SweepSide {
 struct SweepArgs {
   int nx, nz;
   int nfa;                 // faces per row of the face array
-  size_t nf;               // faces in the face array (record stride between surfaces)
+  uint32_t fa, fb;         // record (face, s) at face*fa + s*fb
   int i0, j0;              // first computed face
   int di, dj;              // left column = (i - di, j - dj)
   const double* wet;
@@ -166,8 +171,8 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
   const int i = a.i0 + fi, j = a.j0 + fj;
   const size_t fo = (size_t)i + (size_t)a.nfa * j;
   const int nk = a.nz, ns = 2 * nk + 2;
-  FaceRec* o = a.out + fo;
-  const size_t fst = a.nf;
+  FaceRec* o = a.out + fo * a.fa;
+  const size_t fst = a.fb;
   if (!(a.wet[fo] > 0.0)) {  // dry face: the inert values of :722-730
     for (int s = 0; s < ns; ++s) o[s * fst] = FaceRec{0.0, 0.0, 0.0, 1, 1};
     a.mono[fo] = kMonoL | kMonoR;
@@ -295,7 +300,8 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
 __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
                                                       const double* __restrict__ q,
                                                       ReconRec* __restrict__ rec,
-                                                      double* __restrict__ ti) {
+                                                      double* __restrict__ ti, uint32_t ca,
+                                                      uint32_t cb, uint32_t ta, uint32_t tb) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
@@ -311,9 +317,9 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
       const double tl = s_above[0];
       double al = e_prev, ar = e[0];
       ppm_limit<double>(tl, al, ar);
-      rec[(size_t)(k - 1) * n2 + c] = ReconRec{tl, al, ar, e_prev};
+      rec[(size_t)c * ca + (size_t)(k - 1) * cb] = ReconRec{tl, al, ar, e_prev};
     }
-    ti[(size_t)k * n2 + c] = e[0];
+    ti[(size_t)c * ta + (size_t)k * tb] = e[0];
     e_prev = e[0];
   };
   stream_edges<double, 1>(nz, load, emit);
@@ -328,16 +334,22 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
 // the left / right columns (layer-major with stride n2)
 // (32-bit element offsets from uniform bases: the host checks that every
 // layer-major scratch array holds fewer than 2^31 elements)
+struct ReconLayout {
+  uint32_t ca, cb;  // ReconRec (k, col) at col*ca + k*cb
+  uint32_t ta, tb;  // Ti (k, col) at col*ta + k*tb
+};
+
 __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
                                               double PoLb, double PoRt, double PoRb,
                                               const ReconRec* __restrict__ rec,
                                               const double* __restrict__ ti, uint32_t cl,
-                                              uint32_t cr, uint32_t n2) {
-  const ReconRec lt = rec[(uint32_t)(klt - 1) * n2 + cl], rt = rec[(uint32_t)(krt - 1) * n2 + cr];
-  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * ti[(uint32_t)klt * n2 + cl];
-  const double tlb = (1.0 - PoLb) * ti[(uint32_t)(klb - 1) * n2 + cl] + PoLb * ti[(uint32_t)klb * n2 + cl];
-  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * ti[(uint32_t)krt * n2 + cr];
-  const double trb = (1.0 - PoRb) * ti[(uint32_t)(krb - 1) * n2 + cr] + PoRb * ti[(uint32_t)krb * n2 + cr];
+                                              uint32_t cr, const ReconLayout& y) {
+  const uint32_t rl = cl * y.ca, rr = cr * y.ca, il = cl * y.ta, ir = cr * y.ta;
+  const ReconRec lt = rec[rl + (uint32_t)(klt - 1) * y.cb], rt = rec[rr + (uint32_t)(krt - 1) * y.cb];
+  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * ti[il + (uint32_t)klt * y.tb];
+  const double tlb = (1.0 - PoLb) * ti[il + (uint32_t)(klb - 1) * y.tb] + PoLb * ti[il + (uint32_t)klb * y.tb];
+  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * ti[ir + (uint32_t)krt * y.tb];
+  const double trb = (1.0 - PoRb) * ti[ir + (uint32_t)(krb - 1) * y.tb] + PoRb * ti[ir + (uint32_t)krb * y.tb];
   const double tlay = ppm_ave<double>(PoLt, PoLb + (double)(klb - klt), lt.aL, lt.aR, lt.Tl);
   const double trlay = ppm_ave<double>(PoRt, PoRb + (double)(krb - krt), rt.aL, rt.aR, rt.Tl);
   const double dT_top = trt - tlt;
@@ -362,7 +374,8 @@ struct FluxArgs {
   const uint8_t *mono_u, *mono_v;
   int nt;                          // tracers in this launch (<= NT)
   int force_general;               // take the order-general path for every cell
-  size_t nfu, nfv;                 // u / v faces (record stride between surfaces)
+  uint32_t fau, fbu, fav, fbv;     // u / v face record (face, s) at face*fa + s*fb
+  ReconLayout ry;
   const ReconRec* recon[kMaxNT];   // per tracer, [kt][col]
   const double* ti[kMaxNT];        // per tracer, [kt][col], kt = 0..nz
   double* htr[kMaxNT];
@@ -382,20 +395,23 @@ struct FaceRef {
 template <int NT>
 __device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, const FluxArgs& a,
                                         size_t n2, double* acc) {
-  while (s < ns - 1) {
-    const FaceRec top = f.rec[(size_t)s * f.fst];
+  if (s >= ns - 1) return s;
+  FaceRec top = f.rec[(size_t)s * f.fst];
+  while (true) {
+    if (top.hEff != 0.0 && (f.left ? top.KoL : top.KoR) != K) break;
+    const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
     if (top.hEff != 0.0) {
-      if ((f.left ? top.KoL : top.KoR) != K) break;
-      const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                     bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, (uint32_t)n2);
+                                     bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, a.ry);
         const double flx = (d * top.hEff) * f.coef;
         acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
       }
     }
     ++s;
+    if (s >= ns - 1) break;
+    top = bot;
   }
   return s;
 }
@@ -412,7 +428,7 @@ __device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, const
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                   bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, (uint32_t)n2);
+                                   bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, a.ry);
       const double flx = (d * top.hEff) * f.coef;
       acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
     }
@@ -433,7 +449,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[0] = i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e));
   if (act[0]) {
     const size_t fo = (size_t)(i - 1) + (size_t)(nx + 1) * (j - 1);
-    F[0] = FaceRef{a.u + fo, a.nfu, (uint32_t)(c - 1), (uint32_t)c,
+    F[0] = FaceRef{a.u + fo * a.fau, a.fbu, (uint32_t)(c - 1), (uint32_t)c,
                    ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], false};
     mono = mono && (a.mono_u[fo] & kMonoR);
   }
@@ -441,7 +457,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[1] = i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e));
   if (act[1]) {
     const size_t fo = (size_t)i + (size_t)(nx + 1) * (j - 1);
-    F[1] = FaceRef{a.u + fo, a.nfu, (uint32_t)c, (uint32_t)(c + 1),
+    F[1] = FaceRef{a.u + fo * a.fau, a.fbu, (uint32_t)c, (uint32_t)(c + 1),
                    ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], true};
     mono = mono && (a.mono_u[fo] & kMonoL);
   }
@@ -449,7 +465,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[2] = j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n));
   if (act[2]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * (j - 1);
-    F[2] = FaceRef{a.v + fo, a.nfv, (uint32_t)(c - nx), (uint32_t)c,
+    F[2] = FaceRef{a.v + fo * a.fav, a.fbv, (uint32_t)(c - nx), (uint32_t)c,
                    ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], false};
     mono = mono && (a.mono_v[fo] & kMonoR);
   }
@@ -457,7 +473,7 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
   act[3] = j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n));
   if (act[3]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * j;
-    F[3] = FaceRef{a.v + fo, a.nfv, (uint32_t)c, (uint32_t)(c + nx),
+    F[3] = FaceRef{a.v + fo * a.fav, a.fbv, (uint32_t)c, (uint32_t)(c + nx),
                    ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], true};
     mono = mono && (a.mono_v[fo] & kMonoL);
   }
@@ -532,15 +548,15 @@ struct CoeffArrays {  // the reference layout: [s][face] (Fortran (nfa,nfb,ns))
 };
 
 // Faces (i, j), i in [i0, i1), j in [j0, j1) of an nfa-wide face array.
-__global__ void k_export_coeffs(const FaceRec* __restrict__ rec, int nfa, int nfb, int ns, int i0,
-                                int i1, int j0, int j1, CoeffArrays out) {
+__global__ void k_export_coeffs(const FaceRec* __restrict__ rec, uint32_t fa, uint32_t fb, int nfa,
+                                int nfb, int ns, int i0, int i1, int j0, int j1, CoeffArrays out) {
   const size_t nf = (size_t)nfa * nfb;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nf * ns) return;
   const size_t f = t % nf, s = t / nf;  // writes coalesced along the face index
   const int i = (int)(f % nfa), j = (int)(f / nfa);
   if (i < i0 || i >= i1 || j < j0 || j >= j1) return;
-  const FaceRec r = rec[t];  // surface-major, same order as the output
+  const FaceRec r = rec[f * fa + s * fb];
   out.PoL[t] = r.PoL;
   out.PoR[t] = r.PoR;
   out.KoL[t] = r.KoL;

@@ -133,10 +133,18 @@ int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
 }
 
 struct Tiles {
-  int tx = 16, ty = 4;  // one 64-lane wavefront per tile
+  int tx = 64, ty = 1;       // one 64-lane wavefront per tile
   int force_general = 0;
   int group = kTracerGroup;  // tracers per flux launch (1..4)
+  int face_major = 1;        // face records per face (1) or surface-major (0)
+  int col_major = 1;         // reconstruction per column (1) or layer-major (0)
 };
+
+// record (face, s) at face*fa + s*fb
+void face_strides(const Tiles& t, size_t nfaces, int nz, uint32_t* fa, uint32_t* fb) {
+  const uint32_t ns = 2 * (uint32_t)nz + 2;
+  if (t.face_major) { *fa = ns; *fb = 1; } else { *fa = 1; *fb = (uint32_t)nfaces; }
+}
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
   TileMap m;
@@ -159,7 +167,8 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
-    SweepArgs a{nx, nz, nx + 1, fs.nu, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
+    SweepArgs a{nx, nz, nx + 1, 0, 0, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
+    face_strides(tiles, fs.nu, nz, &a.fa, &a.fb);
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
@@ -167,7 +176,8 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
     tm.end(slot, st);
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
-    SweepArgs a{nx, nz, nx, fs.nv, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
+    SweepArgs a{nx, nz, nx, 0, 0, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
+    face_strides(tiles, fs.nv, nz, &a.fa, &a.fb);
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
@@ -204,7 +214,13 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
   fa.dy_cu = p.dy_cu; fa.dx_cv = p.dx_cv; fa.idxCu = p.idxCu; fa.idyCv = p.idyCv;
   fa.areaT = p.areaT;
   fa.u = fs.u; fa.v = fs.v; fa.mono_u = fs.mono_u; fa.mono_v = fs.mono_v;
-  // tracer t of a group: ReconRec[nz][N2] at rc + t*rstride, Ti[nz+1][N2] after them
+  // tracer t of a group: nz*N2 ReconRec at rc + t*rstride, (nz+1)*N2 Ti after them
+  if (tiles.col_major)
+    fa.ry = ReconLayout{(uint32_t)nz, 1, (uint32_t)nz + 1, 1};
+  else
+    fa.ry = ReconLayout{1, (uint32_t)N2, 1, (uint32_t)N2};
+  face_strides(tiles, fs.nu, nz, &fa.fau, &fa.fbu);
+  face_strides(tiles, fs.nv, nz, &fa.fav, &fa.fbv);
   ReconRec* rc = (ReconRec*)ws;
   const size_t rstride = (size_t)N2 * nz;
   double* ti0 = (double*)(rc + (size_t)kTracerGroup * rstride);
@@ -218,7 +234,8 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
     slot = tm.begin(REDI_K_TRACER_RECON, st);
     for (int t = 0; t < gn; ++t)
       k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t],
-                                                          rc + t * rstride, ti0 + t * tistride);
+                                                          rc + t * rstride, ti0 + t * tistride,
+                                                          fa.ry.ca, fa.ry.cb, fa.ry.ta, fa.ry.tb);
     tm.end(slot, st);
     fa.nt = gn;
     for (int t = 0; t < kMaxNT; ++t) {
@@ -227,8 +244,7 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
       fa.ti[t] = ti0 + (size_t)tt * tistride;
       fa.htr[t] = tracers[g0 + tt];
     }
-    fa.nfu = fs.nu;
-    fa.nfv = fs.nv;
+
     fa.force_general = tiles.force_general;
     slot = tm.begin(REDI_K_APPLY_FLUX, st);
     switch (gn) {
@@ -255,17 +271,20 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
 // all_faces: every face (the reference's init values where Phase A never
 // writes); otherwise only the faces redi_calc_coeffs_x/_y compute.
 int export_coeffs(const FaceState& fs, int nx, int ny, int nz, const redi_coeffs_t& d,
-                  bool all_faces, bool with_khtr, hipStream_t st) {
+                  bool all_faces, bool with_khtr, const Tiles& tiles, hipStream_t st) {
   const int ns = 2 * nz + 2;
   const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  uint32_t ua, ub, va, vb;
+  face_strides(tiles, nu, nz, &ua, &ub);
+  face_strides(tiles, nv, nz, &va, &vb);
   CoeffArrays cu{d.uPoL, d.uPoR, d.uhEff, d.uKoL, d.uKoR};
   CoeffArrays cv{d.vPoL, d.vPoR, d.vhEff, d.vKoL, d.vKoR};
   if (all_faces) {
-    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, nx + 1, ny, ns, 0, nx + 1, 0, ny, cu);
-    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, nx, ny + 1, ns, 0, nx, 0, ny + 1, cv);
+    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, ua, ub, nx + 1, ny, ns, 0, nx + 1, 0, ny, cu);
+    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, va, vb, nx, ny + 1, ns, 0, nx, 0, ny + 1, cv);
   } else {
-    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, nx + 1, ny, ns, 1, nx, 0, ny, cu);
-    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, nx, ny + 1, ns, 0, nx, 1, ny, cv);
+    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, ua, ub, nx + 1, ny, ns, 1, nx, 0, ny, cu);
+    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, va, vb, nx, ny + 1, ns, 0, nx, 1, ny, cv);
   }
   if (with_khtr) {
     if (d.khtr_u) HIP_TRY(hipMemcpyAsync(d.khtr_u, fs.khtr_u, nu * 8, hipMemcpyDeviceToDevice, st));
@@ -425,6 +444,19 @@ int redi_set_tracer_group(redi_t* r, int n) {
   return REDI_OK;
 }
 
+int redi_set_layout(redi_t* r, int face_major, int column_major) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_layout: null handle");
+  const int fm = face_major ? 1 : 0, cm = column_major ? 1 : 0;
+  if (fm != r->tiles.face_major) {  // stored records change order: back to init values
+    r->tiles.face_major = fm;
+    int rc = fill_face_state(r->fs, r->nz, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  r->tiles.col_major = cm;
+  return REDI_OK;
+}
+
 int redi_set_flux_path(redi_t* r, int general) {
   if (!r) return fail(REDI_ERR_ARG, "redi_set_flux_path: null handle");
   r->tiles.force_general = general ? 1 : 0;
@@ -448,7 +480,7 @@ int redi_export_coeffs(const redi_t* r, const redi_coeffs_t* dst, void* stream) 
   if (!dst->uPoL || !dst->uPoR || !dst->uKoL || !dst->uKoR || !dst->uhEff || !dst->vPoL ||
       !dst->vPoR || !dst->vKoL || !dst->vKoR || !dst->vhEff)
     return fail(REDI_ERR_ARG, "redi_export_coeffs: incomplete destination arrays");
-  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
+  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, r->tiles, (hipStream_t)stream);
 }
 
 int redi_calc_coeffs(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_t* eos,
@@ -551,7 +583,7 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   rc = launch_phase_a(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1], tracers[idx_S - 1],
                       wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
   if (rc) return rc;
-  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, st);
+  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, tiles, st);
   if (rc) return rc;
   PhaseB p;
   p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;

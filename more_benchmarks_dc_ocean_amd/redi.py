@@ -73,7 +73,7 @@ COEFF_NAMES = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", 
 # every extern "C" symbol declared in include/redi_hip.h
 EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs", "redi_set_tiles",
             "redi_state_bytes", "redi_set_flux_path", "redi_nonmonotone_faces",
-            "redi_set_tracer_group",
+            "redi_set_tracer_group", "redi_set_layout",
             "redi_calc_coeffs", "redi_apply_flux", "redi_apply_flux_hoist", "redi_step",
             "redi_workspace_bytes", "redi_last_error", "redi_version", "redi_opt_launch",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_tiles.argtypes = [_P, _I, _I]
         L.redi_set_flux_path.argtypes = [_P, _I]
         L.redi_set_tracer_group.argtypes = [_P, _I]
+        L.redi_set_layout.argtypes = [_P, _I, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
                                              ctypes.POINTER(ctypes.c_longlong)]
         L.redi_state_bytes.restype = ctypes.c_longlong
@@ -292,6 +293,12 @@ class OceanRedi:
         _check(lib().redi_nonmonotone_faces(self.handle, ctypes.byref(nu), ctypes.byref(nv)),
                "redi_nonmonotone_faces")
         return nu.value, nv.value
+
+    def set_layout(self, face_major: bool, column_major: bool) -> None:
+        """Storage order of the face records and of the reconstruction;
+        changing the face order resets the Phase-A state."""
+        _check(lib().redi_set_layout(self.handle, int(bool(face_major)), int(bool(column_major))),
+               "redi_set_layout")
 
     def set_tracer_group(self, n: int) -> None:
         _check(lib().redi_set_tracer_group(self.handle, int(n)), "redi_set_tracer_group")
