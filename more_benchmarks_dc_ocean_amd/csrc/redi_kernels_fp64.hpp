@@ -395,12 +395,11 @@ struct FaceRef {
 template <int NT>
 __device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, const FluxArgs& a,
                                         size_t n2, double* acc) {
-  if (s >= ns - 1) return s;
-  FaceRec top = f.rec[(size_t)s * f.fst];
-  while (true) {
-    if (top.hEff != 0.0 && (f.left ? top.KoL : top.KoR) != K) break;
-    const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
+  while (s < ns - 1) {
+    const FaceRec top = f.rec[(size_t)s * f.fst];
     if (top.hEff != 0.0) {
+      if ((f.left ? top.KoL : top.KoR) != K) break;
+      const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
@@ -410,8 +409,6 @@ __device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, 
       }
     }
     ++s;
-    if (s >= ns - 1) break;
-    top = bot;
   }
   return s;
 }
