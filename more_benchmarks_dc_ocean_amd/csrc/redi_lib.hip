@@ -76,12 +76,16 @@ Timer g_no_timer;
 
 constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 
-// Device scratch of one step: Phase A holds ColRec + P per column interface
-// (40 B), Phase B a ReconRec (32 B) plus an edge value (8 B) per interface
-// and tracer of a group; they are live at different times and share one
-// buffer.
+// Tracer slots of the Phase-B scratch for an nt-tracer state.
+int recon_slots(int nt) { return std::min(std::max(nt, 1), kTracerGroup); }
+
+// Device scratch of one step.  Phase A holds ColRec + P per column interface
+// (40 B).  Phase B holds, per tracer slot, a ReconRec (32 B) per layer and an
+// edge value (8 B) per interface, then one discard layer field (8 B per
+// cell) that takes the second output of a two-tracer flux launch run for a
+// lone tracer.  The phases are live at different times and share one buffer.
 size_t workspace_bytes(int nx, int ny, int nz, int nt) {
-  const size_t per = (size_t)40 * std::min(std::max(nt, 1), kTracerGroup);
+  const size_t per = (size_t)40 * recon_slots(nt) + 8;
   return per * (size_t)(nz + 1) * nx * ny;
 }
 
@@ -214,21 +218,24 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
   fa.dy_cu = p.dy_cu; fa.dx_cv = p.dx_cv; fa.idxCu = p.idxCu; fa.idyCv = p.idyCv;
   fa.areaT = p.areaT;
   fa.u = fs.u; fa.v = fs.v; fa.mono_u = fs.mono_u; fa.mono_v = fs.mono_v;
-  // tracer t of a group: nz*N2 ReconRec at rc + t*rstride, (nz+1)*N2 Ti after them
+  // tracer slot t: nz*N2 ReconRec at rc + t*rstride; (nz+1)*N2 Ti per slot
+  // after every slot's records; then the nz*N2 discard field
   if (tiles.col_major)
     fa.ry = ReconLayout{(uint32_t)nz, 1, (uint32_t)nz + 1, 1};
   else
     fa.ry = ReconLayout{1, (uint32_t)N2, 1, (uint32_t)N2};
   face_strides(tiles, fs.nu, nz, &fa.fau, &fa.fbu);
   face_strides(tiles, fs.nv, nz, &fa.fav, &fa.fbv);
+  const int slots = recon_slots(nt);
   ReconRec* rc = (ReconRec*)ws;
   const size_t rstride = (size_t)N2 * nz;
-  double* ti0 = (double*)(rc + (size_t)kTracerGroup * rstride);
+  double* ti0 = (double*)(rc + (size_t)slots * rstride);
   const size_t tistride = (size_t)N2 * (nz + 1);
+  double* discard = ti0 + (size_t)slots * tistride;
   const int bs = tiles.tx * tiles.ty;
   unsigned nb;
   const TileMap m = make_tiles(nx, ny, tiles, &nb);
-  const int G = std::max(1, std::min(tiles.group, kTracerGroup));
+  const int G = std::max(1, std::min(tiles.group, slots));
   for (int g0 = 0; g0 < nt; g0 += G) {
     const int gn = std::min(G, nt - g0);
     slot = tm.begin(REDI_K_TRACER_RECON, st);
@@ -244,19 +251,23 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
       fa.ti[t] = ti0 + (size_t)tt * tistride;
       fa.htr[t] = tracers[g0 + tt];
     }
-
+    // A lone tracer goes through the two-tracer kernels: slot 1 repeats its
+    // reconstruction and its result lands in the discard field.  (The
+    // one-tracer instantiation gave results that differ from the oracle at
+    // -O1 and above while its source is exact in a thread-by-thread CPU
+    // build and at -O0, so it is not built.)
+    if (gn == 1) fa.htr[1] = discard;
+    const int kn = gn == 1 ? 2 : gn;
     fa.force_general = tiles.force_general;
     slot = tm.begin(REDI_K_APPLY_FLUX, st);
-    switch (gn) {
-      case 1: k_apply_flux<1><<<nb, bs, 0, st>>>(fa, m); break;
+    switch (kn) {
       case 2: k_apply_flux<2><<<nb, bs, 0, st>>>(fa, m); break;
       case 3: k_apply_flux<3><<<nb, bs, 0, st>>>(fa, m); break;
       default: k_apply_flux<4><<<nb, bs, 0, st>>>(fa, m); break;
     }
     tm.end(slot, st);
     slot = tm.begin(REDI_K_APPLY_FLUX_GENERAL, st);
-    switch (gn) {
-      case 1: k_apply_flux_general<1><<<nb, bs, 0, st>>>(fa, m); break;
+    switch (kn) {
       case 2: k_apply_flux_general<2><<<nb, bs, 0, st>>>(fa, m); break;
       case 3: k_apply_flux_general<3><<<nb, bs, 0, st>>>(fa, m); break;
       default: k_apply_flux_general<4><<<nb, bs, 0, st>>>(fa, m); break;

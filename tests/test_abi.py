@@ -54,9 +54,9 @@ def test_library_loads_and_host_only_calls(libpath):
     for n in declared_functions():
         assert hasattr(L, n)
     assert L.redi_version() >= 100
-    # one 32-byte record per interface, column and tracer of a 4-tracer
-    # group (128 B) for a 10x8x4 domain
-    assert L.redi_workspace_bytes(10, 8, 4) == 128 * 5 * 10 * 8
+    # per column interface of a 10x8x4 domain: four tracer slots of a 32-B
+    # reconstruction record plus an 8-B edge, and the 8-B discard field
+    assert L.redi_workspace_bytes(10, 8, 4) == (4 * 40 + 8) * 5 * 10 * 8
     assert L.redi_workspace_bytes(0, 8, 4) == 0
     assert L.redi_state_bytes(10, 8, 4) > L.redi_workspace_bytes(10, 8, 4)
     assert isinstance(redi.last_error(), str)

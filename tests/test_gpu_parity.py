@@ -142,6 +142,22 @@ def test_hip_launch_shapes_same_bits(tiles, group, layout, gpu_device):
     _compare(got, ref, f"roquet_two_fronts_mini tiles={tiles} group={group} layout={layout}")
 
 
+@pytest.mark.parametrize("nt", [1, 2, 3])
+@pytest.mark.parametrize("group", [1, 2, 3, 4])
+def test_hip_tracer_counts_and_groups(nt, group, gpu_device):
+    """1..3 tracers under every tracer grouping, including groups of one
+    tracer (run through the two-tracer kernels with the second result
+    discarded) and the Phase-B scratch sized for fewer than four tracers."""
+    import dataclasses
+    base, _ = load_golden("roquet_two_fronts_mini")
+    case = dataclasses.replace(base, hTr=base.hTr[:nt].copy(), idx_T=1, idx_S=min(2, nt),
+                               name=f"roquet_two_fronts_mini_nt{nt}")
+    o = redi_oracle.run_oracle(case, nsteps=1)
+    for general in (False, True):
+        got = _run_hip(case, 1, gpu_device, group=group, general=general)
+        assert np.array_equal(got["hTr1"], o["hTr1"]), (nt, group, general)
+
+
 def test_nonmonotone_faces_reported(gpu_device):
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     case = inputs.case_front(nsteps=1)
