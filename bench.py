@@ -50,40 +50,6 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(case, kernel: str) -> float:
-    """Compulsory HBM bytes of one launch of ``kernel`` under the byte model
-    of the public two-call contract: h read once per step; each tracer read
-    and written once; the Phase-A coefficients written once and read back
-    once; the 2-D metrics read once.  A per-tracer Phase-B launch carries one
-    tracer's read+write and a 1/nt share of the coefficient read-back and of
-    the metrics.  FP64: 8-byte values, 4-byte Ko."""
-    nx, ny, nz, nt = case.nx, case.ny, case.nz, case.nt
-    ns = 2 * nz + 2
-    n3 = nx * ny * nz
-    nu, nv, n2 = (nx + 1) * ny, nx * (ny + 1), nx * ny
-    coeff = (nu + nv) * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
-    metrics = 8 * (4 * nu + 4 * nv + n2)
-    if kernel == "k_apply_flux":
-        return 2 * 8 * n3 + (coeff + metrics) / nt
-    if kernel == "k_tracer_recon":
-        return 0.0
-    if kernel == "k_column_state":
-        return 3 * 8 * n3
-    if kernel in ("k_neutral_sweep_u", "k_neutral_sweep_v"):
-        nf = nu if kernel.endswith("_u") else nv
-        return nf * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
-    return 0.0
-
-
-def step_algorithmic_bytes(case) -> float:
-    nx, ny, nz, nt = case.nx, case.ny, case.nz, case.nt
-    ns = 2 * nz + 2
-    n3 = nx * ny * nz
-    nu, nv, n2 = (nx + 1) * ny, nx * (ny + 1), nx * ny
-    return (8 * n3 + 2 * nt * 8 * n3 + 2 * (nu + nv) * (ns * (2 * 8 + 2 * 4) + (ns - 1) * 8)
-            + 8 * (4 * nu + 4 * nv + n2))
-
-
 def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 counter summary."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
@@ -147,7 +113,8 @@ def main() -> int:
         return 2
 
     import torch
-    from more_benchmarks_dc_ocean_amd import inputs
+    from more_benchmarks_dc_ocean_amd import byte_model, inputs
+    from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
     from more_benchmarks_dc_ocean_amd.redi import make_redi
 
     builder, wname = WORKLOADS[args.workload]
@@ -157,6 +124,8 @@ def main() -> int:
         f"land={case.land_fraction:.3f}) in {time.time() - t0:.1f}s")
     dev = "cuda:0"
     r, dom = make_redi(case, dev)
+    shape = TuneCache(TUNED).apply(r, case.nx, case.ny, case.nz)
+    log(f"launch shape {shape}")
     stream = torch.cuda.current_stream(dev)
     for _ in range(args.warmup):
         r.step(dom, 1)
@@ -184,13 +153,16 @@ def main() -> int:
 
     dom_k = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_ms = kt[dom_k][0] / kt[dom_k][1]
-    alg = algorithmic_bytes(case, dom_k)
+    alg = byte_model.launch_bytes(case, dom_k, shape["group"])
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    step_gbs = byte_model.step_bytes(case) * args.steps / (ms_total * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(dom_k, args.workload),
             "alg_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 4),
-            "step_alg_GBps": round(step_algorithmic_bytes(case) * args.steps / (ms_total * 1e-3) / 1e9, 2)}
+            "launches_per_step": kt[dom_k][1] // args.steps,
+            "step_alg_bytes_per_cell": round(byte_model.bytes_per_cell_step(case), 2),
+            "step_alg_GBps": round(step_gbs, 2), "step_frac": round(step_gbs / HBM_PEAK_GBS, 5)}
     r.close()
     del dom
     torch.cuda.empty_cache()
@@ -210,7 +182,7 @@ def main() -> int:
             "config": {"workload": wname, "nx_phys": case.nx_phys, "ny_phys": case.ny_phys,
                        "nz": case.nz, "ntracers": case.nt, "eos": "roquet_spv",
                        "land_fraction": round(case.land_fraction, 4), "form": "fp64",
-                       "parallelism": "single_gpu"},
+                       "parallelism": "single_gpu", "launch_shape": shape},
             "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     return 0

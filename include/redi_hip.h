@@ -195,8 +195,9 @@ long long redi_state_bytes(int nx_total, int ny_total, int nz);
 #define REDI_K_SWEEP_V 2
 #define REDI_K_FACE_KHTR 3
 #define REDI_K_TRACER_RECON 4
-#define REDI_K_APPLY_FLUX 5
-#define REDI_K_APPLY_FLUX_GENERAL 6
+#define REDI_K_CELL_FLUX 5          /* flux divergence, monotone cells */
+#define REDI_K_CELL_FLUX_GENERAL 6  /* flux divergence, remaining cells */
+#define REDI_K_FACE_FLUX 7          /* sublayer fluxes per face */
 typedef struct {
     double ms[REDI_NKERNELS];           /* summed kernel time */
     long long launches[REDI_NKERNELS];  /* launches timed */

@@ -25,6 +25,8 @@ import time
 from typing import Dict, Iterable, List, Optional, Tuple
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the committed winners, measured on MI355X
+TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_shapes.json")
 
 DEFAULT = {"tx": 64, "ty": 1, "group": 4, "face_major": 1, "col_major": 1}
 

@@ -6,31 +6,34 @@
 //                    surface sweep over the two adjacent columns.
 //   k_tracer_recon   Phase B, once per column and tracer: layer averages,
 //                    PPM edges and limited left/right edge values.
-//   k_apply_flux     Phase B, once per cell for up to NT tracers: the four
-//                    bounding faces' sublayer fluxes accumulated into the
-//                    cell's layers in the reference's order, then
-//                    hTr += dTr / areaT.
+//   k_face_flux      Phase B, once per face for up to NT tracers: the flux
+//                    of every contributing sublayer and its target layers.
+//   k_cell_flux      Phase B, once per cell: the four bounding faces'
+//                    sublayer fluxes added into the cell's layers in the
+//                    reference's order, then hTr += dTr / areaT.
 //   k_export_coeffs  face records -> the reference's Po/Ko/hEff arrays.
 //
 // Device memory layout:
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
 //   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
 //                               kt = top-down interface 0..nz
+//   face coefficients (K2->K4)  FaceRec per face and neutral surface s;
+//                               record (face, s) at face*fa + s*fb: per face
+//                               (fa = ns, fb = 1) or surface-major (fa = 1,
+//                               fb = faces); one monotonicity flag per face
 //   tracer recon (K3 -> K4)     ReconRec (layer mean, limited edges, top edge)
 //                               per layer and column, and the interface edges
 //                               Ti per interface and column; element (k, col)
 //                               at col*ca + k*cb: per column (ca = depth,
 //                               cb = 1) or layer-major (ca = 1, cb = N2)
-//   face coefficients (K2->K4)  FaceRec per face and neutral surface s;
-//                               record (face, s) at face*fa + s*fb: per face
-//                               (fa = ns, fb = 1) or surface-major (fa = 1,
-//                               fb = faces); one monotonicity flag per face
+//   sublayer fluxes (K4 -> K5)  surface-major: packed targets [s][face] and
+//                               fluxes [s][face][tracer]
 // The sweep reads each of its two columns once, front to back, while lanes
 // sit at unrelated depths, so the column state is always stored per column
-// and each lane streams whole 128-byte lines of its own columns.  For the
-// face records and the reconstruction both orders are supported: per-lane
-// contiguity gives each lane reuse of its own lines, the major orders let
-// the lanes of a tile row share lines; the autotuner picks per domain size.
+// and each lane streams whole 128-byte lines of its own columns.  The face
+// flux kernel's lanes step through the sublayers together, so surface-major
+// face records and a layer-major reconstruction give contiguous wavefront
+// reads; the other orders are kept for the autotuner.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -326,14 +329,14 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
 }
 
 // ---------------------------------------------------------------------------
-// K4: sublayer fluxes and divergence (redi_apply_flux_impl + redi_face_flux
-// + redi_sublayer_dT, :893-925, :1037-1181)
+// K4: sublayer fluxes, once per face (redi_face_flux + redi_sublayer_dT,
+// :893-925, :1037-1092)
 // ---------------------------------------------------------------------------
 
 // redi_sublayer_dT (1-based top-down layer indices) on the recon arrays of
-// the left / right columns (layer-major with stride n2)
+// the left / right columns
 // (32-bit element offsets from uniform bases: the host checks that every
-// layer-major scratch array holds fewer than 2^31 elements)
+// per-tracer reconstruction array holds fewer than 2^31 elements)
 struct ReconLayout {
   uint32_t ca, cb;  // ReconRec (k, col) at col*ca + k*cb
   uint32_t ta, tb;  // Ti (k, col) at col*ta + k*tb
@@ -364,117 +367,171 @@ __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb
 
 constexpr int kMaxNT = 4;
 
-struct FluxArgs {
-  int nx, ny, nz;
+// A face's sublayer ks contributes when hEff(ks) != 0, to layer KoL(ks) of
+// its left column and KoR(ks) of its right column (1-based, top-down, at
+// most 256).  Its targets are stored packed as KoL | KoR << 16; 0 marks a
+// sublayer that contributes nothing.
+constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
+
+// The fluxes of one face grid (u or v): every face that some cell takes
+// fluxes from, every sublayer, NT tracers.
+struct FaceFluxArgs {
+  int nfa;                // faces per row of the face array
+  int i0, j0;             // first face computed (0-based face coordinates)
+  int di, dj;             // left column = (i - di, j - dj), right column = (i, j)
+  int nx, ns;             // columns per row, neutral surfaces
+  int wall0, wall1;       // 0-based face coordinate across the faces (i for u,
+                          // j for v) of the two wall faces, -1 where open
+  uint32_t fa, fb;        // face record (face, s) at face*fa + s*fb
+  uint32_t nf;            // faces in the face array
   double dt;
-  int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
-  int wall_w, wall_e, wall_s, wall_n;
-  const double *khtr_u, *khtr_v, *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT;
-  const FaceRec *u, *v;
-  const uint8_t *mono_u, *mono_v;
-  int nt;                          // tracers in this launch (<= NT)
-  int force_general;               // take the order-general path for every cell
-  uint32_t fau, fbu, fav, fbv;     // u / v face record (face, s) at face*fa + s*fb
+  const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
+  const FaceRec* rec;
   ReconLayout ry;
-  const ReconRec* recon[kMaxNT];   // per tracer, [kt][col]
-  const double* ti[kMaxNT];        // per tracer, [kt][col], kt = 0..nz
-  double* htr[kMaxNT];
+  const ReconRec* recon[kMaxNT];
+  const double* ti[kMaxNT];
+  uint32_t* tgt;          // packed targets, (s, face) at s*nf + face
+  double* flx;            // fluxes, (s, face, t) at (s*nf + face)*NT + t
 };
 
-struct FaceRef {
-  const FaceRec* rec;  // surface 0 of this face; surface s at rec[s * fst]
-  size_t fst;
-  uint32_t cl, cr;     // left / right column indices
-  double coef;
-  bool left;           // this cell is the face's left (west/south) column
-};
-
-// Accumulate the contributing sublayers of one face whose target layer is K,
-// starting at sublayer s and stopping at the first contributing sublayer
-// with another target; returns the new s.
+// flx = dT * hEff * Coef with Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
+// for every contributing sublayer, in the reference's rounding order
+// (:1060-1090).  The reference evaluates this twice per face, once from
+// each bounding cell; both evaluations are the same expression on the same
+// operands, so one evaluation serves both cells.  Lanes are adjacent faces
+// and step through the sublayers together, so the face records
+// (surface-major) and the layer-major reconstruction are read in
+// contiguous runs across the wavefront.
 template <int NT>
-__device__ __forceinline__ int face_run(const FaceRef& f, int s, int ns, int K, const FluxArgs& a,
-                                        size_t n2, double* acc) {
-  while (s < ns - 1) {
-    const FaceRec top = f.rec[(size_t)s * f.fst];
+__global__ void __launch_bounds__(256) k_face_flux(FaceFluxArgs a, TileMap tm) {
+  int fi, fj;
+  if (!tm.map(fi, fj)) return;
+  const int i = a.i0 + fi, j = a.j0 + fj;
+  const int across = a.di ? i : j;
+  if (across == a.wall0 || across == a.wall1) return;  // no cell takes a wall's fluxes
+  const size_t fo = (size_t)i + (size_t)a.nfa * j;
+  const uint32_t cr = (uint32_t)i + (uint32_t)a.nx * (uint32_t)j;
+  const uint32_t cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
+  const double coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
+  const FaceRec* r = a.rec + fo * a.fa;
+  const size_t fb = a.fb, nf = a.nf;
+  uint32_t* tg = a.tgt + fo;
+  double* fx = a.flx + fo * NT;
+  FaceRec top = r[0];
+  for (int s = 0; s < a.ns - 1; ++s) {
+    const FaceRec bot = r[(size_t)(s + 1) * fb];
+    uint32_t packed = 0;
     if (top.hEff != 0.0) {
-      if ((f.left ? top.KoL : top.KoR) != K) break;
-      const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
+      packed = ((uint32_t)top.KoL << kTargetShiftL) | ((uint32_t)top.KoR << kTargetShiftR);
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                     bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, a.ry);
-        const double flx = (d * top.hEff) * f.coef;
-        acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
+                                     bot.PoR, a.recon[t], a.ti[t], cl, cr, a.ry);
+        fx[(size_t)s * nf * NT + t] = (d * top.hEff) * coef;
       }
     }
-    ++s;
+    tg[(size_t)s * nf] = packed;
+    top = bot;
   }
-  return s;
 }
 
-// The same accumulation without the monotonicity assumption: for layer K,
-// scan every sublayer of the face and take those whose target is K.
+// ---------------------------------------------------------------------------
+// K5: flux divergence per cell (redi_apply_flux_impl, :1094-1181)
+// ---------------------------------------------------------------------------
+struct CellFluxArgs {
+  int nx, ny, nz;
+  int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
+  int wall_w, wall_e, wall_s, wall_n;
+  const double* areaT;
+  const uint8_t *mono_u, *mono_v;
+  const uint32_t *tgt_u, *tgt_v;
+  const double *flx_u, *flx_v;
+  uint32_t nu, nv;                 // faces in the u / v face arrays
+  int force_general;               // take the order-general path for every cell
+  double* htr[kMaxNT];
+};
+
+struct FaceSide {
+  const uint32_t* tg;  // packed targets of sublayer 0; sublayer s at tg[s * nf]
+  const double* fx;    // fluxes of sublayer 0; sublayer s at fx[s * nf * NT]
+  uint32_t nf;
+  int shift;           // kTargetShiftL where this cell is the face's left column
+  bool left;
+};
+
+// The four bounding faces of cell (ci, cj) (0-based) and which of them are
+// open (:1136-1175).  Returns whether every open face's contributing
+// sublayers target non-decreasing layers on this cell's side.
 template <int NT>
-__device__ __forceinline__ void face_scan(const FaceRef& f, int ns, int K, const FluxArgs& a,
-                                          size_t n2, double* acc) {
-  for (int s = 0; s < ns - 1; ++s) {
-    const FaceRec top = f.rec[(size_t)s * f.fst];
-    if (top.hEff == 0.0 || (f.left ? top.KoL : top.KoR) != K) continue;
-    const FaceRec bot = f.rec[(size_t)(s + 1) * f.fst];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                   bot.PoR, a.recon[t], a.ti[t], f.cl, f.cr, a.ry);
-      const double flx = (d * top.hEff) * f.coef;
-      acc[t] = f.left ? acc[t] + flx : acc[t] - flx;
-    }
-  }
-}
-
-// The four bounding faces of cell (ci, cj) (0-based): their records, the
-// left/right recon columns, Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
-// and which of them are open (:1136-1175).  Returns whether every open
-// face's contributing sublayers are monotone on this cell's side.
-__device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, FaceRef* F,
+__device__ __forceinline__ bool cell_sides(const CellFluxArgs& a, int ci, int cj, FaceSide* F,
                                            bool* act) {
   const int nx = a.nx, ny = a.ny;
-  const size_t c = (size_t)ci + (size_t)nx * cj;
   const int i = ci + 1, j = cj + 1;  // 1-based
   bool mono = true;
   // WEST u-face (face i): this cell is the right column
   act[0] = i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e));
   if (act[0]) {
     const size_t fo = (size_t)(i - 1) + (size_t)(nx + 1) * (j - 1);
-    F[0] = FaceRef{a.u + fo * a.fau, a.fbu, (uint32_t)(c - 1), (uint32_t)c,
-                   ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], false};
+    F[0] = FaceSide{a.tgt_u + fo, a.flx_u + fo * NT, a.nu, kTargetShiftR, false};
     mono = mono && (a.mono_u[fo] & kMonoR);
   }
   // EAST u-face (face i+1): this cell is the left column
   act[1] = i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e));
   if (act[1]) {
     const size_t fo = (size_t)i + (size_t)(nx + 1) * (j - 1);
-    F[1] = FaceRef{a.u + fo * a.fau, a.fbu, (uint32_t)c, (uint32_t)(c + 1),
-                   ((a.dt * a.khtr_u[fo]) * a.dy_cu[fo]) * a.idxCu[fo], true};
+    F[1] = FaceSide{a.tgt_u + fo, a.flx_u + fo * NT, a.nu, kTargetShiftL, true};
     mono = mono && (a.mono_u[fo] & kMonoL);
   }
   // SOUTH v-face (face j): this cell is the right (north) column
   act[2] = j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n));
   if (act[2]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * (j - 1);
-    F[2] = FaceRef{a.v + fo * a.fav, a.fbv, (uint32_t)(c - nx), (uint32_t)c,
-                   ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], false};
+    F[2] = FaceSide{a.tgt_v + fo, a.flx_v + fo * NT, a.nv, kTargetShiftR, false};
     mono = mono && (a.mono_v[fo] & kMonoR);
   }
   // NORTH v-face (face j+1): this cell is the left (south) column
   act[3] = j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n));
   if (act[3]) {
     const size_t fo = (size_t)(i - 1) + (size_t)nx * j;
-    F[3] = FaceRef{a.v + fo * a.fav, a.fbv, (uint32_t)c, (uint32_t)(c + nx),
-                   ((a.dt * a.khtr_v[fo]) * a.dx_cv[fo]) * a.idyCv[fo], true};
+    F[3] = FaceSide{a.tgt_v + fo, a.flx_v + fo * NT, a.nv, kTargetShiftL, true};
     mono = mono && (a.mono_v[fo] & kMonoL);
   }
   return mono;
+}
+
+__device__ __forceinline__ int side_target(const FaceSide& f, uint32_t packed) {
+  return (int)((packed >> f.shift) & 0xffffu);
+}
+
+// Add the run of contributing sublayers of one face that targets layer K,
+// starting at sublayer s and stopping at the first contributing sublayer
+// with another target; returns the new s.
+template <int NT>
+__device__ __forceinline__ int side_run(const FaceSide& f, int s, int nsub, int K, double* acc) {
+  while (s < nsub) {
+    const uint32_t packed = f.tg[(size_t)s * f.nf];
+    if (packed != 0) {
+      if (side_target(f, packed) != K) break;
+      const double* x = f.fx + (size_t)s * f.nf * NT;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = f.left ? acc[t] + x[t] : acc[t] - x[t];
+    }
+    ++s;
+  }
+  return s;
+}
+
+// Every contributing sublayer of one face that targets layer K, in s order,
+// without assuming that the targets are monotone.
+template <int NT>
+__device__ __forceinline__ void side_scan(const FaceSide& f, int nsub, int K, double* acc) {
+  for (int s = 0; s < nsub; ++s) {
+    const uint32_t packed = f.tg[(size_t)s * f.nf];
+    if (packed == 0 || side_target(f, packed) != K) continue;
+    const double* x = f.fx + (size_t)s * f.nf * NT;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f.left ? acc[t] + x[t] : acc[t] - x[t];
+  }
 }
 
 // Fast path: cells whose open faces are all monotone.  Layers are visited
@@ -482,14 +539,14 @@ __device__ __forceinline__ bool cell_faces(const FluxArgs& a, int ci, int cj, Fa
 // sublayers targeting K, in W, E, S, N order, which is exactly the order in
 // which the reference adds them into dTr(nz+1-K); dTr lives in registers.
 template <int NT>
-__global__ void __launch_bounds__(256) k_apply_flux(FluxArgs a, TileMap tm) {
+__global__ void __launch_bounds__(256) k_cell_flux(CellFluxArgs a, TileMap tm) {
   int ci, cj;
   if (!tm.map(ci, cj)) return;
-  FaceRef F[4];
+  FaceSide F[4];
   bool act[4];
-  const bool mono = cell_faces(a, ci, cj, F, act);
-  if (!mono || a.force_general) return;  // k_apply_flux_general's cell
-  const int nz = a.nz, ns = 2 * nz + 2;
+  const bool mono = cell_sides<NT>(a, ci, cj, F, act);
+  if (!mono || a.force_general) return;  // k_cell_flux_general's cell
+  const int nz = a.nz, nsub = 2 * nz + 1;
   const size_t n2 = (size_t)a.nx * a.ny;
   const size_t c = (size_t)ci + (size_t)a.nx * cj;
   const double iaij = 1.0 / a.areaT[c];
@@ -498,10 +555,10 @@ __global__ void __launch_bounds__(256) k_apply_flux(FluxArgs a, TileMap tm) {
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) s0 = face_run<NT>(F[0], s0, ns, K, a, n2, acc);
-    if (act[1]) s1 = face_run<NT>(F[1], s1, ns, K, a, n2, acc);
-    if (act[2]) s2 = face_run<NT>(F[2], s2, ns, K, a, n2, acc);
-    if (act[3]) s3 = face_run<NT>(F[3], s3, ns, K, a, n2, acc);
+    if (act[0]) s0 = side_run<NT>(F[0], s0, nsub, K, acc);
+    if (act[1]) s1 = side_run<NT>(F[1], s1, nsub, K, acc);
+    if (act[2]) s2 = side_run<NT>(F[2], s2, nsub, K, acc);
+    if (act[3]) s3 = side_run<NT>(F[3], s3, nsub, K, acc);
     const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
 #pragma unroll
     for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
@@ -511,14 +568,14 @@ __global__ void __launch_bounds__(256) k_apply_flux(FluxArgs a, TileMap tm) {
 // Order-general path for the remaining cells: for every layer K, scan all
 // sublayers of faces W, E, S, N in order and take those targeting K.
 template <int NT>
-__global__ void __launch_bounds__(256) k_apply_flux_general(FluxArgs a, TileMap tm) {
+__global__ void __launch_bounds__(256) k_cell_flux_general(CellFluxArgs a, TileMap tm) {
   int ci, cj;
   if (!tm.map(ci, cj)) return;
-  FaceRef F[4];
+  FaceSide F[4];
   bool act[4];
-  const bool mono = cell_faces(a, ci, cj, F, act);
-  if (mono && !a.force_general) return;  // done by k_apply_flux
-  const int nz = a.nz, ns = 2 * nz + 2;
+  const bool mono = cell_sides<NT>(a, ci, cj, F, act);
+  if (mono && !a.force_general) return;  // done by k_cell_flux
+  const int nz = a.nz, nsub = 2 * nz + 1;
   const size_t n2 = (size_t)a.nx * a.ny;
   const size_t c = (size_t)ci + (size_t)a.nx * cj;
   const double iaij = 1.0 / a.areaT[c];
@@ -526,16 +583,15 @@ __global__ void __launch_bounds__(256) k_apply_flux_general(FluxArgs a, TileMap 
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) face_scan<NT>(F[0], ns, K, a, n2, acc);
-    if (act[1]) face_scan<NT>(F[1], ns, K, a, n2, acc);
-    if (act[2]) face_scan<NT>(F[2], ns, K, a, n2, acc);
-    if (act[3]) face_scan<NT>(F[3], ns, K, a, n2, acc);
+    if (act[0]) side_scan<NT>(F[0], nsub, K, acc);
+    if (act[1]) side_scan<NT>(F[1], nsub, K, acc);
+    if (act[2]) side_scan<NT>(F[2], nsub, K, acc);
+    if (act[3]) side_scan<NT>(F[3], nsub, K, acc);
     const size_t o = (size_t)(nz - K) * n2 + c;
 #pragma unroll
     for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
   }
 }
-
 // ---------------------------------------------------------------------------
 // Face records <-> reference arrays, khtr faces, initial fill
 // ---------------------------------------------------------------------------

@@ -79,14 +79,51 @@ constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 // Tracer slots of the Phase-B scratch for an nt-tracer state.
 int recon_slots(int nt) { return std::min(std::max(nt, 1), kTracerGroup); }
 
-// Device scratch of one step.  Phase A holds ColRec + P per column interface
-// (40 B).  Phase B holds, per tracer slot, a ReconRec (32 B) per layer and an
-// edge value (8 B) per interface, then one discard layer field (8 B per
-// cell) that takes the second output of a two-tracer flux launch run for a
-// lone tracer.  The phases are live at different times and share one buffer.
+// Device scratch of one step; Phase A and Phase B are live at different
+// times and share one buffer.
+//   Phase A: ColRec + P per column interface (40 B).
+//   Phase B, per tracer slot: a ReconRec (32 B) per layer and an edge value
+//   (8 B) per interface of every column, and a flux (8 B) per sublayer of
+//   every u and v face; once: the packed targets (4 B) per sublayer and face,
+//   and a discard layer field (8 B per cell) that takes the second output of
+//   a two-tracer launch run for a lone tracer.
+// Every array starts on a 256-byte boundary.
+struct PhaseBScratch {
+  ReconRec* recon;   // slot t at recon + t * N2 * nz
+  double* ti;        // slot t at ti + t * N2 * (nz + 1)
+  double* discard;   // N2 * nz
+  double* flx_u;     // (ns - 1) * Nu * slots
+  double* flx_v;     // (ns - 1) * Nv * slots
+  uint32_t* tgt_u;   // (ns - 1) * Nu
+  uint32_t* tgt_v;   // (ns - 1) * Nv
+  size_t bytes;
+};
+
+PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots) {
+  const size_t N2 = (size_t)nx * ny, nsub = 2 * (size_t)nz + 1;
+  const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  char* b = (char*)base;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = b ? b + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  PhaseBScratch s;
+  s.recon = (ReconRec*)take(sizeof(ReconRec) * N2 * nz * slots);
+  s.ti = (double*)take(8 * N2 * (nz + 1) * slots);
+  s.discard = (double*)take(8 * N2 * nz);
+  s.flx_u = (double*)take(8 * nsub * nu * slots);
+  s.flx_v = (double*)take(8 * nsub * nv * slots);
+  s.tgt_u = (uint32_t*)take(4 * nsub * nu);
+  s.tgt_v = (uint32_t*)take(4 * nsub * nv);
+  s.bytes = off;
+  return s;
+}
+
 size_t workspace_bytes(int nx, int ny, int nz, int nt) {
-  const size_t per = (size_t)40 * recon_slots(nt) + 8;
-  return per * (size_t)(nz + 1) * nx * ny;
+  const size_t phase_a = (size_t)40 * (nz + 1) * nx * ny;
+  return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt)).bytes);
 }
 
 // The per-face Phase-A state: FaceRec[face][s] for u and v faces plus the
@@ -203,74 +240,101 @@ struct PhaseB {
 
 int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceState& fs, void* ws,
                    const Tiles& tiles, hipStream_t st, Timer& tm) {
-  const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny;
+  const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny, ns = 2 * nz + 2;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   int slot = tm.begin(REDI_K_FACE_KHTR, st);
   k_face_khtr<<<blocks_for((size_t)nu + nv, 256), 256, 0, st>>>(
       nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v);
   tm.end(slot, st);
-  FluxArgs fa{};
-  fa.nx = nx; fa.ny = ny; fa.nz = nz; fa.dt = p.dt;
-  fa.wuf_w = p.nghost + 1; fa.wuf_e = p.nghost + p.nxp + 1;
-  fa.wvf_s = p.nghost + 1; fa.wvf_n = p.nghost + p.nyp + 1;
-  fa.wall_w = p.ww; fa.wall_e = p.we; fa.wall_s = p.ws; fa.wall_n = p.wn;
-  fa.khtr_u = fs.khtr_u; fa.khtr_v = fs.khtr_v;
-  fa.dy_cu = p.dy_cu; fa.dx_cv = p.dx_cv; fa.idxCu = p.idxCu; fa.idyCv = p.idyCv;
-  fa.areaT = p.areaT;
-  fa.u = fs.u; fa.v = fs.v; fa.mono_u = fs.mono_u; fa.mono_v = fs.mono_v;
-  // tracer slot t: nz*N2 ReconRec at rc + t*rstride; (nz+1)*N2 Ti per slot
-  // after every slot's records; then the nz*N2 discard field
-  if (tiles.col_major)
-    fa.ry = ReconLayout{(uint32_t)nz, 1, (uint32_t)nz + 1, 1};
-  else
-    fa.ry = ReconLayout{1, (uint32_t)N2, 1, (uint32_t)N2};
-  face_strides(tiles, fs.nu, nz, &fa.fau, &fa.fbu);
-  face_strides(tiles, fs.nv, nz, &fa.fav, &fa.fbv);
   const int slots = recon_slots(nt);
-  ReconRec* rc = (ReconRec*)ws;
-  const size_t rstride = (size_t)N2 * nz;
-  double* ti0 = (double*)(rc + (size_t)slots * rstride);
-  const size_t tistride = (size_t)N2 * (nz + 1);
-  double* discard = ti0 + (size_t)slots * tistride;
+  const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots);
+  const size_t rstride = (size_t)N2 * nz, tistride = (size_t)N2 * (nz + 1);
+  ReconLayout ry;
+  if (tiles.col_major)
+    ry = ReconLayout{(uint32_t)nz, 1, (uint32_t)nz + 1, 1};
+  else
+    ry = ReconLayout{1, (uint32_t)N2, 1, (uint32_t)N2};
+
+  FaceFluxArgs fu{};  // u faces i = 2..nx (1-based), every row
+  fu.nfa = nx + 1; fu.i0 = 1; fu.j0 = 0; fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
+  fu.wall0 = p.ww ? p.nghost : -1;
+  fu.wall1 = p.we ? p.nghost + p.nxp : -1;
+  face_strides(tiles, fs.nu, nz, &fu.fa, &fu.fb);
+  fu.nf = (uint32_t)nu; fu.dt = p.dt;
+  fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu; fu.rec = fs.u; fu.ry = ry;
+  fu.tgt = sc.tgt_u; fu.flx = sc.flx_u;
+  FaceFluxArgs fv{};  // v faces j = 2..ny (1-based), every column
+  fv.nfa = nx; fv.i0 = 0; fv.j0 = 1; fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
+  fv.wall0 = p.ws ? p.nghost : -1;
+  fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
+  face_strides(tiles, fs.nv, nz, &fv.fa, &fv.fb);
+  fv.nf = (uint32_t)nv; fv.dt = p.dt;
+  fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv; fv.rec = fs.v; fv.ry = ry;
+  fv.tgt = sc.tgt_v; fv.flx = sc.flx_v;
+
+  CellFluxArgs ca{};
+  ca.nx = nx; ca.ny = ny; ca.nz = nz;
+  ca.wuf_w = p.nghost + 1; ca.wuf_e = p.nghost + p.nxp + 1;
+  ca.wvf_s = p.nghost + 1; ca.wvf_n = p.nghost + p.nyp + 1;
+  ca.wall_w = p.ww; ca.wall_e = p.we; ca.wall_s = p.ws; ca.wall_n = p.wn;
+  ca.areaT = p.areaT; ca.mono_u = fs.mono_u; ca.mono_v = fs.mono_v;
+  ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v; ca.flx_u = sc.flx_u; ca.flx_v = sc.flx_v;
+  ca.nu = (uint32_t)nu; ca.nv = (uint32_t)nv;
+  ca.force_general = tiles.force_general;
+
   const int bs = tiles.tx * tiles.ty;
-  unsigned nb;
-  const TileMap m = make_tiles(nx, ny, tiles, &nb);
+  unsigned nbu = 0, nbv = 0, nbc = 0;
+  const TileMap mu = make_tiles(nx - 1, ny, tiles, &nbu);
+  const TileMap mv = make_tiles(nx, ny - 1, tiles, &nbv);
+  const TileMap mc = make_tiles(nx, ny, tiles, &nbc);
   const int G = std::max(1, std::min(tiles.group, slots));
   for (int g0 = 0; g0 < nt; g0 += G) {
     const int gn = std::min(G, nt - g0);
     slot = tm.begin(REDI_K_TRACER_RECON, st);
     for (int t = 0; t < gn; ++t)
       k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t],
-                                                          rc + t * rstride, ti0 + t * tistride,
-                                                          fa.ry.ca, fa.ry.cb, fa.ry.ta, fa.ry.tb);
+                                                          sc.recon + t * rstride,
+                                                          sc.ti + t * tistride, ry.ca, ry.cb,
+                                                          ry.ta, ry.tb);
     tm.end(slot, st);
-    fa.nt = gn;
     for (int t = 0; t < kMaxNT; ++t) {
       const int tt = std::min(t, gn - 1);
-      fa.recon[t] = rc + (size_t)tt * rstride;
-      fa.ti[t] = ti0 + (size_t)tt * tistride;
-      fa.htr[t] = tracers[g0 + tt];
+      fu.recon[t] = fv.recon[t] = sc.recon + (size_t)tt * rstride;
+      fu.ti[t] = fv.ti[t] = sc.ti + (size_t)tt * tistride;
+      ca.htr[t] = tracers[g0 + tt];
     }
     // A lone tracer goes through the two-tracer kernels: slot 1 repeats its
     // reconstruction and its result lands in the discard field.  (The
-    // one-tracer instantiation gave results that differ from the oracle at
-    // -O1 and above while its source is exact in a thread-by-thread CPU
-    // build and at -O0, so it is not built.)
-    if (gn == 1) fa.htr[1] = discard;
+    // one-tracer instantiation of the earlier flux kernels gave results that
+    // differ from the oracle at -O1 and above while their source is exact in
+    // a thread-by-thread CPU build and at -O0, so it is not built.)
+    if (gn == 1) ca.htr[1] = sc.discard;
     const int kn = gn == 1 ? 2 : gn;
-    fa.force_general = tiles.force_general;
-    slot = tm.begin(REDI_K_APPLY_FLUX, st);
+    for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
+      const unsigned nb = grid ? nbv : nbu;
+      if (!nb) continue;
+      const FaceFluxArgs& fa = grid ? fv : fu;
+      const TileMap& m = grid ? mv : mu;
+      slot = tm.begin(REDI_K_FACE_FLUX, st);
+      switch (kn) {
+        case 2: k_face_flux<2><<<nb, bs, 0, st>>>(fa, m); break;
+        case 3: k_face_flux<3><<<nb, bs, 0, st>>>(fa, m); break;
+        default: k_face_flux<4><<<nb, bs, 0, st>>>(fa, m); break;
+      }
+      tm.end(slot, st);
+    }
+    slot = tm.begin(REDI_K_CELL_FLUX, st);
     switch (kn) {
-      case 2: k_apply_flux<2><<<nb, bs, 0, st>>>(fa, m); break;
-      case 3: k_apply_flux<3><<<nb, bs, 0, st>>>(fa, m); break;
-      default: k_apply_flux<4><<<nb, bs, 0, st>>>(fa, m); break;
+      case 2: k_cell_flux<2><<<nbc, bs, 0, st>>>(ca, mc); break;
+      case 3: k_cell_flux<3><<<nbc, bs, 0, st>>>(ca, mc); break;
+      default: k_cell_flux<4><<<nbc, bs, 0, st>>>(ca, mc); break;
     }
     tm.end(slot, st);
-    slot = tm.begin(REDI_K_APPLY_FLUX_GENERAL, st);
+    slot = tm.begin(REDI_K_CELL_FLUX_GENERAL, st);
     switch (kn) {
-      case 2: k_apply_flux_general<2><<<nb, bs, 0, st>>>(fa, m); break;
-      case 3: k_apply_flux_general<3><<<nb, bs, 0, st>>>(fa, m); break;
-      default: k_apply_flux_general<4><<<nb, bs, 0, st>>>(fa, m); break;
+      case 2: k_cell_flux_general<2><<<nbc, bs, 0, st>>>(ca, mc); break;
+      case 3: k_cell_flux_general<3><<<nbc, bs, 0, st>>>(ca, mc); break;
+      default: k_cell_flux_general<4><<<nbc, bs, 0, st>>>(ca, mc); break;
     }
     tm.end(slot, st);
   }
@@ -630,8 +694,8 @@ int redi_timing_collect(redi_t* r, redi_kernel_times_t* out) {
 const char* redi_kernel_name(int kind) {
   static const char* names[REDI_NKERNELS] = {"k_column_state", "k_neutral_sweep_u",
                                              "k_neutral_sweep_v", "k_face_khtr",
-                                             "k_tracer_recon", "k_apply_flux",
-                                             "k_apply_flux_general", ""};
+                                             "k_tracer_recon", "k_cell_flux",
+                                             "k_cell_flux_general", "k_face_flux"};
   return (kind >= 0 && kind < REDI_NKERNELS) ? names[kind] : "";
 }
 

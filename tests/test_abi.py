@@ -54,9 +54,13 @@ def test_library_loads_and_host_only_calls(libpath):
     for n in declared_functions():
         assert hasattr(L, n)
     assert L.redi_version() >= 100
-    # per column interface of a 10x8x4 domain: four tracer slots of a 32-B
-    # reconstruction record plus an 8-B edge, and the 8-B discard field
-    assert L.redi_workspace_bytes(10, 8, 4) == (4 * 40 + 8) * 5 * 10 * 8
+    # 10x8x4 domain (80 columns, 9 sublayers, 88 u and 90 v faces), four
+    # tracer slots, every array rounded up to 256 B: reconstruction records
+    # and edges, the discard field, u and v sublayer fluxes, u and v targets
+    up = lambda b: -(-b // 256) * 256
+    parts = [32 * 80 * 4 * 4, 8 * 80 * 5 * 4, 8 * 80 * 4, 8 * 9 * 88 * 4, 8 * 9 * 90 * 4,
+             4 * 9 * 88, 4 * 9 * 90]
+    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 114432
     assert L.redi_workspace_bytes(0, 8, 4) == 0
     assert L.redi_state_bytes(10, 8, 4) > L.redi_workspace_bytes(10, 8, 4)
     assert isinstance(redi.last_error(), str)
