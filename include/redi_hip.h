@@ -122,28 +122,18 @@ int redi_configure(redi_t *self, int enable, double khtr, int continuous,
 int redi_export_coeffs(const redi_t *self, const redi_coeffs_t *dst,
                        void *stream);
 
-/* Thread-tile shape (tx * ty lanes: 64, 128, 192 or 256) of the sweep and
- * flux kernels: tiles of adjacent faces / cells share columns and v-faces
- * in cache.  Default 64 x 1. */
+/* Thread-tile shape (tx * ty lanes: 64, 128, 192 or 256) of the neutral
+ * sweep: tiles of adjacent faces share columns in cache.  Default 64 x 1. */
 int redi_set_tiles(redi_t *self, int tx, int ty);
-
-/* Storage order of the face records (face_major: each face's surfaces
- * contiguous, else surface-major) and of the tracer reconstruction
- * (column_major: each column's layers contiguous, else layer-major).
- * Default 1, 1.  Changing the face order resets the Phase-A state to its
- * initial values, so it must be followed by redi_calc_coeffs. */
-int redi_set_layout(redi_t *self, int face_major, int column_major);
 
 /* Tracers handled per flux launch (1..4, default 4): more tracers share one
  * pass over the face records, fewer need fewer registers. */
 int redi_set_tracer_group(redi_t *self, int n);
 
-/* The flux kernel accumulates each cell's layers top-down with one cursor
- * per face, which needs the layer index of a face's contributing sublayers
- * to be non-decreasing; cells next to a face where it is not take an
- * order-general path with the same result.  general != 0 sends every cell
- * down the general path (for testing). */
-int redi_set_flux_path(redi_t *self, int general);
+/* Storage order of the per-step tracer reconstruction: layer-major (1,
+ * default) or per column (0); the tracers of a group are interleaved in
+ * both.  Changes only the schedule, never the result. */
+int redi_set_recon_order(redi_t *self, int layer_major);
 
 /* Faces of the last redi_calc_coeffs whose contributing sublayers are not
  * monotone in layer index (u and v faces).  Synchronous. */
@@ -189,15 +179,14 @@ long long redi_state_bytes(int nx_total, int ny_total, int nz);
  * by HIP events recorded on the launch's own stream.  redi_timing_collect
  * waits for those events, adds their elapsed times per kernel kind into
  * `out` and forgets them. */
-#define REDI_NKERNELS 8
+#define REDI_NKERNELS 7
 #define REDI_K_COLUMN_STATE 0
 #define REDI_K_SWEEP_U 1
 #define REDI_K_SWEEP_V 2
 #define REDI_K_FACE_KHTR 3
 #define REDI_K_TRACER_RECON 4
-#define REDI_K_CELL_FLUX 5          /* flux divergence, monotone cells */
-#define REDI_K_CELL_FLUX_GENERAL 6  /* flux divergence, remaining cells */
-#define REDI_K_FACE_FLUX 7          /* sublayer fluxes per face */
+#define REDI_K_CELL_FLUX 5   /* flux divergence per cell */
+#define REDI_K_FACE_FLUX 6   /* sublayer fluxes per face */
 typedef struct {
     double ms[REDI_NKERNELS];           /* summed kernel time */
     long long launches[REDI_NKERNELS];  /* launches timed */

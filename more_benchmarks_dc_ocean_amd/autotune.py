@@ -1,7 +1,8 @@
 """Launch-shape autotuner for the Redi kernels.
 
-The tunables are the thread tile (tx x ty lanes, 64..256) of the sweep and
-flux kernels and the tracer group (tracers per flux launch, 1..4).  A tile
+The tunables are the thread tile (tx x ty lanes, 64..256) of the sweep
+kernel, the tracer group (tracers per flux launch, 1..4) and the storage
+order of the per-step reconstruction (layer-major or per column).  A tile
 of adjacent faces / cells shares columns and v-faces in cache; a larger
 tracer group shares one pass over the face records between more tracers at
 the price of registers (and so of occupancy).  Which wins depends on the
@@ -28,54 +29,65 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # the committed winners, measured on MI355X
 TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_shapes.json")
 
-DEFAULT = {"tx": 64, "ty": 1, "group": 4, "face_major": 1, "col_major": 1}
+DEFAULT = {"tx": 64, "ty": 1, "group": 4, "recon_layer_major": 1}
 
 TILES = [(64, 1), (16, 4), (8, 8), (128, 1), (64, 2), (32, 4), (256, 1)]
 GROUPS = [2, 4]
-LAYOUTS = [(1, 1), (0, 1), (1, 0), (0, 0)]  # (face_major, col_major)
+ORDERS = [1, 0]  # reconstruction layer-major / per column
 
 
 def bucket_of(nx: int, ny: int, nz: int) -> str:
     return f"nz{nz}_a{int(round(math.log2(max(1, nx * ny))))}"
 
 
-def _time_config(r, dom, tx: int, ty: int, group: int, layout, steps: int) -> Tuple[float, Dict]:
+def _time_config(r, dom, tx: int, ty: int, group: int, order: int,
+                 steps: int) -> Tuple[float, Dict]:
     import torch
-    r.set_layout(*layout)
+    r.set_recon_order(order)
     r.set_tiles(tx, ty)
     r.set_tracer_group(group)
     r.step(dom, 1)  # warm the new shape
     torch.cuda.synchronize()
+    # whole steps between two events on the launch stream, then the same
+    # steps again with every kernel bracketed for the per-kernel split
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        r.step(dom, 1)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    total = e0.elapsed_time(e1) / steps
     r.timing(True)
     for _ in range(steps):
         r.step(dom, 1)
     kt = r.collect_timing()
     r.timing(False)
-    total = sum(ms for ms, _ in kt.values()) / steps
     return total, {k: ms / steps for k, (ms, _) in kt.items()}
 
 
-KEYS = ("tx", "ty", "group", "face_major", "col_major")
+KEYS = ("tx", "ty", "group", "recon_layer_major")
 
 
 def tune(case, device: str = "cuda:0", tiles: Iterable = TILES, groups: Iterable = GROUPS,
-         layouts: Iterable = LAYOUTS, steps: int = 2, log=None) -> Dict:
-    """Time every (layout, tile, group) on ``case``; returns the winner and
-    the table."""
+         orders: Iterable = ORDERS, steps: int = 2, log=None) -> Dict:
+    """Time every (reconstruction order, tile, group) on ``case``; returns
+    the winner and the table."""
     from .redi import make_redi
     r, dom = make_redi(case, device)
     rows: List[Dict] = []
-    for lay in layouts:
+    for order in orders:
         for (tx, ty) in tiles:
             for g in groups:
                 if g > case.nt and g != min(groups):
                     continue
-                ms, per = _time_config(r, dom, tx, ty, g, lay, steps)
-                rows.append({"tx": tx, "ty": ty, "group": g, "face_major": lay[0],
-                             "col_major": lay[1], "ms_per_step": ms, "kernels": per})
+                ms, per = _time_config(r, dom, tx, ty, g, order, steps)
+                rows.append({"tx": tx, "ty": ty, "group": g, "recon_layer_major": order,
+                             "ms_per_step": ms, "kernels": per})
                 if log:
-                    log(f"layout f{lay[0]}c{lay[1]} tile {tx:3d}x{ty:<3d} group {g}: {ms:9.3f} "
-                        "ms/step  " + " ".join(f"{k}={v:.2f}" for k, v in sorted(per.items())))
+                    log(f"recon {'layer' if order else 'column'}-major tile {tx:3d}x{ty:<3d} "
+                        f"group {g}: {ms:9.3f} ms/step  "
+                        + " ".join(f"{k}={v:.2f}" for k, v in sorted(per.items())))
     r.close()
     best = min(rows, key=lambda d: d["ms_per_step"])
     return {"bucket": bucket_of(case.nx, case.ny, case.nz), "best": {k: best[k] for k in KEYS},
@@ -115,11 +127,10 @@ class TuneCache:
         return dict(DEFAULT)
 
     def apply(self, r, nx: int, ny: int, nz: int) -> Dict:
-        """Set the bucket's launch shape on an OceanRedi (before its first
-        redi_calc_coeffs: a face-order change resets the Phase-A state)."""
+        """Set the bucket's launch shape on an OceanRedi."""
         cfg = dict(DEFAULT)
-        cfg.update(self.lookup(nx, ny, nz))
-        r.set_layout(cfg["face_major"], cfg["col_major"])
+        cfg.update({k: v for k, v in self.lookup(nx, ny, nz).items() if k in KEYS})
+        r.set_recon_order(cfg["recon_layer_major"])
         r.set_tiles(cfg["tx"], cfg["ty"])
         r.set_tracer_group(cfg["group"])
         return cfg

@@ -86,7 +86,6 @@ def step_launches(case, group: int) -> Dict[str, list]:
         cf.append(8.0 * d["N3"] * gn + share * 8.0 * d["N2"])
     out["k_face_flux"] = ff
     out["k_cell_flux"] = cf
-    out["k_cell_flux_general"] = [0.0] * len(groups)
     return out
 
 

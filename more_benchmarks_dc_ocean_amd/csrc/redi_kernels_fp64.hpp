@@ -17,23 +17,23 @@
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
 //   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
 //                               kt = top-down interface 0..nz
-//   face coefficients (K2->K4)  FaceRec per face and neutral surface s;
-//                               record (face, s) at face*fa + s*fb: per face
-//                               (fa = ns, fb = 1) or surface-major (fa = 1,
-//                               fb = faces); one monotonicity flag per face
+//   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
+//                               surface-major: (face, s) at s*Nf' + face with
+//                               rows padded to Nf' = a multiple of kFaceGroup
+//                               faces; one monotonicity flag per face
 //   tracer recon (K3 -> K4)     ReconRec (layer mean, limited edges, top edge)
-//                               per layer and column, and the interface edges
-//                               Ti per interface and column; element (k, col)
-//                               at col*ca + k*cb: per column (ca = depth,
-//                               cb = 1) or layer-major (ca = 1, cb = N2)
+//                               per layer, column and tracer of a group, and
+//                               the interface edges Ti; per column, tracers
+//                               interleaved: (col, k, t) at (col*nz + k)*NT + t
 //   sublayer fluxes (K4 -> K5)  surface-major: packed targets [s][face] and
-//                               fluxes [s][face][tracer]
+//                               fluxes [s][face][tracer], rows of Nf' faces
 // The sweep reads each of its two columns once, front to back, while lanes
-// sit at unrelated depths, so the column state is always stored per column
-// and each lane streams whole 128-byte lines of its own columns.  The face
-// flux kernel's lanes step through the sublayers together, so surface-major
-// face records and a layer-major reconstruction give contiguous wavefront
-// reads; the other orders are kept for the autotuner.
+// sit at unrelated depths, so the column state is stored per column and
+// each lane streams whole 128-byte lines of its own columns; its lanes are
+// adjacent faces, so its surface-major writes are contiguous.  The face flux
+// kernel gives each face one wavefront with a lane per sublayer: a face's
+// targets are non-decreasing in s, so the lanes walk down the two columns
+// together and read consecutive per-column records.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,14 +67,17 @@ constexpr uint8_t kMonoL = 1, kMonoR = 2;
 // round-robin over XCDs, so workgroup b runs on XCD b % 8; giving each XCD a
 // contiguous run of tiles keeps vertically adjacent tiles (which share
 // v-faces and neighbour columns) in the same L2.
+__device__ __forceinline__ int xcd_band(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = bid & 7, local = bid >> 3;
+  return x * q + (x < r ? x : r) + local;
+}
+
 struct TileMap {
   int n0, n1;    // index space extent (fast, slow)
   int tx, ty;    // tile shape (tx * ty == blockDim.x)
   int tiles0;    // tiles along the fast axis
   __device__ __forceinline__ bool map(int& a, int& b) const {
-    const int nb = gridDim.x, bid = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7, x = bid & 7, local = bid >> 3;
-    const int tile = x * q + (x < r ? x : r) + local;
+    const int tile = xcd_band((int)blockIdx.x, (int)gridDim.x);
     const int t0 = tile % tiles0, t1 = tile / tiles0;
     a = t0 * tx + (int)(threadIdx.x % (unsigned)tx);
     b = t1 * ty + (int)(threadIdx.x / (unsigned)tx);
@@ -298,13 +301,30 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
 }
 
 // ---------------------------------------------------------------------------
-// K3: tracer column reconstruction (redi_tracer_column, :852-891)
+// K3: tracer column reconstruction (redi_tracer_column, :852-891), the NT
+// tracers of a group per launch, tracers interleaved: ReconRec (k, col, t)
+// and Ti (k, col, t) at k*cb + col*ca + t (ReconLayout, tb / ta for Ti) with
+// ca = NT.  Layer-major (cb = N2*NT): the lanes of a launch, adjacent
+// columns, write contiguous rows, and a face-flux lane reads one whole
+// line (all tracers) per record.  Per column (cb = NT, ca = nz*NT): each
+// column's records are contiguous.
 // ---------------------------------------------------------------------------
+struct TracerSet {
+  const double* q[4];  // thickness-weighted tracers of the group
+};
+
+// redi_sublayer_dT's operands: ReconRec (k, col) at col*ca + k*cb and Ti
+// (k, col) at col*ta + k*tb, from per-tracer bases.  (32-bit element offsets
+// from uniform bases: the host checks that (nz+1) * columns * 4 < 2^31.)
+struct ReconLayout {
+  uint32_t ca, cb;
+  uint32_t ta, tb;
+};
+
+template <int NT>
 __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
-                                                      const double* __restrict__ q,
-                                                      ReconRec* __restrict__ rec,
-                                                      double* __restrict__ ti, uint32_t ca,
-                                                      uint32_t cb, uint32_t ta, uint32_t tb) {
+                                                      TracerSet tr, ReconRec* __restrict__ rec,
+                                                      double* __restrict__ ti, ReconLayout y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
@@ -312,20 +332,29 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
     hh = h[o];
     const double he = hh > kHDivEps ? hh : kHDivEps;
-    s[0] = q[o] / he;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) s[t] = tr.q[t][o] / he;
   };
-  double e_prev = 0.0;
+  double e_prev[NT];
+  ReconRec* rc = rec + (size_t)c * y.ca;
+  double* tc = ti + (size_t)c * y.ta;
   auto emit = [&](int k, const double* e, const double* s_above) {
     if (k > 0) {
-      const double tl = s_above[0];
-      double al = e_prev, ar = e[0];
-      ppm_limit<double>(tl, al, ar);
-      rec[(size_t)c * ca + (size_t)(k - 1) * cb] = ReconRec{tl, al, ar, e_prev};
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double tl = s_above[t];
+        double al = e_prev[t], ar = e[t];
+        ppm_limit<double>(tl, al, ar);
+        rc[(size_t)(k - 1) * y.cb + t] = ReconRec{tl, al, ar, e_prev[t]};
+      }
     }
-    ti[(size_t)c * ta + (size_t)k * tb] = e[0];
-    e_prev = e[0];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      tc[(size_t)k * y.tb + t] = e[t];
+      e_prev[t] = e[t];
+    }
   };
-  stream_edges<double, 1>(nz, load, emit);
+  stream_edges<double, NT>(nz, load, emit);
 }
 
 // ---------------------------------------------------------------------------
@@ -335,12 +364,6 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
 
 // redi_sublayer_dT (1-based top-down layer indices) on the recon arrays of
 // the left / right columns
-// (32-bit element offsets from uniform bases: the host checks that every
-// per-tracer reconstruction array holds fewer than 2^31 elements)
-struct ReconLayout {
-  uint32_t ca, cb;  // ReconRec (k, col) at col*ca + k*cb
-  uint32_t ta, tb;  // Ti (k, col) at col*ta + k*tb
-};
 
 __device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
                                               double PoLb, double PoRt, double PoRb,
@@ -373,65 +396,95 @@ constexpr int kMaxNT = 4;
 // sublayer that contributes nothing.
 constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
 
-// The fluxes of one face grid (u or v): every face that some cell takes
-// fluxes from, every sublayer, NT tracers.
+// The fluxes of one face grid (u or v).  One wavefront per face, lanes
+// over its sublayers; a workgroup takes kFaceGroup consecutive faces.
+constexpr int kFaceGroup = 4;
+
 struct FaceFluxArgs {
   int nfa;                // faces per row of the face array
-  int i0, j0;             // first face computed (0-based face coordinates)
+  int ilo, ihi, jlo, jhi; // faces computed: i in [ilo, ihi), j in [jlo, jhi) (0-based)
   int di, dj;             // left column = (i - di, j - dj), right column = (i, j)
   int nx, ns;             // columns per row, neutral surfaces
   int wall0, wall1;       // 0-based face coordinate across the faces (i for u,
                           // j for v) of the two wall faces, -1 where open
-  uint32_t fa, fb;        // face record (face, s) at face*fa + s*fb
   uint32_t nf;            // faces in the face array
+  uint32_t fa, fb;        // face record (face, s) at face*fa + s*fb
+  uint32_t nfp;           // faces per sublayer row of tgt / flx (nf rounded up to kFaceGroup)
   double dt;
   const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
   const FaceRec* rec;
   ReconLayout ry;
   const ReconRec* recon[kMaxNT];
   const double* ti[kMaxNT];
-  uint32_t* tgt;          // packed targets, (s, face) at s*nf + face
-  double* flx;            // fluxes, (s, face, t) at (s*nf + face)*NT + t
+  uint32_t* tgt;          // packed targets, (s, face) at s*nfp + face
+  double* flx;            // fluxes, (s, face, t) at (s*nfp + face)*NT + t
 };
 
 // flx = dT * hEff * Coef with Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
 // for every contributing sublayer, in the reference's rounding order
 // (:1060-1090).  The reference evaluates this twice per face, once from
 // each bounding cell; both evaluations are the same expression on the same
-// operands, so one evaluation serves both cells.  Lanes are adjacent faces
-// and step through the sublayers together, so the face records
-// (surface-major) and the layer-major reconstruction are read in
-// contiguous runs across the wavefront.
+// operands, so one evaluation serves both cells.
+// Lane s of a face's wavefront takes sublayer s: its targets are
+// non-decreasing in s for a monotone face, so the lanes read consecutive
+// records of the per-column reconstruction.  The kFaceGroup waves of a
+// workgroup take consecutive faces and read one line of each surface-major
+// face-record row between them; their fluxes are staged in LDS and written
+// back as whole rows (kFaceGroup faces x NT tracers per sublayer).
 template <int NT>
-__global__ void __launch_bounds__(256) k_face_flux(FaceFluxArgs a, TileMap tm) {
-  int fi, fj;
-  if (!tm.map(fi, fj)) return;
-  const int i = a.i0 + fi, j = a.j0 + fj;
+__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
+  __shared__ double sx[kFaceGroup * 64 * NT];
+  __shared__ uint32_t stg[kFaceGroup * 64];
+  const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const size_t f0 = (size_t)xcd_band((int)blockIdx.x, (int)gridDim.x) * kFaceGroup;
+  const size_t fo = f0 + wv;
+  const int i = (int)(fo % (size_t)a.nfa), j = (int)(fo / (size_t)a.nfa);
   const int across = a.di ? i : j;
-  if (across == a.wall0 || across == a.wall1) return;  // no cell takes a wall's fluxes
-  const size_t fo = (size_t)i + (size_t)a.nfa * j;
-  const uint32_t cr = (uint32_t)i + (uint32_t)a.nx * (uint32_t)j;
-  const uint32_t cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
-  const double coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
-  const FaceRec* r = a.rec + fo * a.fa;
-  const size_t fb = a.fb, nf = a.nf;
-  uint32_t* tg = a.tgt + fo;
-  double* fx = a.flx + fo * NT;
-  FaceRec top = r[0];
-  for (int s = 0; s < a.ns - 1; ++s) {
-    const FaceRec bot = r[(size_t)(s + 1) * fb];
+  const bool live = fo < a.nf && i >= a.ilo && i < a.ihi && j >= a.jlo && j < a.jhi &&
+                    across != a.wall0 && across != a.wall1;
+  uint32_t cl = 0, cr = 0;
+  double coef = 0.0;
+  if (live) {
+    cr = (uint32_t)i + (uint32_t)a.nx * (uint32_t)j;
+    cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
+    coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
+  }
+  const FaceRec* r = a.rec + (live ? fo * a.fa : 0);
+  const size_t fb = a.fb;
+  const int nsub = a.ns - 1;
+  for (int c0 = 0; c0 < nsub; c0 += 64) {
+    const int s = c0 + lane;
     uint32_t packed = 0;
-    if (top.hEff != 0.0) {
-      packed = ((uint32_t)top.KoL << kTargetShiftL) | ((uint32_t)top.KoR << kTargetShiftR);
+    double x[NT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL, top.PoR,
-                                     bot.PoR, a.recon[t], a.ti[t], cl, cr, a.ry);
-        fx[(size_t)s * nf * NT + t] = (d * top.hEff) * coef;
+    for (int t = 0; t < NT; ++t) x[t] = 0.0;
+    if (live && s < nsub) {
+      const FaceRec top = r[(size_t)s * fb];
+      if (top.hEff != 0.0) {
+        const FaceRec bot = r[(size_t)(s + 1) * fb];
+        packed = ((uint32_t)top.KoL << kTargetShiftL) | ((uint32_t)top.KoR << kTargetShiftR);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL,
+                                       top.PoR, bot.PoR, a.recon[t], a.ti[t], cl, cr, a.ry);
+          x[t] = (d * top.hEff) * coef;
+        }
       }
     }
-    tg[(size_t)s * nf] = packed;
-    top = bot;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) sx[(wv * 64 + lane) * NT + t] = x[t];
+    stg[wv * 64 + lane] = packed;
+    __syncthreads();
+    // row rr of the chunk, face q of the group
+    const int rr = (int)(threadIdx.x / kFaceGroup), q = (int)(threadIdx.x % kFaceGroup);
+    const int so = c0 + rr;
+    if (so < nsub) {
+      const size_t o = (size_t)so * a.nfp + f0 + q;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) a.flx[o * NT + t] = sx[(q * 64 + rr) * NT + t];
+      a.tgt[o] = stg[q * 64 + rr];
+    }
+    __syncthreads();
   }
 }
 
@@ -443,155 +496,91 @@ struct CellFluxArgs {
   int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
   int wall_w, wall_e, wall_s, wall_n;
   const double* areaT;
-  const uint8_t *mono_u, *mono_v;
   const uint32_t *tgt_u, *tgt_v;
   const double *flx_u, *flx_v;
-  uint32_t nu, nv;                 // faces in the u / v face arrays
-  int force_general;               // take the order-general path for every cell
+  uint32_t nu, nv;                 // faces per sublayer row of the u / v arrays
   double* htr[kMaxNT];
 };
 
-struct FaceSide {
-  const uint32_t* tg;  // packed targets of sublayer 0; sublayer s at tg[s * nf]
-  const double* fx;    // fluxes of sublayer 0; sublayer s at fx[s * nf * NT]
-  uint32_t nf;
-  int shift;           // kTargetShiftL where this cell is the face's left column
-  bool left;
-};
+// Layers an accumulator column of k_cell_flux holds.
+constexpr int kCellKCap[4] = {32, 76, 128, 256};
 
-// The four bounding faces of cell (ci, cj) (0-based) and which of them are
-// open (:1136-1175).  Returns whether every open face's contributing
-// sublayers target non-decreasing layers on this cell's side.
-template <int NT>
-__device__ __forceinline__ bool cell_sides(const CellFluxArgs& a, int ci, int cj, FaceSide* F,
-                                           bool* act) {
+// One wavefront per workgroup; lane = (cell, tracer), 64 / NT consecutive
+// cells of one row.  dTr(1:nz) of each lane lives in LDS, one column per lane
+// (acc[K][lane]), so a lane can add into any layer: every open face of the
+// cell (W, E, S, N) is streamed sublayer by sublayer (s ascending) and each
+// contributing sublayer's flux is added into dTr(nz+1-Ko(s)), exactly the
+// order of the reference's loops, whatever the pattern of Ko.  The lanes of
+// a wavefront read one contiguous stretch of each surface-major sublayer
+// row.  Then hTr += dTr * (1 / areaT).
+template <int NT, int KCAP>
+__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
+  __shared__ double acc[KCAP * 64];
+  constexpr int kCells = 64 / NT;  // cells per wavefront
+  constexpr int kBatch = 8;        // sublayer rows in flight per face
+  const int lane = (int)threadIdx.x;
+  const int cw = lane / NT, t = lane % NT;
+  const int segs = (a.nx + kCells - 1) / kCells;
+  const int w = xcd_band((int)blockIdx.x, (int)gridDim.x);
+  const int cj = w / segs, ci = (w % segs) * kCells + cw;
+  const bool cell = cw < kCells && ci < a.nx && cj < a.ny;
+  const int nz = a.nz, nsub = 2 * nz + 1;
+  for (int K = 0; K < nz; ++K) acc[K * 64 + lane] = 0.0;
+  if (!cell) return;  // lanes only ever touch their own accumulator column
   const int nx = a.nx, ny = a.ny;
   const int i = ci + 1, j = cj + 1;  // 1-based
-  bool mono = true;
-  // WEST u-face (face i): this cell is the right column
-  act[0] = i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e));
-  if (act[0]) {
-    const size_t fo = (size_t)(i - 1) + (size_t)(nx + 1) * (j - 1);
-    F[0] = FaceSide{a.tgt_u + fo, a.flx_u + fo * NT, a.nu, kTargetShiftR, false};
-    mono = mono && (a.mono_u[fo] & kMonoR);
-  }
-  // EAST u-face (face i+1): this cell is the left column
-  act[1] = i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e));
-  if (act[1]) {
-    const size_t fo = (size_t)i + (size_t)(nx + 1) * (j - 1);
-    F[1] = FaceSide{a.tgt_u + fo, a.flx_u + fo * NT, a.nu, kTargetShiftL, true};
-    mono = mono && (a.mono_u[fo] & kMonoL);
-  }
-  // SOUTH v-face (face j): this cell is the right (north) column
-  act[2] = j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n));
-  if (act[2]) {
-    const size_t fo = (size_t)(i - 1) + (size_t)nx * (j - 1);
-    F[2] = FaceSide{a.tgt_v + fo, a.flx_v + fo * NT, a.nv, kTargetShiftR, false};
-    mono = mono && (a.mono_v[fo] & kMonoR);
-  }
-  // NORTH v-face (face j+1): this cell is the left (south) column
-  act[3] = j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n));
-  if (act[3]) {
-    const size_t fo = (size_t)(i - 1) + (size_t)nx * j;
-    F[3] = FaceSide{a.tgt_v + fo, a.flx_v + fo * NT, a.nv, kTargetShiftL, true};
-    mono = mono && (a.mono_v[fo] & kMonoL);
-  }
-  return mono;
-}
-
-__device__ __forceinline__ int side_target(const FaceSide& f, uint32_t packed) {
-  return (int)((packed >> f.shift) & 0xffffu);
-}
-
-// Add the run of contributing sublayers of one face that targets layer K,
-// starting at sublayer s and stopping at the first contributing sublayer
-// with another target; returns the new s.
-template <int NT>
-__device__ __forceinline__ int side_run(const FaceSide& f, int s, int nsub, int K, double* acc) {
-  while (s < nsub) {
-    const uint32_t packed = f.tg[(size_t)s * f.nf];
-    if (packed != 0) {
-      if (side_target(f, packed) != K) break;
-      const double* x = f.fx + (size_t)s * f.nf * NT;
+  // the four faces: W (u face i, this cell on the right), E (u face i+1,
+  // left), S (v face j, right), N (v face j+1, left) (:1136-1175)
+  const bool act[4] = {
+      i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e)),
+      i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e)),
+      j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n)),
+      j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n))};
+  const size_t fo[4] = {(size_t)(i - 1) + (size_t)(nx + 1) * (j - 1),
+                        (size_t)i + (size_t)(nx + 1) * (j - 1),
+                        (size_t)(i - 1) + (size_t)nx * (j - 1),
+                        (size_t)(i - 1) + (size_t)nx * j};
+#pragma unroll 1
+  for (int f = 0; f < 4; ++f) {
+    if (!act[f]) continue;
+    const bool left = (f & 1) != 0;
+    const int shift = left ? kTargetShiftL : kTargetShiftR;
+    const size_t nf = f < 2 ? a.nu : a.nv;
+    const uint32_t* tg = (f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
+    const double* fx = (f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
+    for (int s0 = 0; s0 < nsub; s0 += kBatch) {
+      uint32_t pk[kBatch];
+      double x[kBatch];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = f.left ? acc[t] + x[t] : acc[t] - x[t];
+      for (int u = 0; u < kBatch; ++u) {
+        const int s = s0 + u;
+        // every (sublayer, face) slot of a computed face holds a flux (0 where
+        // nothing contributes), so both loads issue without waiting
+        pk[u] = s < nsub ? tg[(size_t)s * nf] : 0u;
+        x[u] = s < nsub ? fx[(size_t)s * nf * NT] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (pk[u] == 0) continue;
+        const int K = (int)((pk[u] >> shift) & 0xffffu);
+        double& d = acc[(K - 1) * 64 + lane];
+        d = left ? d + x[u] : d - x[u];
+      }
     }
-    ++s;
   }
-  return s;
-}
-
-// Every contributing sublayer of one face that targets layer K, in s order,
-// without assuming that the targets are monotone.
-template <int NT>
-__device__ __forceinline__ void side_scan(const FaceSide& f, int nsub, int K, double* acc) {
-  for (int s = 0; s < nsub; ++s) {
-    const uint32_t packed = f.tg[(size_t)s * f.nf];
-    if (packed == 0 || side_target(f, packed) != K) continue;
-    const double* x = f.fx + (size_t)s * f.nf * NT;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f.left ? acc[t] + x[t] : acc[t] - x[t];
-  }
-}
-
-// Fast path: cells whose open faces are all monotone.  Layers are visited
-// top-down (K = 1..nz); at each K every face contributes its run of
-// sublayers targeting K, in W, E, S, N order, which is exactly the order in
-// which the reference adds them into dTr(nz+1-K); dTr lives in registers.
-template <int NT>
-__global__ void __launch_bounds__(256) k_cell_flux(CellFluxArgs a, TileMap tm) {
-  int ci, cj;
-  if (!tm.map(ci, cj)) return;
-  FaceSide F[4];
-  bool act[4];
-  const bool mono = cell_sides<NT>(a, ci, cj, F, act);
-  if (!mono || a.force_general) return;  // k_cell_flux_general's cell
-  const int nz = a.nz, nsub = 2 * nz + 1;
-  const size_t n2 = (size_t)a.nx * a.ny;
-  const size_t c = (size_t)ci + (size_t)a.nx * cj;
+  const size_t n2 = (size_t)nx * ny;
+  const size_t c = (size_t)ci + (size_t)nx * cj;
   const double iaij = 1.0 / a.areaT[c];
-  int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  for (int K = 1; K <= nz; ++K) {
-    double acc[NT];
+  double* hp = a.htr[0];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) s0 = side_run<NT>(F[0], s0, nsub, K, acc);
-    if (act[1]) s1 = side_run<NT>(F[1], s1, nsub, K, acc);
-    if (act[2]) s2 = side_run<NT>(F[2], s2, nsub, K, acc);
-    if (act[3]) s3 = side_run<NT>(F[3], s3, nsub, K, acc);
+  for (int q = 1; q < NT; ++q)
+    if (t == q) hp = a.htr[q];
+  for (int K = 1; K <= nz; ++K) {
     const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
+    hp[o] = hp[o] + acc[(K - 1) * 64 + lane] * iaij;
   }
 }
 
-// Order-general path for the remaining cells: for every layer K, scan all
-// sublayers of faces W, E, S, N in order and take those targeting K.
-template <int NT>
-__global__ void __launch_bounds__(256) k_cell_flux_general(CellFluxArgs a, TileMap tm) {
-  int ci, cj;
-  if (!tm.map(ci, cj)) return;
-  FaceSide F[4];
-  bool act[4];
-  const bool mono = cell_sides<NT>(a, ci, cj, F, act);
-  if (mono && !a.force_general) return;  // done by k_cell_flux
-  const int nz = a.nz, nsub = 2 * nz + 1;
-  const size_t n2 = (size_t)a.nx * a.ny;
-  const size_t c = (size_t)ci + (size_t)a.nx * cj;
-  const double iaij = 1.0 / a.areaT[c];
-  for (int K = 1; K <= nz; ++K) {
-    double acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-    if (act[0]) side_scan<NT>(F[0], nsub, K, acc);
-    if (act[1]) side_scan<NT>(F[1], nsub, K, acc);
-    if (act[2]) side_scan<NT>(F[2], nsub, K, acc);
-    if (act[3]) side_scan<NT>(F[3], nsub, K, acc);
-    const size_t o = (size_t)(nz - K) * n2 + c;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) a.htr[t][o] = a.htr[t][o] + acc[t] * iaij;
-  }
-}
 // ---------------------------------------------------------------------------
 // Face records <-> reference arrays, khtr faces, initial fill
 // ---------------------------------------------------------------------------

@@ -76,32 +76,37 @@ Timer g_no_timer;
 
 constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 
-// Tracer slots of the Phase-B scratch for an nt-tracer state.
-int recon_slots(int nt) { return std::min(std::max(nt, 1), kTracerGroup); }
+// Tracer slots of the Phase-B scratch for an nt-tracer state (a lone
+// tracer runs through the two-tracer kernels, so at least two).
+int recon_slots(int nt) { return std::min(std::max(nt, 2), kTracerGroup); }
+
+// Faces per sublayer row of the surface-major face arrays: the face count
+// rounded up to whole face-flux workgroups.
+size_t padded_faces(size_t nf) { return (nf + kFaceGroup - 1) / kFaceGroup * kFaceGroup; }
 
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
 //   Phase A: ColRec + P per column interface (40 B).
 //   Phase B, per tracer slot: a ReconRec (32 B) per layer and an edge value
 //   (8 B) per interface of every column, and a flux (8 B) per sublayer of
-//   every u and v face; once: the packed targets (4 B) per sublayer and face,
-//   and a discard layer field (8 B per cell) that takes the second output of
-//   a two-tracer launch run for a lone tracer.
+//   every (padded) u and v face; once: the packed targets (4 B) per sublayer
+//   and face, and a discard layer field (8 B per cell) that takes the second
+//   output of a two-tracer launch run for a lone tracer.
 // Every array starts on a 256-byte boundary.
 struct PhaseBScratch {
-  ReconRec* recon;   // slot t at recon + t * N2 * nz
-  double* ti;        // slot t at ti + t * N2 * (nz + 1)
+  ReconRec* recon;   // (col, k, t) at (col*nz + k)*slots_used + t
+  double* ti;        // (col, k, t) at (col*(nz+1) + k)*slots_used + t
   double* discard;   // N2 * nz
-  double* flx_u;     // (ns - 1) * Nu * slots
-  double* flx_v;     // (ns - 1) * Nv * slots
-  uint32_t* tgt_u;   // (ns - 1) * Nu
-  uint32_t* tgt_v;   // (ns - 1) * Nv
+  double* flx_u;     // (ns - 1) * Nu' * slots, Nu' = padded u faces
+  double* flx_v;     // (ns - 1) * Nv' * slots
+  uint32_t* tgt_u;   // (ns - 1) * Nu'
+  uint32_t* tgt_v;   // (ns - 1) * Nv'
   size_t bytes;
 };
 
 PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots) {
   const size_t N2 = (size_t)nx * ny, nsub = 2 * (size_t)nz + 1;
-  const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  const size_t nu = padded_faces((size_t)(nx + 1) * ny), nv = padded_faces((size_t)nx * (ny + 1));
   char* b = (char*)base;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -126,8 +131,9 @@ size_t workspace_bytes(int nx, int ny, int nz, int nt) {
   return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt)).bytes);
 }
 
-// The per-face Phase-A state: FaceRec[face][s] for u and v faces plus the
-// monotonicity flags and the resolved face diffusivities.
+// The per-face Phase-A state: FaceRec (face, s) at s*nup + face
+// (surface-major, rows padded) for u and v faces, plus the monotonicity
+// flags and the resolved face diffusivities.
 struct FaceState {
   FaceRec* u = nullptr;
   FaceRec* v = nullptr;
@@ -135,20 +141,24 @@ struct FaceState {
   uint8_t* mono_v = nullptr;
   double* khtr_u = nullptr;
   double* khtr_v = nullptr;
-  size_t nu = 0, nv = 0;  // faces
+  size_t nu = 0, nv = 0;    // faces
+  size_t nup = 0, nvp = 0;  // faces per surface row
 };
 
 size_t face_state_bytes(int nx, int ny, int nz) {
   const size_t ns = 2 * (size_t)nz + 2, nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
-  return (nu + nv) * (ns * sizeof(FaceRec) + 1 + sizeof(double));
+  return (padded_faces(nu) + padded_faces(nv)) * ns * sizeof(FaceRec) +
+         (nu + nv) * (1 + sizeof(double));
 }
 
 int alloc_face_state(FaceState* fs, int nx, int ny, int nz, bool own_khtr) {
   const size_t ns = 2 * (size_t)nz + 2;
   fs->nu = (size_t)(nx + 1) * ny;
   fs->nv = (size_t)nx * (ny + 1);
-  HIP_TRY(hipMalloc(&fs->u, fs->nu * ns * sizeof(FaceRec)));
-  HIP_TRY(hipMalloc(&fs->v, fs->nv * ns * sizeof(FaceRec)));
+  fs->nup = padded_faces(fs->nu);
+  fs->nvp = padded_faces(fs->nv);
+  HIP_TRY(hipMalloc(&fs->u, fs->nup * ns * sizeof(FaceRec)));
+  HIP_TRY(hipMalloc(&fs->v, fs->nvp * ns * sizeof(FaceRec)));
   HIP_TRY(hipMalloc(&fs->mono_u, fs->nu));
   HIP_TRY(hipMalloc(&fs->mono_v, fs->nv));
   if (own_khtr) {
@@ -167,25 +177,17 @@ void free_face_state(FaceState* fs, bool own_khtr) {
 
 int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
   const size_t ns = 2 * (size_t)nz + 2;
-  k_fill_faces<<<blocks_for(fs.nu * ns, 256), 256, 0, st>>>(fs.nu * ns, fs.u, fs.nu, fs.mono_u);
-  k_fill_faces<<<blocks_for(fs.nv * ns, 256), 256, 0, st>>>(fs.nv * ns, fs.v, fs.nv, fs.mono_v);
+  k_fill_faces<<<blocks_for(fs.nup * ns, 256), 256, 0, st>>>(fs.nup * ns, fs.u, fs.nu, fs.mono_u);
+  k_fill_faces<<<blocks_for(fs.nvp * ns, 256), 256, 0, st>>>(fs.nvp * ns, fs.v, fs.nv, fs.mono_v);
   HIP_TRY(hipGetLastError());
   return REDI_OK;
 }
 
 struct Tiles {
   int tx = 64, ty = 1;       // one 64-lane wavefront per tile
-  int force_general = 0;
+  int recon_layer_major = 1; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
-  int face_major = 1;        // face records per face (1) or surface-major (0)
-  int col_major = 1;         // reconstruction per column (1) or layer-major (0)
 };
-
-// record (face, s) at face*fa + s*fb
-void face_strides(const Tiles& t, size_t nfaces, int nz, uint32_t* fa, uint32_t* fb) {
-  const uint32_t ns = 2 * (uint32_t)nz + 2;
-  if (t.face_major) { *fa = ns; *fb = 1; } else { *fa = 1; *fb = (uint32_t)nfaces; }
-}
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
   TileMap m;
@@ -209,7 +211,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
     SweepArgs a{nx, nz, nx + 1, 0, 0, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
-    face_strides(tiles, fs.nu, nz, &a.fa, &a.fb);
+    a.fa = 1; a.fb = (uint32_t)fs.nup;
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
@@ -218,7 +220,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
     SweepArgs a{nx, nz, nx, 0, 0, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
-    face_strides(tiles, fs.nv, nz, &a.fa, &a.fb);
+    a.fa = 1; a.fb = (uint32_t)fs.nvp;
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
@@ -248,28 +250,22 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
   tm.end(slot, st);
   const int slots = recon_slots(nt);
   const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots);
-  const size_t rstride = (size_t)N2 * nz, tistride = (size_t)N2 * (nz + 1);
-  ReconLayout ry;
-  if (tiles.col_major)
-    ry = ReconLayout{(uint32_t)nz, 1, (uint32_t)nz + 1, 1};
-  else
-    ry = ReconLayout{1, (uint32_t)N2, 1, (uint32_t)N2};
 
   FaceFluxArgs fu{};  // u faces i = 2..nx (1-based), every row
-  fu.nfa = nx + 1; fu.i0 = 1; fu.j0 = 0; fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
+  fu.nfa = nx + 1; fu.ilo = 1; fu.ihi = nx; fu.jlo = 0; fu.jhi = ny;
+  fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
   fu.wall0 = p.ww ? p.nghost : -1;
   fu.wall1 = p.we ? p.nghost + p.nxp : -1;
-  face_strides(tiles, fs.nu, nz, &fu.fa, &fu.fb);
-  fu.nf = (uint32_t)nu; fu.dt = p.dt;
-  fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu; fu.rec = fs.u; fu.ry = ry;
+  fu.nf = (uint32_t)nu; fu.fa = 1; fu.fb = (uint32_t)fs.nup; fu.nfp = (uint32_t)fs.nup;
+  fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu; fu.rec = fs.u;
   fu.tgt = sc.tgt_u; fu.flx = sc.flx_u;
   FaceFluxArgs fv{};  // v faces j = 2..ny (1-based), every column
-  fv.nfa = nx; fv.i0 = 0; fv.j0 = 1; fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
+  fv.nfa = nx; fv.ilo = 0; fv.ihi = nx; fv.jlo = 1; fv.jhi = ny;
+  fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
   fv.wall0 = p.ws ? p.nghost : -1;
   fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
-  face_strides(tiles, fs.nv, nz, &fv.fa, &fv.fb);
-  fv.nf = (uint32_t)nv; fv.dt = p.dt;
-  fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv; fv.rec = fs.v; fv.ry = ry;
+  fv.nf = (uint32_t)nv; fv.fa = 1; fv.fb = (uint32_t)fs.nvp; fv.nfp = (uint32_t)fs.nvp;
+  fv.dt = p.dt; fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv; fv.rec = fs.v;
   fv.tgt = sc.tgt_v; fv.flx = sc.flx_v;
 
   CellFluxArgs ca{};
@@ -277,65 +273,72 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
   ca.wuf_w = p.nghost + 1; ca.wuf_e = p.nghost + p.nxp + 1;
   ca.wvf_s = p.nghost + 1; ca.wvf_n = p.nghost + p.nyp + 1;
   ca.wall_w = p.ww; ca.wall_e = p.we; ca.wall_s = p.ws; ca.wall_n = p.wn;
-  ca.areaT = p.areaT; ca.mono_u = fs.mono_u; ca.mono_v = fs.mono_v;
+  ca.areaT = p.areaT;
   ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v; ca.flx_u = sc.flx_u; ca.flx_v = sc.flx_v;
-  ca.nu = (uint32_t)nu; ca.nv = (uint32_t)nv;
-  ca.force_general = tiles.force_general;
+  ca.nu = (uint32_t)fs.nup; ca.nv = (uint32_t)fs.nvp;
 
-  const int bs = tiles.tx * tiles.ty;
-  unsigned nbu = 0, nbv = 0, nbc = 0;
-  const TileMap mu = make_tiles(nx - 1, ny, tiles, &nbu);
-  const TileMap mv = make_tiles(nx, ny - 1, tiles, &nbv);
-  const TileMap mc = make_tiles(nx, ny, tiles, &nbc);
+  int kcap = 0;  // smallest accumulator depth that holds nz layers
+  while (kCellKCap[kcap] < nz) ++kcap;
+  const unsigned nbu = nx > 1 ? (unsigned)(fs.nup / kFaceGroup) : 0;
+  const unsigned nbv = ny > 1 ? (unsigned)(fs.nvp / kFaceGroup) : 0;
   const int G = std::max(1, std::min(tiles.group, slots));
   for (int g0 = 0; g0 < nt; g0 += G) {
     const int gn = std::min(G, nt - g0);
-    slot = tm.begin(REDI_K_TRACER_RECON, st);
-    for (int t = 0; t < gn; ++t)
-      k_tracer_recon<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, p.h, tracers[g0 + t],
-                                                          sc.recon + t * rstride,
-                                                          sc.ti + t * tistride, ry.ca, ry.cb,
-                                                          ry.ta, ry.tb);
-    tm.end(slot, st);
+    // A lone tracer goes through the two-tracer kernels: slot 1 repeats it
+    // and its result lands in the discard field.  (A one-tracer build of
+    // the flux kernels gave results that differ from the oracle at -O1 and
+    // above while its source is exact in a thread-by-thread CPU build and
+    // at -O0, so none is built.)
+    const int kn = gn == 1 ? 2 : gn;
+    TracerSet ts{};
     for (int t = 0; t < kMaxNT; ++t) {
       const int tt = std::min(t, gn - 1);
-      fu.recon[t] = fv.recon[t] = sc.recon + (size_t)tt * rstride;
-      fu.ti[t] = fv.ti[t] = sc.ti + (size_t)tt * tistride;
+      ts.q[t] = tracers[g0 + tt];
       ca.htr[t] = tracers[g0 + tt];
+      fu.recon[t] = fv.recon[t] = sc.recon + std::min(t, kn - 1);
+      fu.ti[t] = fv.ti[t] = sc.ti + std::min(t, kn - 1);
     }
-    // A lone tracer goes through the two-tracer kernels: slot 1 repeats its
-    // reconstruction and its result lands in the discard field.  (The
-    // one-tracer instantiation of the earlier flux kernels gave results that
-    // differ from the oracle at -O1 and above while their source is exact in
-    // a thread-by-thread CPU build and at -O0, so it is not built.)
     if (gn == 1) ca.htr[1] = sc.discard;
-    const int kn = gn == 1 ? 2 : gn;
+    const uint32_t k32 = (uint32_t)kn, n2k = (uint32_t)N2 * k32;
+    const ReconLayout ry = tiles.recon_layer_major
+                               ? ReconLayout{k32, n2k, k32, n2k}
+                               : ReconLayout{(uint32_t)nz * k32, k32, (uint32_t)(nz + 1) * k32, k32};
+    fu.ry = fv.ry = ry;
+    slot = tm.begin(REDI_K_TRACER_RECON, st);
+    const unsigned nbr = blocks_for(N2, 128);
+    switch (kn) {
+      case 2: k_tracer_recon<2><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
+      case 3: k_tracer_recon<3><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
+      default: k_tracer_recon<4><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
+    }
+    tm.end(slot, st);
     for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
       const unsigned nb = grid ? nbv : nbu;
       if (!nb) continue;
       const FaceFluxArgs& fa = grid ? fv : fu;
-      const TileMap& m = grid ? mv : mu;
       slot = tm.begin(REDI_K_FACE_FLUX, st);
       switch (kn) {
-        case 2: k_face_flux<2><<<nb, bs, 0, st>>>(fa, m); break;
-        case 3: k_face_flux<3><<<nb, bs, 0, st>>>(fa, m); break;
-        default: k_face_flux<4><<<nb, bs, 0, st>>>(fa, m); break;
+        case 2: k_face_flux<2><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
+        case 3: k_face_flux<3><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
+        default: k_face_flux<4><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
       }
       tm.end(slot, st);
     }
     slot = tm.begin(REDI_K_CELL_FLUX, st);
+    const unsigned nbc = (unsigned)ny * (unsigned)((nx + 64 / kn - 1) / (64 / kn));
+#define REDI_CELL_FLUX(NT)                                                           \
+  switch (kcap) {                                                                    \
+    case 0: k_cell_flux<NT, kCellKCap[0]><<<nbc, 64, 0, st>>>(ca); break;            \
+    case 1: k_cell_flux<NT, kCellKCap[1]><<<nbc, 64, 0, st>>>(ca); break;            \
+    case 2: k_cell_flux<NT, kCellKCap[2]><<<nbc, 64, 0, st>>>(ca); break;            \
+    default: k_cell_flux<NT, kCellKCap[3]><<<nbc, 64, 0, st>>>(ca); break;           \
+  }
     switch (kn) {
-      case 2: k_cell_flux<2><<<nbc, bs, 0, st>>>(ca, mc); break;
-      case 3: k_cell_flux<3><<<nbc, bs, 0, st>>>(ca, mc); break;
-      default: k_cell_flux<4><<<nbc, bs, 0, st>>>(ca, mc); break;
+      case 2: REDI_CELL_FLUX(2) break;
+      case 3: REDI_CELL_FLUX(3) break;
+      default: REDI_CELL_FLUX(4) break;
     }
-    tm.end(slot, st);
-    slot = tm.begin(REDI_K_CELL_FLUX_GENERAL, st);
-    switch (kn) {
-      case 2: k_cell_flux_general<2><<<nbc, bs, 0, st>>>(ca, mc); break;
-      case 3: k_cell_flux_general<3><<<nbc, bs, 0, st>>>(ca, mc); break;
-      default: k_cell_flux_general<4><<<nbc, bs, 0, st>>>(ca, mc); break;
-    }
+#undef REDI_CELL_FLUX
     tm.end(slot, st);
   }
   HIP_TRY(hipGetLastError());
@@ -346,12 +349,10 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
 // all_faces: every face (the reference's init values where Phase A never
 // writes); otherwise only the faces redi_calc_coeffs_x/_y compute.
 int export_coeffs(const FaceState& fs, int nx, int ny, int nz, const redi_coeffs_t& d,
-                  bool all_faces, bool with_khtr, const Tiles& tiles, hipStream_t st) {
+                  bool all_faces, bool with_khtr, hipStream_t st) {
   const int ns = 2 * nz + 2;
   const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
-  uint32_t ua, ub, va, vb;
-  face_strides(tiles, nu, nz, &ua, &ub);
-  face_strides(tiles, nv, nz, &va, &vb);
+  const uint32_t ua = 1, ub = (uint32_t)fs.nup, va = 1, vb = (uint32_t)fs.nvp;
   CoeffArrays cu{d.uPoL, d.uPoR, d.uhEff, d.uKoL, d.uKoR};
   CoeffArrays cv{d.vPoL, d.vPoR, d.vhEff, d.vKoL, d.vKoR};
   if (all_faces) {
@@ -455,8 +456,8 @@ int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
   if (nz > REDI_NZ_MAX)
     return fail(REDI_ERR_NZ, "redi_init: 2*nz_ml+2 = %d exceeds the neutral-surface limit %d",
                 2 * nz + 2, 2 * REDI_NZ_MAX + 2);
-  if ((double)(nz + 1) * nx * ny >= 2147483648.0)
-    return fail(REDI_ERR_ARG, "redi_init: %d x %d x %d exceeds 2^31 cell interfaces per domain",
+  if ((double)(nz + 1) * nx * ny * kTracerGroup >= 2147483648.0)
+    return fail(REDI_ERR_ARG, "redi_init: %d x %d x %d exceeds 2^29 cell interfaces per domain",
                 nx, ny, nz);
   redi_t* r = new redi_t();
   auto cleanup = [&](int code) {
@@ -519,22 +520,9 @@ int redi_set_tracer_group(redi_t* r, int n) {
   return REDI_OK;
 }
 
-int redi_set_layout(redi_t* r, int face_major, int column_major) {
-  if (!r) return fail(REDI_ERR_ARG, "redi_set_layout: null handle");
-  const int fm = face_major ? 1 : 0, cm = column_major ? 1 : 0;
-  if (fm != r->tiles.face_major) {  // stored records change order: back to init values
-    r->tiles.face_major = fm;
-    int rc = fill_face_state(r->fs, r->nz, nullptr);
-    if (rc) return rc;
-    HIP_TRY(hipDeviceSynchronize());
-  }
-  r->tiles.col_major = cm;
-  return REDI_OK;
-}
-
-int redi_set_flux_path(redi_t* r, int general) {
-  if (!r) return fail(REDI_ERR_ARG, "redi_set_flux_path: null handle");
-  r->tiles.force_general = general ? 1 : 0;
+int redi_set_recon_order(redi_t* r, int layer_major) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_recon_order: null handle");
+  r->tiles.recon_layer_major = layer_major ? 1 : 0;
   return REDI_OK;
 }
 
@@ -555,7 +543,7 @@ int redi_export_coeffs(const redi_t* r, const redi_coeffs_t* dst, void* stream) 
   if (!dst->uPoL || !dst->uPoR || !dst->uKoL || !dst->uKoR || !dst->uhEff || !dst->vPoL ||
       !dst->vPoR || !dst->vKoL || !dst->vKoR || !dst->vhEff)
     return fail(REDI_ERR_ARG, "redi_export_coeffs: incomplete destination arrays");
-  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, r->tiles, (hipStream_t)stream);
+  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
 }
 
 int redi_calc_coeffs(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_t* eos,
@@ -641,8 +629,8 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   if (!eos || !cf || !tracers || nt < 1 || nx < 1 || ny < 1 || nz < 1)
     return fail(REDI_ERR_ARG, "redi_launch_nt: bad argument");
   if (nz > REDI_NZ_MAX) return fail(REDI_ERR_NZ, "redi_launch_nt: nz %d > %d", nz, REDI_NZ_MAX);
-  if ((double)(nz + 1) * nx * ny >= 2147483648.0)
-    return fail(REDI_ERR_ARG, "redi_launch_nt: domain exceeds 2^31 cell interfaces");
+  if ((double)(nz + 1) * nx * ny * kTracerGroup >= 2147483648.0)
+    return fail(REDI_ERR_ARG, "redi_launch_nt: domain exceeds 2^29 cell interfaces");
   if (idx_T < 1 || idx_T > nt || idx_S < 1 || idx_S > nt)
     return fail(REDI_ERR_ARG, "redi_launch_nt: tracer index out of range");
   if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "redi_launch_nt: unknown EOS variant");
@@ -658,7 +646,7 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   rc = launch_phase_a(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1], tracers[idx_S - 1],
                       wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
   if (rc) return rc;
-  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, tiles, st);
+  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, st);
   if (rc) return rc;
   PhaseB p;
   p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;
@@ -695,7 +683,7 @@ const char* redi_kernel_name(int kind) {
   static const char* names[REDI_NKERNELS] = {"k_column_state", "k_neutral_sweep_u",
                                              "k_neutral_sweep_v", "k_face_khtr",
                                              "k_tracer_recon", "k_cell_flux",
-                                             "k_cell_flux_general", "k_face_flux"};
+                                             "k_face_flux"};
   return (kind >= 0 && kind < REDI_NKERNELS) ? names[kind] : "";
 }
 

@@ -59,8 +59,6 @@ class Bc(ctypes.Structure):
     _fields_ = [("west", _I), ("east", _I), ("south", _I), ("north", _I)]
 
 
-class KernelTimes(ctypes.Structure):
-    _fields_ = [("ms", _D * 8), ("launches", ctypes.c_longlong * 8)]
 
 
 class Coeffs(ctypes.Structure):
@@ -72,13 +70,17 @@ COEFF_NAMES = ("uPoL", "uPoR", "uKoL", "uKoR", "uhEff", "vPoL", "vPoR", "vKoL", 
 
 # every extern "C" symbol declared in include/redi_hip.h
 EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs", "redi_set_tiles",
-            "redi_state_bytes", "redi_set_flux_path", "redi_nonmonotone_faces",
-            "redi_set_tracer_group", "redi_set_layout",
+            "redi_state_bytes", "redi_set_recon_order", "redi_nonmonotone_faces",
+            "redi_set_tracer_group",
             "redi_calc_coeffs", "redi_apply_flux", "redi_apply_flux_hoist", "redi_step",
             "redi_workspace_bytes", "redi_last_error", "redi_version", "redi_opt_launch",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
             "redi_kernel_name")
-NKERNELS = 8
+NKERNELS = 7  # REDI_NKERNELS
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [("ms", _D * NKERNELS), ("launches", ctypes.c_longlong * NKERNELS)]
 
 _lib = None
 
@@ -99,9 +101,8 @@ def lib() -> ctypes.CDLL:
         L.redi_configure.argtypes = [_P, _I, _D, _I, _D]
         L.redi_export_coeffs.argtypes = [_P, ctypes.POINTER(Coeffs), _P]
         L.redi_set_tiles.argtypes = [_P, _I, _I]
-        L.redi_set_flux_path.argtypes = [_P, _I]
+        L.redi_set_recon_order.argtypes = [_P, _I]
         L.redi_set_tracer_group.argtypes = [_P, _I]
-        L.redi_set_layout.argtypes = [_P, _I, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
                                              ctypes.POINTER(ctypes.c_longlong)]
         L.redi_state_bytes.restype = ctypes.c_longlong
@@ -285,20 +286,15 @@ class OceanRedi:
                 out[lib().redi_kernel_name(k).decode()] = (kt.ms[k], int(kt.launches[k]))
         return out
 
-    def set_flux_path(self, general: bool) -> None:
-        _check(lib().redi_set_flux_path(self.handle, int(bool(general))), "redi_set_flux_path")
+    def set_recon_order(self, layer_major: bool) -> None:
+        _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
+               "redi_set_recon_order")
 
     def nonmonotone_faces(self) -> tuple:
         nu, nv = ctypes.c_longlong(), ctypes.c_longlong()
         _check(lib().redi_nonmonotone_faces(self.handle, ctypes.byref(nu), ctypes.byref(nv)),
                "redi_nonmonotone_faces")
         return nu.value, nv.value
-
-    def set_layout(self, face_major: bool, column_major: bool) -> None:
-        """Storage order of the face records and of the reconstruction;
-        changing the face order resets the Phase-A state."""
-        _check(lib().redi_set_layout(self.handle, int(bool(face_major)), int(bool(column_major))),
-               "redi_set_layout")
 
     def set_tracer_group(self, n: int) -> None:
         _check(lib().redi_set_tracer_group(self.handle, int(n)), "redi_set_tracer_group")

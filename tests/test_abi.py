@@ -54,16 +54,28 @@ def test_library_loads_and_host_only_calls(libpath):
     for n in declared_functions():
         assert hasattr(L, n)
     assert L.redi_version() >= 100
-    # 10x8x4 domain (80 columns, 9 sublayers, 88 u and 90 v faces), four
-    # tracer slots, every array rounded up to 256 B: reconstruction records
-    # and edges, the discard field, u and v sublayer fluxes, u and v targets
+    # 10x8x4 domain (80 columns, 9 sublayers, 88 u and 90 v faces; face rows
+    # padded to multiples of 4: 88 and 92), four tracer slots, every array
+    # rounded up to 256 B: reconstruction records and edges, the discard
+    # field, u and v sublayer fluxes, u and v targets
     up = lambda b: -(-b // 256) * 256
-    parts = [32 * 80 * 4 * 4, 8 * 80 * 5 * 4, 8 * 80 * 4, 8 * 9 * 88 * 4, 8 * 9 * 90 * 4,
-             4 * 9 * 88, 4 * 9 * 90]
-    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 114432
+    parts = [32 * 80 * 4 * 4, 8 * 80 * 5 * 4, 8 * 80 * 4, 8 * 9 * 88 * 4, 8 * 9 * 92 * 4,
+             4 * 9 * 88, 4 * 9 * 92]
+    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 114944
     assert L.redi_workspace_bytes(0, 8, 4) == 0
     assert L.redi_state_bytes(10, 8, 4) > L.redi_workspace_bytes(10, 8, 4)
     assert isinstance(redi.last_error(), str)
+
+
+def test_kernel_timing_struct_matches_header():
+    from more_benchmarks_dc_ocean_amd import redi
+    src = open(HEADER).read()
+    n = int(re.search(r"#define REDI_NKERNELS (\d+)", src).group(1))
+    assert redi.NKERNELS == n
+    assert ctypes.sizeof(redi.KernelTimes) == 16 * n
+    L = redi.lib()
+    names = [L.redi_kernel_name(k).decode() for k in range(n)]
+    assert all(names) and len(set(names)) == n and L.redi_kernel_name(n) == b""
 
 
 def test_fortran_binding_names_match_reference():
