@@ -130,9 +130,9 @@ int redi_set_tiles(redi_t *self, int tx, int ty);
  * pass over the face records, fewer need fewer registers. */
 int redi_set_tracer_group(redi_t *self, int n);
 
-/* Storage order of the per-step tracer reconstruction: layer-major (1,
- * default) or per column (0); the tracers of a group are interleaved in
- * both.  Changes only the schedule, never the result. */
+/* Storage order of the per-step tracer reconstruction: per column (0,
+ * default) or layer-major (1).  Changes only the schedule, never the
+ * result. */
 int redi_set_recon_order(redi_t *self, int layer_major);
 
 /* Faces of the last redi_calc_coeffs whose contributing sublayers are not

@@ -29,11 +29,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 # the committed winners, measured on MI355X
 TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_shapes.json")
 
-DEFAULT = {"tx": 64, "ty": 1, "group": 4, "recon_layer_major": 1}
+DEFAULT = {"tx": 64, "ty": 1, "group": 4, "recon_layer_major": 0}
 
 TILES = [(64, 1), (16, 4), (8, 8), (128, 1), (64, 2), (32, 4), (256, 1)]
 GROUPS = [2, 4]
-ORDERS = [1, 0]  # reconstruction layer-major / per column
+ORDERS = [0, 1]  # reconstruction per column / layer-major
 
 
 def bucket_of(nx: int, ny: int, nz: int) -> str:

@@ -54,8 +54,10 @@ struct __align__(16) FaceRec {  // one neutral surface s of a face (K2 output)
   int32_t KoL, KoR;             // 1-based top-down layer indices
 };
 
-struct __align__(16) ReconRec {  // one layer kt of a tracer column (K3 output)
-  double Tl, aL, aR, Ti;         // layer mean, limited top/bottom edges, top interface
+struct __align__(16) ReconRec {  // one layer of a tracer column (K3 output)
+  double Tl, aL, aR;             // layer mean, limited top / bottom edge values
+  double Tt, Tb;                 // interface values at the layer's top / bottom
+  double pad;
 };
 
 // Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
@@ -302,29 +304,28 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
 
 // ---------------------------------------------------------------------------
 // K3: tracer column reconstruction (redi_tracer_column, :852-891), the NT
-// tracers of a group per launch, tracers interleaved: ReconRec (k, col, t)
-// and Ti (k, col, t) at k*cb + col*ca + t (ReconLayout, tb / ta for Ti) with
-// ca = NT.  Layer-major (cb = N2*NT): the lanes of a launch, adjacent
-// columns, write contiguous rows, and a face-flux lane reads one whole
-// line (all tracers) per record.  Per column (cb = NT, ca = nz*NT): each
-// column's records are contiguous.
+// tracers of a group per launch.  ReconRec (k, col, t) at col*ca + k*cb +
+// t*ct (ReconLayout):
+//   per column (ca = NT*nz, cb = 1, ct = nz): each tracer's column is one
+//     run of records, which a face's lanes, walking down the column, read in
+//     contiguous stretches;
+//   layer-major (ca = NT, cb = N2*NT, ct = 1): the lanes of this kernel,
+//     adjacent columns, write contiguous rows.
 // ---------------------------------------------------------------------------
 struct TracerSet {
   const double* q[4];  // thickness-weighted tracers of the group
 };
 
-// redi_sublayer_dT's operands: ReconRec (k, col) at col*ca + k*cb and Ti
-// (k, col) at col*ta + k*tb, from per-tracer bases.  (32-bit element offsets
-// from uniform bases: the host checks that (nz+1) * columns * 4 < 2^31.)
+// (32-bit element offsets from uniform bases: the host checks that
+// (nz+1) * columns * 4 < 2^31.)
 struct ReconLayout {
-  uint32_t ca, cb;
-  uint32_t ta, tb;
+  uint32_t ca, cb, ct;
 };
 
 template <int NT>
 __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
                                                       TracerSet tr, ReconRec* __restrict__ rec,
-                                                      double* __restrict__ ti, ReconLayout y) {
+                                                      ReconLayout y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
@@ -337,24 +338,104 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
   };
   double e_prev[NT];
   ReconRec* rc = rec + (size_t)c * y.ca;
-  double* tc = ti + (size_t)c * y.ta;
   auto emit = [&](int k, const double* e, const double* s_above) {
-    if (k > 0) {
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < NT; ++t) {
+      if (k > 0) {
         const double tl = s_above[t];
         double al = e_prev[t], ar = e[t];
         ppm_limit<double>(tl, al, ar);
-        rc[(size_t)(k - 1) * y.cb + t] = ReconRec{tl, al, ar, e_prev[t]};
+        rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec{tl, al, ar, e_prev[t], e[t], 0.0};
       }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      tc[(size_t)k * y.tb + t] = e[t];
       e_prev[t] = e[t];
     }
   };
   stream_edges<double, NT>(nz, load, emit);
+}
+
+// The same reconstruction, tiled: a workgroup takes kReconCols adjacent
+// columns x kReconLayers layers.  Layer values, PLM slopes and PPM edges are
+// staged in LDS stage by stage (every value from the same functions on the
+// same operands as stream_edges, including its clamped windows at the top
+// and bottom), so global reads run along columns and the per-column record
+// runs are written along layers.  Per-column layout only (cb = 1).
+constexpr int kReconCols = 16, kReconLayers = 16;
+
+template <int NT>
+__global__ void __launch_bounds__(kReconCols * kReconLayers)
+    k_tracer_recon_tiled(int N2, int nz, const double* __restrict__ h, TracerSet tr,
+                         ReconRec* __restrict__ rec, ReconLayout y) {
+  constexpr int CT = kReconCols, KT = kReconLayers, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
+  __shared__ double sh[NL][CT];        // h, layers k0-2 .. k0+KT+1 (clamped)
+  __shared__ double sv[NT][NL][CT];    // layer values hTr / max(h, eps), same layers
+  __shared__ double sd[NT][NSL][CT];   // PLM slopes, layers k0-1 .. k0+KT
+  __shared__ double se[NT][NE][CT];    // PPM edges, interfaces k0 .. k0+KT
+  const int nkb = (nz + KT - 1) / KT;
+  const int b = xcd_band((int)blockIdx.x, (int)gridDim.x);
+  const int c0 = (b / nkb) * CT, k0 = (b % nkb) * KT;
+  const int tid = (int)threadIdx.x;
+  const size_t n2 = (size_t)N2;
+  // 1: layer values, lanes along columns
+  for (int q = tid; q < NL * CT; q += CT * KT) {
+    const int r = q / CT, cc = q % CT;
+    int kl = k0 - 2 + r;
+    kl = kl < 0 ? 0 : (kl > nz - 1 ? nz - 1 : kl);
+    const int c = c0 + cc < N2 ? c0 + cc : N2 - 1;
+    const size_t o = (size_t)(nz - 1 - kl) * n2 + c;  // top-down layer kl
+    const double hh = h[o];
+    const double he = hh > kHDivEps ? hh : kHDivEps;
+    sh[r][cc] = hh;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) sv[t][r][cc] = tr.q[t][o] / he;
+  }
+  __syncthreads();
+  // 2: slopes of layers kl = k0-1+r; 0 for the first and last layer (the
+  // slope stream_edges starts from and the one it skips)
+  for (int q = tid; q < NSL * CT; q += CT * KT) {
+    const int r = q / CT, cc = q % CT;
+    const int kl = k0 - 1 + r;  // rows r, r+1, r+2 of sh / sv hold layers kl-1, kl, kl+1
+    const bool in = kl >= 1 && kl <= nz - 2;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      sd[t][r][cc] = in ? plm_slope<double>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
+                                            sv[t][r + 1][cc], sv[t][r + 2][cc])
+                        : 0.0;
+  }
+  __syncthreads();
+  // 3: edges at interfaces ki = k0+r
+  for (int q = tid; q < NE * CT; q += CT * KT) {
+    const int r = q / CT, cc = q % CT;
+    const int ki = k0 + r;  // sh / sv rows r..r+3: layers ki-2 .. ki+1 (clamped);
+                            // sd rows r, r+1: slopes of layers ki-1, ki
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double e = 0.0;
+      if (ki == 0) {
+        e = sv[t][r + 2][cc] - 0.5 * 0.0;  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
+      } else if (ki == nz) {
+        e = sv[t][r + 1][cc] + 0.5 * 0.0;  // diff(nk) = 0
+      } else if (ki < nz) {
+        e = ppm_edge<double>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sh[r + 3][cc],
+                             sv[t][r + 1][cc], sv[t][r + 2][cc], sd[t][r][cc], sd[t][r + 1][cc]);
+      }
+      se[t][r][cc] = e;
+    }
+  }
+  __syncthreads();
+  // 4: records of layers kl = k0+kk, lanes along layers
+  const int kk = tid % KT, cc = tid / KT;
+  const int kl = k0 + kk, c = c0 + cc;
+  if (kl < nz && c < N2) {
+    ReconRec* rc = rec + (size_t)c * y.ca + (size_t)kl * y.cb;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double tl = sv[t][kk + 2][cc];
+      const double et = se[t][kk][cc], eb = se[t][kk + 1][cc];
+      double al = et, ar = eb;
+      ppm_limit<double>(tl, al, ar);
+      rc[(size_t)t * y.ct] = ReconRec{tl, al, ar, et, eb, 0.0};
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -362,22 +443,17 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const doub
 // :893-925, :1037-1092)
 // ---------------------------------------------------------------------------
 
-// redi_sublayer_dT (1-based top-down layer indices) on the recon arrays of
-// the left / right columns
+// Pieces of redi_sublayer_dT (:893-925).  The interface value at position
+// Po in layer K of a column, (1 - Po) * T(K-1) + Po * T(K), from layer K's
+// record (the tlt / tlb / trt / trb expressions, :905-912):
+__device__ __forceinline__ double edge_value(const ReconRec& r, double Po) {
+  return (1.0 - Po) * r.Tt + Po * r.Tb;
+}
 
-__device__ __forceinline__ double sublayer_dT(int klt, int klb, int krt, int krb, double PoLt,
-                                              double PoLb, double PoRt, double PoRb,
-                                              const ReconRec* __restrict__ rec,
-                                              const double* __restrict__ ti, uint32_t cl,
-                                              uint32_t cr, const ReconLayout& y) {
-  const uint32_t rl = cl * y.ca, rr = cr * y.ca, il = cl * y.ta, ir = cr * y.ta;
-  const ReconRec lt = rec[rl + (uint32_t)(klt - 1) * y.cb], rt = rec[rr + (uint32_t)(krt - 1) * y.cb];
-  const double tlt = (1.0 - PoLt) * lt.Ti + PoLt * ti[il + (uint32_t)klt * y.tb];
-  const double tlb = (1.0 - PoLb) * ti[il + (uint32_t)(klb - 1) * y.tb] + PoLb * ti[il + (uint32_t)klb * y.tb];
-  const double trt = (1.0 - PoRt) * rt.Ti + PoRt * ti[ir + (uint32_t)krt * y.tb];
-  const double trb = (1.0 - PoRb) * ti[ir + (uint32_t)(krb - 1) * y.tb] + PoRb * ti[ir + (uint32_t)krb * y.tb];
-  const double tlay = ppm_ave<double>(PoLt, PoLb + (double)(klb - klt), lt.aL, lt.aR, lt.Tl);
-  const double trlay = ppm_ave<double>(PoRt, PoRb + (double)(krb - krt), rt.aL, rt.aR, rt.Tl);
+// The down-gradient guard (:913-924): the layer difference where the
+// top, bottom, mean and layer differences agree in sign, else 0.
+__device__ __forceinline__ double guarded_dT(double tlt, double tlb, double trt, double trb,
+                                             double tlay, double trlay) {
   const double dT_top = trt - tlt;
   const double dT_bot = trb - tlb;
   const double dT_ave = 0.5 * (dT_top + dT_bot);
@@ -396,9 +472,10 @@ constexpr int kMaxNT = 4;
 // sublayer that contributes nothing.
 constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
 
-// The fluxes of one face grid (u or v).  One wavefront per face, lanes
-// over its sublayers; a workgroup takes kFaceGroup consecutive faces.
+// The fluxes of one face grid (u or v).  One wavefront per face, a
+// workgroup takes kFaceGroup consecutive faces.
 constexpr int kFaceGroup = 4;
+constexpr int kChunk = 63;  // sublayers per pass of a wavefront (64 surfaces)
 
 struct FaceFluxArgs {
   int nfa;                // faces per row of the face array
@@ -414,8 +491,7 @@ struct FaceFluxArgs {
   const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
   const FaceRec* rec;
   ReconLayout ry;
-  const ReconRec* recon[kMaxNT];
-  const double* ti[kMaxNT];
+  const ReconRec* recon;  // tracer t's records from recon + t*ry.ct
   uint32_t* tgt;          // packed targets, (s, face) at s*nfp + face
   double* flx;            // fluxes, (s, face, t) at (s*nfp + face)*NT + t
 };
@@ -425,12 +501,16 @@ struct FaceFluxArgs {
 // (:1060-1090).  The reference evaluates this twice per face, once from
 // each bounding cell; both evaluations are the same expression on the same
 // operands, so one evaluation serves both cells.
-// Lane s of a face's wavefront takes sublayer s: its targets are
-// non-decreasing in s for a monotone face, so the lanes read consecutive
-// records of the per-column reconstruction.  The kFaceGroup waves of a
-// workgroup take consecutive faces and read one line of each surface-major
-// face-record row between them; their fluxes are staged in LDS and written
-// back as whole rows (kFaceGroup faces x NT tracers per sublayer).
+// Lane l of a face's wavefront takes neutral surface s = c0 + l: it loads
+// the surface's record and, on each side, the record of the layer the
+// surface lies in, and forms the interface value there.  Sublayer s lies
+// between surfaces s and s+1, so lane l (< 63) takes the bottom-surface
+// values from lane l+1 by a wavefront shuffle; they are the same
+// expressions the reference evaluates for the sublayer's bottom.  A face's
+// targets are non-decreasing in s, so the lanes read consecutive records of
+// each column.  The kFaceGroup waves of a workgroup read one line of each
+// surface-major face-record row between them; their fluxes are staged in
+// LDS and written back as whole rows (kFaceGroup faces x NT tracers).
 template <int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
   __shared__ double sx[kFaceGroup * 64 * NT];
@@ -450,25 +530,52 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
     coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
   }
   const FaceRec* r = a.rec + (live ? fo * a.fa : 0);
+  const ReconRec* recL = a.recon + cl * a.ry.ca;
+  const ReconRec* recR = a.recon + cr * a.ry.ca;
+  const uint32_t cb = a.ry.cb, ct = a.ry.ct;
   const size_t fb = a.fb;
-  const int nsub = a.ns - 1;
-  for (int c0 = 0; c0 < nsub; c0 += 64) {
+  const int ns = a.ns, nsub = ns - 1;
+  for (int c0 = 0; c0 < nsub; c0 += kChunk) {
     const int s = c0 + lane;
+    const bool surf = live && s < ns;
+    FaceRec F{0.0, 0.0, 0.0, 1, 1};
+    if (surf) F = r[(size_t)s * fb];
+    double eL[NT], eR[NT];
+    ReconRec qL[NT], qR[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      qL[t] = ReconRec{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      qR[t] = qL[t];
+      if (surf) {
+        qL[t] = recL[(uint32_t)(F.KoL - 1) * cb + (uint32_t)t * ct];
+        qR[t] = recR[(uint32_t)(F.KoR - 1) * cb + (uint32_t)t * ct];
+      }
+      eL[t] = edge_value(qL[t], F.PoL);
+      eR[t] = edge_value(qR[t], F.PoR);
+    }
+    // surface s+1, the bottom of sublayer s
+    const double PoLb = __shfl_down(F.PoL, 1, 64), PoRb = __shfl_down(F.PoR, 1, 64);
+    const int KoLb = __shfl_down(F.KoL, 1, 64), KoRb = __shfl_down(F.KoR, 1, 64);
+    double eLb[NT], eRb[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      eLb[t] = __shfl_down(eL[t], 1, 64);
+      eRb[t] = __shfl_down(eR[t], 1, 64);
+    }
     uint32_t packed = 0;
     double x[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) x[t] = 0.0;
-    if (live && s < nsub) {
-      const FaceRec top = r[(size_t)s * fb];
-      if (top.hEff != 0.0) {
-        const FaceRec bot = r[(size_t)(s + 1) * fb];
-        packed = ((uint32_t)top.KoL << kTargetShiftL) | ((uint32_t)top.KoR << kTargetShiftR);
+    if (surf && lane < kChunk && s < nsub && F.hEff != 0.0) {
+      packed = ((uint32_t)F.KoL << kTargetShiftL) | ((uint32_t)F.KoR << kTargetShiftR);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const double d = sublayer_dT(top.KoL, bot.KoL, top.KoR, bot.KoR, top.PoL, bot.PoL,
-                                       top.PoR, bot.PoR, a.recon[t], a.ti[t], cl, cr, a.ry);
-          x[t] = (d * top.hEff) * coef;
-        }
+      for (int t = 0; t < NT; ++t) {
+        const double tlay = ppm_ave<double>(F.PoL, PoLb + (double)(KoLb - F.KoL), qL[t].aL,
+                                            qL[t].aR, qL[t].Tl);
+        const double trlay = ppm_ave<double>(F.PoR, PoRb + (double)(KoRb - F.KoR), qR[t].aL,
+                                             qR[t].aR, qR[t].Tl);
+        const double d = guarded_dT(eL[t], eLb[t], eR[t], eRb[t], tlay, trlay);
+        x[t] = (d * F.hEff) * coef;
       }
     }
 #pragma unroll
@@ -478,7 +585,7 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
     // row rr of the chunk, face q of the group
     const int rr = (int)(threadIdx.x / kFaceGroup), q = (int)(threadIdx.x % kFaceGroup);
     const int so = c0 + rr;
-    if (so < nsub) {
+    if (rr < kChunk && so < nsub) {
       const size_t o = (size_t)so * a.nfp + f0 + q;
 #pragma unroll
       for (int t = 0; t < NT; ++t) a.flx[o * NT + t] = sx[(q * 64 + rr) * NT + t];
@@ -517,7 +624,7 @@ template <int NT, int KCAP>
 __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
   __shared__ double acc[KCAP * 64];
   constexpr int kCells = 64 / NT;  // cells per wavefront
-  constexpr int kBatch = 8;        // sublayer rows in flight per face
+  constexpr int kBatch = 32;       // sublayer rows in flight per face
   const int lane = (int)threadIdx.x;
   const int cw = lane / NT, t = lane % NT;
   const int segs = (a.nx + kCells - 1) / kCells;
@@ -548,6 +655,11 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
     const size_t nf = f < 2 ? a.nu : a.nv;
     const uint32_t* tg = (f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
     const double* fx = (f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
+    // dTr(K) of the run of consecutive contributions to one layer K stays
+    // in a register; it is read from / written back to LDS when K changes,
+    // so every addition still lands in reference order
+    int curK = 0;
+    double cur = 0.0;
     for (int s0 = 0; s0 < nsub; s0 += kBatch) {
       uint32_t pk[kBatch];
       double x[kBatch];
@@ -563,10 +675,15 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
       for (int u = 0; u < kBatch; ++u) {
         if (pk[u] == 0) continue;
         const int K = (int)((pk[u] >> shift) & 0xffffu);
-        double& d = acc[(K - 1) * 64 + lane];
-        d = left ? d + x[u] : d - x[u];
+        if (K != curK) {
+          if (curK) acc[(curK - 1) * 64 + lane] = cur;
+          cur = acc[(K - 1) * 64 + lane];
+          curK = K;
+        }
+        cur = left ? cur + x[u] : cur - x[u];
       }
     }
+    if (curK) acc[(curK - 1) * 64 + lane] = cur;
   }
   const size_t n2 = (size_t)nx * ny;
   const size_t c = (size_t)ci + (size_t)nx * cj;

@@ -87,15 +87,14 @@ size_t padded_faces(size_t nf) { return (nf + kFaceGroup - 1) / kFaceGroup * kFa
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
 //   Phase A: ColRec + P per column interface (40 B).
-//   Phase B, per tracer slot: a ReconRec (32 B) per layer and an edge value
-//   (8 B) per interface of every column, and a flux (8 B) per sublayer of
-//   every (padded) u and v face; once: the packed targets (4 B) per sublayer
-//   and face, and a discard layer field (8 B per cell) that takes the second
-//   output of a two-tracer launch run for a lone tracer.
+//   Phase B, per tracer slot: a ReconRec (48 B) per layer of every column
+//   and a flux (8 B) per sublayer of every (padded) u and v face; once: the
+//   packed targets (4 B) per sublayer and face, and a discard layer field
+//   (8 B per cell) that takes the second output of a two-tracer launch run
+//   for a lone tracer.
 // Every array starts on a 256-byte boundary.
 struct PhaseBScratch {
-  ReconRec* recon;   // (col, k, t) at (col*nz + k)*slots_used + t
-  double* ti;        // (col, k, t) at (col*(nz+1) + k)*slots_used + t
+  ReconRec* recon;   // N2 * nz * slots records (ReconLayout)
   double* discard;   // N2 * nz
   double* flx_u;     // (ns - 1) * Nu' * slots, Nu' = padded u faces
   double* flx_v;     // (ns - 1) * Nv' * slots
@@ -116,7 +115,6 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots) {
   };
   PhaseBScratch s;
   s.recon = (ReconRec*)take(sizeof(ReconRec) * N2 * nz * slots);
-  s.ti = (double*)take(8 * N2 * (nz + 1) * slots);
   s.discard = (double*)take(8 * N2 * nz);
   s.flx_u = (double*)take(8 * nsub * nu * slots);
   s.flx_v = (double*)take(8 * nsub * nv * slots);
@@ -185,7 +183,7 @@ int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
 
 struct Tiles {
   int tx = 64, ty = 1;       // one 64-lane wavefront per tile
-  int recon_layer_major = 1; // reconstruction layer-major (1) or per column (0)
+  int recon_layer_major = 0; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
 };
 
@@ -295,21 +293,30 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
       const int tt = std::min(t, gn - 1);
       ts.q[t] = tracers[g0 + tt];
       ca.htr[t] = tracers[g0 + tt];
-      fu.recon[t] = fv.recon[t] = sc.recon + std::min(t, kn - 1);
-      fu.ti[t] = fv.ti[t] = sc.ti + std::min(t, kn - 1);
     }
+    fu.recon = fv.recon = sc.recon;
     if (gn == 1) ca.htr[1] = sc.discard;
-    const uint32_t k32 = (uint32_t)kn, n2k = (uint32_t)N2 * k32;
+    const uint32_t k32 = (uint32_t)kn;
     const ReconLayout ry = tiles.recon_layer_major
-                               ? ReconLayout{k32, n2k, k32, n2k}
-                               : ReconLayout{(uint32_t)nz * k32, k32, (uint32_t)(nz + 1) * k32, k32};
+                               ? ReconLayout{k32, (uint32_t)N2 * k32, 1}
+                               : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
     fu.ry = fv.ry = ry;
     slot = tm.begin(REDI_K_TRACER_RECON, st);
-    const unsigned nbr = blocks_for(N2, 128);
-    switch (kn) {
-      case 2: k_tracer_recon<2><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
-      case 3: k_tracer_recon<3><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
-      default: k_tracer_recon<4><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, sc.ti, ry); break;
+    if (tiles.recon_layer_major) {
+      const unsigned nbr = blocks_for(N2, 128);
+      switch (kn) {
+        case 2: k_tracer_recon<2><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+        case 3: k_tracer_recon<3><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+        default: k_tracer_recon<4><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+      }
+    } else {
+      const unsigned nbr = blocks_for(N2, kReconCols) * (unsigned)((nz + kReconLayers - 1) / kReconLayers);
+      const unsigned bsr = kReconCols * kReconLayers;
+      switch (kn) {
+        case 2: k_tracer_recon_tiled<2><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+        case 3: k_tracer_recon_tiled<3><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+        default: k_tracer_recon_tiled<4><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
+      }
     }
     tm.end(slot, st);
     for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces

@@ -56,12 +56,11 @@ def test_library_loads_and_host_only_calls(libpath):
     assert L.redi_version() >= 100
     # 10x8x4 domain (80 columns, 9 sublayers, 88 u and 90 v faces; face rows
     # padded to multiples of 4: 88 and 92), four tracer slots, every array
-    # rounded up to 256 B: reconstruction records and edges, the discard
-    # field, u and v sublayer fluxes, u and v targets
+    # rounded up to 256 B: 48-B reconstruction records, the discard field,
+    # u and v sublayer fluxes, u and v targets
     up = lambda b: -(-b // 256) * 256
-    parts = [32 * 80 * 4 * 4, 8 * 80 * 5 * 4, 8 * 80 * 4, 8 * 9 * 88 * 4, 8 * 9 * 92 * 4,
-             4 * 9 * 88, 4 * 9 * 92]
-    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 114944
+    parts = [48 * 80 * 4 * 4, 8 * 80 * 4, 8 * 9 * 88 * 4, 8 * 9 * 92 * 4, 4 * 9 * 88, 4 * 9 * 92]
+    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 122624
     assert L.redi_workspace_bytes(0, 8, 4) == 0
     assert L.redi_state_bytes(10, 8, 4) > L.redi_workspace_bytes(10, 8, 4)
     assert isinstance(redi.last_error(), str)

@@ -24,13 +24,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
 
-def _run_hip(case, nsteps, device, hoist=False, per_column=False, tiles=None, group=None):
+def _run_hip(case, nsteps, device, hoist=False, layer_major=False, tiles=None, group=None):
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     print(f"[gpu] {case.name} {case.nx}x{case.ny}x{case.nz} nt={case.nt} steps={nsteps}", flush=True)
     r, dom = make_redi(case, device)
-    if per_column:
-        r.set_recon_order(False)
+    if layer_major:
+        r.set_recon_order(True)
     if tiles:
         r.set_tiles(*tiles)
     if group:
@@ -121,12 +121,12 @@ def test_hip_matches_oracle_random_cases(seed, eos, nz, walls, use_ext, gpu_devi
 
 
 @pytest.mark.parametrize("name", ["roquet_mini", "wright_vanished_mini", "seamount_mini"])
-def test_hip_per_column_reconstruction_same_bits(name, gpu_device):
-    """The per-column storage order of the reconstruction gives the same
-    result as the layer-major default."""
+def test_hip_layer_major_reconstruction_same_bits(name, gpu_device):
+    """The layer-major storage order of the reconstruction gives the same
+    result as the per-column default."""
     case, ref = load_golden(name)
-    got = _run_hip(case, case.nsteps, gpu_device, per_column=True)
-    _compare(got, ref, name + "/per-column")
+    got = _run_hip(case, case.nsteps, gpu_device, layer_major=True)
+    _compare(got, ref, name + "/layer-major")
 
 
 @pytest.mark.parametrize("tiles,group", [((64, 1), 1), ((8, 8), 2), ((32, 4), 3), ((16, 16), 4),
@@ -150,9 +150,9 @@ def test_hip_tracer_counts_and_groups(nt, group, gpu_device):
     case = dataclasses.replace(base, hTr=base.hTr[:nt].copy(), idx_T=1, idx_S=min(2, nt),
                                name=f"roquet_two_fronts_mini_nt{nt}")
     o = redi_oracle.run_oracle(case, nsteps=1)
-    for per_column in (False, True):
-        got = _run_hip(case, 1, gpu_device, group=group, per_column=per_column)
-        assert np.array_equal(got["hTr1"], o["hTr1"]), (nt, group, per_column)
+    for layer_major in (False, True):
+        got = _run_hip(case, 1, gpu_device, group=group, layer_major=layer_major)
+        assert np.array_equal(got["hTr1"], o["hTr1"]), (nt, group, layer_major)
 
 
 def test_nonmonotone_faces_reported(gpu_device):
