@@ -50,14 +50,18 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 counter summary."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+def measured_traffic(kernel: str, workload: str, form: str = "fp64"):
+    """HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE) of
+    ``kernel`` from the committed rocprofv3 summary of this workload."""
+    p = os.path.join(ROOT, "profiles", f"kernels_{workload}_{form}.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         d = json.load(f)
-    return d.get("bytes_per_launch")
+    for k, e in d.get("kernels", {}).items():
+        if k.split("<")[0] == kernel and "traffic_bytes" in e:
+            return e["traffic_bytes"]
+    return None
 
 
 def cpu_baseline(case, sample_steps: int = 2):
@@ -153,7 +157,7 @@ def main() -> int:
 
     dom_k = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_ms = kt[dom_k][0] / kt[dom_k][1]
-    alg = byte_model.launch_bytes(case, dom_k, shape["group"])
+    alg = byte_model.launch_bytes(case, dom_k.split("<")[0], shape["group"])
     achieved = alg / (avg_ms * 1e-3) / 1e9
     step_gbs = byte_model.step_bytes(case) * args.steps / (ms_total * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

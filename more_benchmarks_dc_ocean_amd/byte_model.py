@@ -20,7 +20,7 @@ Each term is charged to the kernel that moves it:
     k_column_state      h
     k_neutral_sweep_u/v coefficient write of its face grid, wet of that grid
     k_face_khtr         khtr_u / khtr_v
-    k_tracer_recon      one tracer read (one launch per tracer)
+    k_tracer_recon      the group's tracer reads (one launch per tracer group)
     k_face_flux         coefficient read-back and dy_cu/idxCu (dx_cv/idyCv)
                         of its face grid, shared over the tracer groups
     k_cell_flux         the group's tracer writes, areaT shared over groups
@@ -75,15 +75,16 @@ def step_launches(case, group: int) -> Dict[str, list]:
         out["k_neutral_sweep_u"] = [coeff_bytes(d["Nu"], d["ns"]) + 8.0 * d["Nu"]]
     if d["ny"] > 1:
         out["k_neutral_sweep_v"] = [coeff_bytes(d["Nv"], d["ns"]) + 8.0 * d["Nv"]]
-    out["k_tracer_recon"] = [8.0 * d["N3"]] * nt
-    ff, cf = [], []
+    ff, cf, rc = [], [], []
     for gn in groups:
         share = gn / nt
+        rc.append(8.0 * d["N3"] * gn)
         if d["nx"] > 1:
             ff.append(share * (coeff_bytes(d["Nu"], d["ns"]) + 16.0 * d["Nu"]))
         if d["ny"] > 1:
             ff.append(share * (coeff_bytes(d["Nv"], d["ns"]) + 16.0 * d["Nv"]))
         cf.append(8.0 * d["N3"] * gn + share * 8.0 * d["N2"])
+    out["k_tracer_recon"] = rc
     out["k_face_flux"] = ff
     out["k_cell_flux"] = cf
     return out

@@ -60,6 +60,105 @@ def _rows(path_glob: str):
     return out
 
 
+# the device-to-device copy torch issues for the calibration tensor
+CALIBRATION_KERNEL = "__amd_rocclr_copyBuffer"
+GPU_CLOCK_HZ = 2.4e9   # MI355X peak engine clock (agent info: 2400 MHz)
+N_CU, N_SIMD = 256, 1024
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "").replace("redi::", "")
+
+
+def _base(name: str) -> str:
+    """The byte model's name for a kernel (template arguments and the
+    reconstruction's tiled variant folded in)."""
+    b = _short(name).split("<")[0]
+    return "k_tracer_recon" if b.startswith("k_tracer_recon") else b
+
+
+def summarize_dir(d: str, workload: str, copy_mib: int = 2048) -> dict:
+    """Per-kernel figures from one directory of runs: kt/ (kernel trace
+    --stats), fetch/ and write/ (FETCH_SIZE, WRITE_SIZE) and any other
+    --pmc passes (SQ_*, TA_*, TD_*, TCC_* counters), all per dispatch."""
+    from . import byte_model, inputs
+    from .autotune import TUNED, TuneCache
+    builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
+               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
+    case = builder(nsteps=1)
+    group = TuneCache(TUNED).lookup(case.nx, case.ny, case.nz).get("group", 4)
+    alg = {k: sum(v) / len(v) for k, v in byte_model.step_launches(case, group).items() if v}
+    stats = {}
+    for row in _rows(os.path.join(d, "kt", "*kernel_stats.csv")):
+        stats[_short(row["Name"])] = (int(row["Calls"]), float(row["AverageNs"]) * 1e-6)
+    counters: dict = {}
+    for row in _rows(os.path.join(d, "*", "*counter_collection.csv")):
+        k = _short(row["Kernel_Name"])
+        c = counters.setdefault(k, {}).setdefault(row["Counter_Name"], [])
+        c.append(float(row["Counter_Value"]))
+    per = {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in counters.items()}
+    cal = per.get(CALIBRATION_KERNEL, {})
+    fscale = copy_mib * 1024.0 / cal["FETCH_SIZE"] if cal.get("FETCH_SIZE") else None
+    wscale = copy_mib * 1024.0 / cal["WRITE_SIZE"] if cal.get("WRITE_SIZE") else None
+    out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale, "kernels": {}}
+    for k, (calls, ms) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        if not k.startswith("k_"):
+            continue
+        c = per.get(k, {})
+        cyc = ms * 1e-3 * GPU_CLOCK_HZ
+        e = {"calls": calls, "avg_ms": ms}
+        a = alg.get(_base(k) if _base(k) != "k_neutral_sweep" else "k_neutral_sweep_u")
+        if a is not None:
+            e["alg_bytes"] = a
+            e["alg_GBps"] = a / (ms * 1e-3) / 1e9
+        if "FETCH_SIZE" in c and fscale:
+            e["fetch_bytes"] = c["FETCH_SIZE"] * 1024.0 * fscale
+        if "WRITE_SIZE" in c and wscale:
+            e["write_bytes"] = c["WRITE_SIZE"] * 1024.0 * wscale
+        if "fetch_bytes" in e and "write_bytes" in e:
+            e["traffic_bytes"] = e["fetch_bytes"] + e["write_bytes"]
+            e["traffic_GBps"] = e["traffic_bytes"] / (ms * 1e-3) / 1e9
+        if "SQ_ACTIVE_INST_VALU" in c:
+            e["valu_busy"] = c["SQ_ACTIVE_INST_VALU"] * 4.0 / N_SIMD / cyc
+        if "TA_BUSY_avr" in c:
+            e["ta_busy"] = c["TA_BUSY_avr"] / cyc
+        if "TD_TD_BUSY_sum" in c:
+            e["td_busy"] = c["TD_TD_BUSY_sum"] / N_CU / cyc
+        if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c:
+            tot = c["TCC_HIT_sum"] + c["TCC_MISS_sum"]
+            e["l2_hit"] = c["TCC_HIT_sum"] / tot if tot else None
+        for n in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD"):
+            if n in c:
+                e[n] = c[n]
+        out["kernels"][k] = e
+    return out
+
+
+def markdown(summary: dict, title: str, commands: list) -> str:
+    f = lambda v, n=2: "" if v is None else f"{v:.{n}f}"
+    pct = lambda v: "" if v is None else f"{100 * v:.0f}%"
+    L = [f"# {title}", "", "Commands (repository root, one MI355X, `export TMPDIR=/tmp`):", "", "```"]
+    L += commands + ["```", ""]
+    L.append(f"Calibration: the driver's {2048} MiB device copy (`{CALIBRATION_KERNEL}`) reads "
+             f"FETCH_SIZE x {f(summary['fetch_scale'], 5)} and WRITE_SIZE x "
+             f"{f(summary['write_scale'], 5)} of the bytes it moves; the traffic columns are "
+             "scaled by these factors.  `alg` is the kernel's share of the algorithmic bytes of "
+             "one step (compulsory traffic of the two-call contract), per launch.  Busy "
+             "fractions are per dispatch over the kernel's average duration at 2.4 GHz.")
+    L += ["", "| kernel | calls | avg ms | alg GB | alg GB/s | fetch GB | write GB | traffic GB/s | "
+          "VALU busy | TA busy | TD busy | L2 hit |", "|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for k, e in summary["kernels"].items():
+        g = lambda n: e.get(n)
+        L.append(f"| `{k}` | {e['calls']} | {f(e['avg_ms'], 3)} | "
+                 f"{f(g('alg_bytes') / 1e9 if g('alg_bytes') is not None else None)} | "
+                 f"{f(g('alg_GBps'), 0)} | "
+                 f"{f(g('fetch_bytes') / 1e9 if g('fetch_bytes') is not None else None)} | "
+                 f"{f(g('write_bytes') / 1e9 if g('write_bytes') is not None else None)} | "
+                 f"{f(g('traffic_GBps'), 0)} | {pct(g('valu_busy'))} | {pct(g('ta_busy'))} | "
+                 f"{pct(g('td_busy'))} | {pct(g('l2_hit'))} |")
+    return "\n".join(L) + "\n"
+
+
 def summarize_counters(fetch_glob: str, write_glob: str, copy_mib: int) -> dict:
     """Per-kernel average FETCH_SIZE / WRITE_SIZE per dispatch (KiB), scaled by
     the calibration copy (whose true traffic is copy_mib MiB each way)."""
@@ -73,7 +172,7 @@ def summarize_counters(fetch_glob: str, write_glob: str, copy_mib: int) -> dict:
             v = float(row.get("Counter_Value", 0.0))
             acc.setdefault(k, []).append(v)
         res[name] = {k: sum(v) / len(v) for k, v in acc.items()}
-    copy_k = [k for k in res["FETCH_SIZE"] if "copy" in k.lower() or "elementwise" in k.lower()]
+    copy_k = [k for k in res["FETCH_SIZE"] if k.startswith(CALIBRATION_KERNEL)]
     scale = {}
     for name in res:
         ck = [k for k in res[name] if k in copy_k]
@@ -92,7 +191,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--copy-mib", type=int, default=2048)
     ap.add_argument("--summarize", nargs=2, metavar=("FETCH_GLOB", "WRITE_GLOB"))
+    ap.add_argument("--summarize-dir", metavar="DIR",
+                    help="directory with kt/, fetch/, write/ and other --pmc pass outputs")
+    ap.add_argument("--json", metavar="FILE", help="with --summarize-dir: also write the figures")
     args = ap.parse_args()
+    if args.summarize_dir:
+        summ = summarize_dir(args.summarize_dir, args.workload, args.copy_mib)
+        if args.json:
+            with open(args.json, "w") as fh:
+                json.dump(summ, fh, indent=1)
+        print(json.dumps(summ, indent=1))
+        return 0
     if args.summarize:
         print(json.dumps(summarize_counters(args.summarize[0], args.summarize[1], args.copy_mib),
                          indent=1))
