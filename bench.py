@@ -8,6 +8,12 @@ redi_apply_flux over all four tracers (Phase B), every tracer updated in
 place.  Inputs are generated on the host, copied to HBM, and resident there
 before the timed region starts.
 
+Both precision forms are timed on the same inputs, one after the other:
+the plain FP64 form gives ``value``; the FP32-store form (fields, face
+records and fluxes stored in FP32) is reported under ``forms.fp32_store``
+with its own roofline and its ratio to the FP64 form.  ``--form`` times one
+form only.
+
 Printed JSON line: metric "Redi Mcell-steps/s" (interior cells x steps per
 second), plus
   roofline      the dominant kernel's algorithmic bytes per launch divided by
@@ -22,6 +28,7 @@ second), plus
                 restatement when that binary is absent.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload two_fronts_1440x1080x75]
+                       [--form both|fp64|fp32_store]
 """
 from __future__ import annotations
 
@@ -103,45 +110,30 @@ def cpu_baseline(case, sample_steps: int = 2):
             "kind": "port", "sample": sample, "ms_per_step": ms}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="two_fronts_1440x1080x75", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the Redi step of one "
-                                   "domain is not split across GPUs (--gpus must be 1)"}))
-        return 2
-
+def time_form(case, form: str, steps: int, warmup: int, workload: str) -> dict:
+    """Time ``steps`` Redi steps of ``form`` on resident inputs; the
+    dominant kernel's roofline from HIP events around its launches."""
     import torch
-    from more_benchmarks_dc_ocean_amd import byte_model, inputs
+    from more_benchmarks_dc_ocean_amd import byte_model
     from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
     from more_benchmarks_dc_ocean_amd.redi import make_redi
 
-    builder, wname = WORKLOADS[args.workload]
-    t0 = time.time()
-    case = getattr(inputs, builder)(nsteps=args.steps)
-    log(f"generated {wname} ({case.nx}x{case.ny}x{case.nz}, nt={case.nt}, "
-        f"land={case.land_fraction:.3f}) in {time.time() - t0:.1f}s")
     dev = "cuda:0"
-    r, dom = make_redi(case, dev)
+    r, dom = make_redi(case, dev, form)
     shape = TuneCache(TUNED).apply(r, case.nx, case.ny, case.nz)
-    log(f"launch shape {shape}")
+    log(f"[{form}] launch shape {shape}")
     stream = torch.cuda.current_stream(dev)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         r.step(dom, 1)
     torch.cuda.synchronize()
-    log(f"warm-up {args.warmup} steps done")
+    log(f"[{form}] warm-up {warmup} steps done")
 
     r.timing(True)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     ev0.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         r.step(dom, 1)
     ev1.record(stream)
     torch.cuda.synchronize()
@@ -149,27 +141,58 @@ def main() -> int:
     ms_total = ev0.elapsed_time(ev1)
     kt = r.collect_timing()
     r.timing(False)
-    ms_per_step = ms_total / args.steps
-    value = case.interior_cells * args.steps / (ms_total * 1e-3) / 1e6
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall * 1e3 / args.steps:.3f})")
+    ms_per_step = ms_total / steps
+    value = case.interior_cells * steps / (ms_total * 1e-3) / 1e6
+    log(f"[{form}] timed {steps} steps: {ms_per_step:.3f} ms/step (wall {wall * 1e3 / steps:.3f})")
     for k, (ms, n) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
         log(f"  {k:20s} {ms / n:9.4f} ms/launch x{n:4d}  ({100 * ms / ms_total:5.1f}% of step time)")
 
     dom_k = max(kt.items(), key=lambda kv: kv[1][0])[0]
     avg_ms = kt[dom_k][0] / kt[dom_k][1]
-    alg = byte_model.launch_bytes(case, dom_k.split("<")[0], shape["group"])
+    alg = byte_model.launch_bytes(case, dom_k.split("<")[0], shape["group"], form)
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    step_gbs = byte_model.step_bytes(case) * args.steps / (ms_total * 1e-3) / 1e9
+    step_gbs = byte_model.step_bytes(case, form) * steps / (ms_total * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(dom_k, args.workload),
+            "traffic": measured_traffic(dom_k, workload, form),
             "alg_bytes_per_launch": alg, "avg_launch_ms": round(avg_ms, 4),
-            "launches_per_step": kt[dom_k][1] // args.steps,
-            "step_alg_bytes_per_cell": round(byte_model.bytes_per_cell_step(case), 2),
+            "launches_per_step": kt[dom_k][1] // steps,
+            "step_alg_bytes_per_cell": round(byte_model.bytes_per_cell_step(case, form), 2),
             "step_alg_GBps": round(step_gbs, 2), "step_frac": round(step_gbs / HBM_PEAK_GBS, 5)}
+    kernels = {k: round(ms / n, 4) for k, (ms, n) in kt.items()}
     r.close()
     del dom
     torch.cuda.empty_cache()
+    return {"value": round(value, 3), "ms_per_step": round(ms_per_step, 4), "roofline": roof,
+            "launch_shape": shape, "kernel_ms_per_launch": kernels}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="two_fronts_1440x1080x75", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", default="both", choices=["both", "fp64", "fp32_store"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the Redi step of one "
+                                   "domain is not split across GPUs (--gpus must be 1)"}))
+        return 2
+
+    from more_benchmarks_dc_ocean_amd import inputs
+
+    builder, wname = WORKLOADS[args.workload]
+    t0 = time.time()
+    case = getattr(inputs, builder)(nsteps=args.steps)
+    log(f"generated {wname} ({case.nx}x{case.ny}x{case.nz}, nt={case.nt}, "
+        f"land={case.land_fraction:.3f}) in {time.time() - t0:.1f}s")
+    res = {}
+    for form in (["fp64", "fp32_store"] if args.form == "both" else [args.form]):
+        res[form] = time_form(case, form, args.steps, args.warmup, args.workload)
+    main_form = "fp64" if "fp64" in res else args.form
+    m = res[main_form]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -179,15 +202,20 @@ def main() -> int:
         except Exception as e:
             log(f"cpu baseline failed: {e}")
 
-    line = {"metric": "Redi Mcell-steps/s", "value": round(value, 3), "unit": "Mcell-steps/s",
+    line = {"metric": "Redi Mcell-steps/s", "value": m["value"], "unit": "Mcell-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if main_form == "fp64" else "f32-store/f64",
+            "data": "synthetic",
             "config": {"workload": wname, "nx_phys": case.nx_phys, "ny_phys": case.ny_phys,
                        "nz": case.nz, "ntracers": case.nt, "eos": "roquet_spv",
-                       "land_fraction": round(case.land_fraction, 4), "form": "fp64",
-                       "parallelism": "single_gpu", "launch_shape": shape},
-            "roofline": roof, "cpu_baseline": cpu}
+                       "land_fraction": round(case.land_fraction, 4), "form": main_form,
+                       "parallelism": "single_gpu", "launch_shape": m["launch_shape"]},
+            "roofline": m["roofline"], "cpu_baseline": cpu}
+    if "fp64" in res and "fp32_store" in res:
+        f32 = dict(res["fp32_store"])
+        f32["ratio_vs_fp64"] = round(f32["value"] / res["fp64"]["value"], 4)
+        line["forms"] = {"fp32_store": f32}
     print(json.dumps(line), flush=True)
     return 0
 

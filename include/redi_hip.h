@@ -37,6 +37,12 @@ extern "C" {
 
 #define REDI_OBC_WALL 1
 
+/* Precision forms of a redi_t (redi_init_form). */
+#define REDI_FORM_FP64 0       /* the plain form: matches the reference bits */
+#define REDI_FORM_FP32_STORE 1 /* fields, records and fluxes in FP32; density
+                                  differences, neutral positions and the tracer-
+                                  tendency accumulation in FP64 */
+
 /* hgrid_t (redi/grid.F90:17-25) */
 typedef struct {
     int nx_total, ny_total; /* incl. ghosts: n_phys + 2*nghost */
@@ -75,6 +81,16 @@ typedef struct {
     double *const *tracers;
 } redi_state_t;
 
+/* The same state with FP32 fields, for REDI_FORM_FP32_STORE. */
+typedef struct {
+    int nz_ml;
+    int idx_temperature;
+    int idx_salinity;
+    int ntracers;
+    const float *h_layer;
+    float *const *tracers;
+} redi_state32_t;
+
 /* ocean_bc_state_t (redi/ocean_boundary_types.F90:14-22): per-edge tags,
  * REDI_OBC_WALL = no-normal-flow wall. */
 typedef struct {
@@ -103,6 +119,13 @@ typedef struct redi_s redi_t;
  * Allocates the Phase-A state in HBM and fills it like the reference's
  * `allocate(..., source=0 / 1)`.  Synchronous. */
 int redi_init(redi_t **self, const redi_hgrid_t *grid, int nz_ml);
+
+/* redi_init for a precision form (REDI_FORM_*).  A state of one form takes
+ * only that form's entries (redi_calc_coeffs / redi_apply_flux / redi_step
+ * for REDI_FORM_FP64, the _fp32 entries for REDI_FORM_FP32_STORE).  This
+ * and the _fp32 entries have no counterpart in the reference, whose state
+ * is FP64 only. */
+int redi_init_form(redi_t **self, const redi_hgrid_t *grid, int nz_ml, int form);
 
 /* ocean_redi_destroy (redi/ocean_redi.F90:509-529).  Synchronous. */
 int redi_destroy(redi_t *self);
@@ -135,6 +158,14 @@ int redi_set_tracer_group(redi_t *self, int n);
  * result. */
 int redi_set_recon_order(redi_t *self, int layer_major);
 
+/* Optional FP64 output of the tracer tendency dTr * (1/areaT) (the term
+ * redi_apply_flux adds to hTr, :1176-1180) of every later redi_apply_flux:
+ * tend[t] is a device array (nx, ny, nz) for tracer t; ntracers = 0 turns
+ * it off.  No counterpart in the reference (its dTr is a local array); it
+ * lets the FP32-store form's tendency be compared before it is rounded
+ * into the FP32 state. */
+int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
+
 /* Faces of the last redi_calc_coeffs whose contributing sublayers are not
  * monotone in layer index (u and v faces).  Synchronous. */
 int redi_nonmonotone_faces(const redi_t *self, long long *n_u, long long *n_v);
@@ -151,6 +182,21 @@ int redi_apply_flux(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
                     redi_t *self, const redi_state_t *ms, double dt,
                     const double *khtr_u_ext, const double *khtr_v_ext,
                     const redi_bc_t *bc, void *stream);
+
+/* redi_calc_coeffs / redi_apply_flux / redi_step of the FP32-store form:
+ * same contracts, FP32 h and tracers (the metrics, khtr_*_ext and dt stay
+ * FP64).  Results are within the form's stated error of the FP64 form. */
+int redi_calc_coeffs_fp32(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+                          const redi_eos_t *eos, redi_t *self,
+                          const redi_state32_t *ms, void *stream);
+int redi_apply_flux_fp32(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+                         redi_t *self, const redi_state32_t *ms, double dt,
+                         const double *khtr_u_ext, const double *khtr_v_ext,
+                         const redi_bc_t *bc, void *stream);
+int redi_step_fp32(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
+                   const redi_eos_t *eos, redi_t *self, const redi_state32_t *ms,
+                   double dt, const double *khtr_u_ext, const double *khtr_v_ext,
+                   const redi_bc_t *bc, int nsteps, void *stream);
 
 /* redi_apply_flux_hoist (redi/ocean_redi.F90:1201-1262): same contract and
  * same bits as redi_apply_flux; here both run the same kernels, since the
@@ -173,6 +219,7 @@ int redi_step(const redi_hgrid_t *grid, const redi_metrics_t *metrics,
  * holds (scratch plus the Phase-A face state). */
 long long redi_workspace_bytes(int nx_total, int ny_total, int nz);
 long long redi_state_bytes(int nx_total, int ny_total, int nz);
+long long redi_state_bytes_form(int nx_total, int ny_total, int nz, int form);
 
 /* ---- per-kernel timing ---------------------------------------------------
  * When enabled, every kernel launch made for this ocean_redi_t is bracketed

@@ -63,36 +63,38 @@ def bytes_per_cell_step(case, form: str = "fp64") -> float:
     return step_bytes(case, form) / case.interior_cells
 
 
-def step_launches(case, group: int) -> Dict[str, list]:
+def step_launches(case, group: int, form: str = "fp64") -> Dict[str, list]:
     """Algorithmic bytes of every launch of one step, per kernel name, for
-    tracer groups of ``group`` tracers."""
+    tracer groups of ``group`` tracers, with fields and coefficients of
+    ``form``'s element sizes (the 2-D metrics are FP64 in every form)."""
     d = _dims(case)
     nt = case.nt
+    s = float(ELEMENT_BYTES[form][0])
     g = max(1, min(group, 4, nt))
     groups = [min(g, nt - g0) for g0 in range(0, nt, g)]
-    out: Dict[str, list] = {"k_column_state": [8.0 * d["N3"]], "k_face_khtr": [8.0 * (d["Nu"] + d["Nv"])]}
+    out: Dict[str, list] = {"k_column_state": [s * d["N3"]], "k_face_khtr": [8.0 * (d["Nu"] + d["Nv"])]}
     if d["nx"] > 1:
-        out["k_neutral_sweep_u"] = [coeff_bytes(d["Nu"], d["ns"]) + 8.0 * d["Nu"]]
+        out["k_neutral_sweep_u"] = [coeff_bytes(d["Nu"], d["ns"], form) + 8.0 * d["Nu"]]
     if d["ny"] > 1:
-        out["k_neutral_sweep_v"] = [coeff_bytes(d["Nv"], d["ns"]) + 8.0 * d["Nv"]]
+        out["k_neutral_sweep_v"] = [coeff_bytes(d["Nv"], d["ns"], form) + 8.0 * d["Nv"]]
     ff, cf, rc = [], [], []
     for gn in groups:
         share = gn / nt
-        rc.append(8.0 * d["N3"] * gn)
+        rc.append(s * d["N3"] * gn)
         if d["nx"] > 1:
-            ff.append(share * (coeff_bytes(d["Nu"], d["ns"]) + 16.0 * d["Nu"]))
+            ff.append(share * (coeff_bytes(d["Nu"], d["ns"], form) + 16.0 * d["Nu"]))
         if d["ny"] > 1:
-            ff.append(share * (coeff_bytes(d["Nv"], d["ns"]) + 16.0 * d["Nv"]))
-        cf.append(8.0 * d["N3"] * gn + share * 8.0 * d["N2"])
+            ff.append(share * (coeff_bytes(d["Nv"], d["ns"], form) + 16.0 * d["Nv"]))
+        cf.append(s * d["N3"] * gn + share * 8.0 * d["N2"])
     out["k_tracer_recon"] = rc
     out["k_face_flux"] = ff
     out["k_cell_flux"] = cf
     return out
 
 
-def launch_bytes(case, kernel: str, group: int) -> float:
+def launch_bytes(case, kernel: str, group: int, form: str = "fp64") -> float:
     """Mean algorithmic bytes per launch of ``kernel`` in one step."""
-    v = step_launches(case, group).get(kernel, [])
+    v = step_launches(case, group, form).get(kernel, [])
     return sum(v) / len(v) if v else 0.0
 
 

@@ -1,39 +1,56 @@
-// redi_kernels_fp64.hpp -- the plain FP64 Redi step as gfx950 kernels.
+// redi_kernels.hpp -- the Redi step as gfx950 kernels, in two precision forms.
 //
-//   k_column_state   Phase A, once per column: top-down interface pressure,
-//                    PPM T/S edges and dR/dT, dR/dS from the EOS.
-//   k_neutral_sweep  Phase A, once per wet face: the continuous neutral
-//                    surface sweep over the two adjacent columns.
-//   k_tracer_recon   Phase B, once per column and tracer: layer averages,
-//                    PPM edges and limited left/right edge values.
-//   k_face_flux      Phase B, once per face for up to NT tracers: the flux
-//                    of every contributing sublayer and its target layers.
-//   k_cell_flux      Phase B, once per cell: the four bounding faces'
-//                    sublayer fluxes added into the cell's layers in the
-//                    reference's order, then hTr += dTr / areaT.
-//   k_export_coeffs  face records -> the reference's Po/Ko/hEff arrays.
+//   k_column_state        Phase A, once per column: top-down interface
+//                         pressure, PPM T/S edges and dR/dT, dR/dS (EOS).
+//   k_neutral_sweep       Phase A, once per wet face: the continuous neutral
+//                         surface sweep over the two adjacent columns.
+//   k_tracer_recon_tiled  Phase B, per tracer group: layer averages, PPM edges
+//                         and limited edge values of every column.
+//   k_face_flux           Phase B, once per face for up to NT tracers: the flux
+//                         of every contributing sublayer and its target layers.
+//   k_cell_flux           Phase B, once per cell: the four bounding faces'
+//                         sublayer fluxes added into the cell's layers in the
+//                         reference's order, then hTr += dTr / areaT.
+//   k_export_coeffs       face records -> the reference's Po/Ko/hEff arrays.
 //
-// Device memory layout:
+// Every kernel is a template on a precision form F:
+//   Fp64  the plain form: fields, records and arithmetic in FP64, in the
+//         reference's rounding order, so results match the reference bits;
+//   Fp32  the FP32-store form: the fields h / hTr, the face records, the
+//         per-step reconstruction and the sublayer fluxes are stored in
+//         32-bit words (FP32 values, int16 layer indices, positions as
+//         32-bit fractions; the reconstruction as an FP64 layer mean plus
+//         FP32 offsets of its edge values, so differences between columns
+//         keep FP64 accuracy).  The arithmetic stays FP64: every difference
+//         of nearly equal values -- the T/S edges, EOS derivatives and
+//         density differences that place the neutral surfaces, the tracer
+//         differences across a face, the tendency accumulation -- is
+//         formed in FP64 from the stored values.
+// Both forms come from one source: records convert through get() / put(),
+// which are identities for Fp64.
+//
+// Device memory layout (R = the form's storage type):
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
-//   column state (K1 -> K2)     per column contiguous: ColRec[col][kt], P[col][kt],
-//                               kt = top-down interface 0..nz
+//   column state (K1 -> K2)     FP64 in both forms, per column contiguous:
+//                               ColRec[col][kt], P[col][kt], kt = top-down
+//                               interface 0..nz
 //   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
 //                               surface-major: (face, s) at s*Nf' + face with
 //                               rows padded to Nf' = a multiple of kFaceGroup
 //                               faces; one monotonicity flag per face
-//   tracer recon (K3 -> K4)     ReconRec (layer mean, limited edges, top edge)
-//                               per layer, column and tracer of a group, and
-//                               the interface edges Ti; per column, tracers
-//                               interleaved: (col, k, t) at (col*nz + k)*NT + t
+//   tracer recon (K3 -> K4)     ReconRec (layer mean, interface values at the
+//                               layer's top and bottom; FP64 in both forms)
+//                               per layer, column and tracer of a group; each
+//                               tracer's column contiguous by default
 //   sublayer fluxes (K4 -> K5)  surface-major: packed targets [s][face] and
 //                               fluxes [s][face][tracer], rows of Nf' faces
 // The sweep reads each of its two columns once, front to back, while lanes
 // sit at unrelated depths, so the column state is stored per column and
 // each lane streams whole 128-byte lines of its own columns; its lanes are
 // adjacent faces, so its surface-major writes are contiguous.  The face flux
-// kernel gives each face one wavefront with a lane per sublayer: a face's
-// targets are non-decreasing in s, so the lanes walk down the two columns
-// together and read consecutive per-column records.
+// kernel gives each face one wavefront with a lane per neutral surface: a
+// face's targets are non-decreasing in s, so the lanes walk down the two
+// columns together and read consecutive per-column records.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,21 +61,86 @@
 
 namespace redi {
 
-struct __align__(16) ColRec {  // one interface of a column (K1 output)
-  double T, S, dT, dS;         // interface T, S, dR/dT, dR/dS
+struct alignas(16) ColRec {  // one interface of a column (K1 output)
+  double T, S, dT, dS;        // interface T, S, dR/dT, dR/dS
 };
 
-struct __align__(16) FaceRec {  // one neutral surface s of a face (K2 output)
-  double PoL, PoR;              // positions in the left / right column
-  double hEff;                  // effective thickness of sublayer s (m); 0 for s = ns-1
-  int32_t KoL, KoR;             // 1-based top-down layer indices
+// One neutral surface s of a face (K2 output), as the kernels use it:
+// positions in the left / right column (in [0, 1]), effective thickness of
+// sublayer s (m; 0 for s = ns-1), 1-based top-down layer indices.
+struct SurfVals {
+  double PoL, PoR, hEff;
+  int KoL, KoR;
 };
 
-struct __align__(16) ReconRec {  // one layer of a tracer column (K3 output)
-  double Tl, aL, aR;             // layer mean, limited top / bottom edge values
-  double Tt, Tb;                 // interface values at the layer's top / bottom
-  double pad;
+struct alignas(16) FaceRec64 {  // stored exactly
+  double PoL, PoR, hEff;
+  int32_t KoL, KoR;
+  __device__ __forceinline__ SurfVals get() const { return SurfVals{PoL, PoR, hEff, KoL, KoR}; }
+  static __device__ __forceinline__ FaceRec64 put(const SurfVals& v) {
+    return FaceRec64{v.PoL, v.PoR, v.hEff, v.KoL, v.KoR};
+  }
+  static __device__ __forceinline__ double stored_heff(double h) { return h; }
 };
+
+// Positions as unsigned fractions of 2^31: 0 and 1 are exact and the
+// absolute resolution is 2^-31 over the whole interval (an FP32 position
+// near 1 resolves only 2^-24), in the same four bytes.  Sublayer
+// interfaces sit at these positions, and the down-gradient guard compares
+// interface values formed from them, so their resolution decides how
+// often a guard of the FP32-store form decides differently from FP64.
+struct alignas(16) FaceRec32 {
+  uint32_t PoL, PoR;
+  float hEff;
+  int16_t KoL, KoR;
+  static __device__ __forceinline__ uint32_t enc(double po) {
+    return (uint32_t)__double2uint_rn(po * 0x1p31);
+  }
+  static __device__ __forceinline__ double dec(uint32_t q) { return (double)q * 0x1p-31; }
+  __device__ __forceinline__ SurfVals get() const {
+    return SurfVals{dec(PoL), dec(PoR), (double)hEff, KoL, KoR};
+  }
+  static __device__ __forceinline__ FaceRec32 put(const SurfVals& v) {
+    return FaceRec32{enc(v.PoL), enc(v.PoR), (float)v.hEff, (int16_t)v.KoL, (int16_t)v.KoR};
+  }
+  static __device__ __forceinline__ double stored_heff(double h) { return (double)(float)h; }
+};
+
+// One layer of a tracer column, as the face kernel needs it: layer mean,
+// limited top / bottom edge values, interface values at the layer's top /
+// bottom.
+struct LayerVals {
+  double Tl, aL, aR, Tt, Tb;
+};
+
+// One layer of a tracer column (K3 output), FP64 in both forms: the layer
+// mean and the PPM interface values at its top and bottom.  The limited
+// edge values are not stored: each branch of ppm_limit yields them from
+// exactly these three values (unchanged, flattened to the mean, or one
+// edge rebuilt from the mean and the other), so the reader re-runs the
+// limiter on the same operands and obtains the same bits.  Every value a
+// guard compares across a face is then exact in both forms -- the sweep
+// places neutral surfaces where the T edges of the two columns nearly
+// agree, so rounding them would flip guards.
+struct ReconRec {
+  double Tl, Tt, Tb;
+  __device__ __forceinline__ LayerVals get() const {
+    double al = Tt, ar = Tb;
+    ppm_limit<double>(Tl, al, ar);
+    return LayerVals{Tl, al, ar, Tt, Tb};
+  }
+};
+
+struct Fp64 {
+  using R = double;  // stored fields and fluxes
+  using Face = FaceRec64;
+};
+struct Fp32 {
+  using R = float;
+  using Face = FaceRec32;
+};
+
+template <class F> using FaceRecF = typename F::Face;
 
 // Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
 // non-decreasing in s on the left (bit 0) / right (bit 1) column.
@@ -90,9 +172,15 @@ struct TileMap {
 // ---------------------------------------------------------------------------
 // K1: column interface state (redi_build_column, redi/ocean_redi.F90:609-654)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_column_state(int N2, int nz, const double* __restrict__ h,
-                                                      const double* __restrict__ thtr,
-                                                      const double* __restrict__ shtr,
+// FP64 in both forms, the EOS included (the FP32-store form reads FP32
+// fields): the sweep's density differences are differences of these
+// derivatives' products, and FP32 derivatives move neutral surfaces into
+// other layers.
+template <class F>
+__global__ void __launch_bounds__(128) k_column_state(int N2, int nz,
+                                                      const typename F::R* __restrict__ h,
+                                                      const typename F::R* __restrict__ thtr,
+                                                      const typename F::R* __restrict__ shtr,
                                                       EosParams eos, ColRec* __restrict__ rec,
                                                       double* __restrict__ Pout) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,13 +192,13 @@ __global__ void __launch_bounds__(128) k_column_state(int N2, int nz, const doub
   double P = 0.0;  // Pint(1) = 0
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
-    hh = h[o];
+    hh = (double)h[o];
     const double he = hh > kHDivEps ? hh : kHDivEps;
-    s[0] = thtr[o] / he;
-    s[1] = shtr[o] / he;
+    s[0] = (double)thtr[o] / he;
+    s[1] = (double)shtr[o] / he;
   };
   auto emit = [&](int k, const double* e, const double*) {
-    if (k > 0) P = P + g_rho0 * h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
+    if (k > 0) P = P + g_rho0 * (double)h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
     double dRdT, dRdS;
     interface_drho<double>(eos, e[0], e[1], P, dRdT, dRdS);
     r[k] = ColRec{e[0], e[1], dRdT, dRdS};
@@ -147,8 +235,8 @@ struct SweepSide {
     p0 = p[0]; p1 = p[1]; p2 = p[nk < 2 ? nk : 2];
   }
   __device__ __forceinline__ ColRec cur() const { return k == 1 ? w0 : w1; }  // record at k
-  __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }   // P(k)
-  __device__ __forceinline__ double pk1() const { return k == 1 ? p1 : p2; }  // P(k+1)
+  __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }        // P(k)
+  __device__ __forceinline__ double pk1() const { return k == 1 ? p1 : p2; }       // P(k+1)
   __device__ __forceinline__ void advance(int nk) {  // k -> k+1, k <= nk
     if (k >= 2) {
       m += 1;
@@ -159,6 +247,7 @@ struct SweepSide {
   }
 };
 
+template <class F>
 struct SweepArgs {
   int nx, nz;
   int nfa;                 // faces per row of the face array
@@ -168,42 +257,47 @@ struct SweepArgs {
   const double* wet;
   const ColRec* rec;
   const double* P;
-  FaceRec* out;            // [s][face]
+  FaceRecF<F>* out;        // [s][face]
   uint8_t* mono;           // [face]
   double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
 };
 
-__global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) {
+// Density differences, positions, pressures and thicknesses are carried in
+// FP64 in both forms; positions and thicknesses are stored in the form's
+// type.
+template <class F>
+__global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap tm) {
+  using Face = FaceRecF<F>;
   int fi, fj;
   if (!tm.map(fi, fj)) return;
   const int i = a.i0 + fi, j = a.j0 + fj;
   const size_t fo = (size_t)i + (size_t)a.nfa * j;
   const int nk = a.nz, ns = 2 * nk + 2;
-  FaceRec* o = a.out + fo * a.fa;
+  FaceRecF<F>* o = a.out + fo * a.fa;
   const size_t fst = a.fb;
   if (!(a.wet[fo] > 0.0)) {  // dry face: the inert values of :722-730
-    for (int s = 0; s < ns; ++s) o[s * fst] = FaceRec{0.0, 0.0, 0.0, 1, 1};
+    for (int s = 0; s < ns; ++s) o[s * fst] = Face::put(SurfVals{0.0, 0.0, 0.0, 1, 1});
     a.mono[fo] = kMonoL | kMonoR;
     return;
   }
   const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
   const size_t cr = (size_t)i + (size_t)a.nx * j;
-  SweepSide L, R;
+  SweepSide L, Rt;
   L.init(a.rec + cl * (nk + 1), a.P + cl * (nk + 1), nk);
-  R.init(a.rec + cr * (nk + 1), a.P + cr * (nk + 1), nk);
+  Rt.init(a.rec + cr * (nk + 1), a.P + cr * (nk + 1), nk);
 
   int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
   double lastP_l = 0.0, lastP_r = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;
   bool reached_bottom = false, search_l = false, mono_l = true, mono_r = true;
-  FaceRec pend{};
+  SurfVals pend{};
   for (int ks = 1; ks <= ns; ++ks) {
     if (!reached_bottom) {
-      const double dRho = sec_drho(R.cur(), L.cur());
+      const double dRho = sec_drho(Rt.cur(), L.cur());
       if (dRho < 0.0)
         search_l = true;
       else if (dRho > 0.0)
         search_l = false;
-      else if (L.k + R.k == 2)
+      else if (L.k + Rt.k == 2)
         search_l = true;
       else
         search_l = !search_l;
@@ -211,62 +305,62 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
     double PoL, PoR, posL, posR;
     int KoL, KoR;
     if (search_l) {
-      const ColRec cr_ = R.cur();
+      const ColRec cr_ = Rt.cur();
       const double top = sec_drho(L.w0, cr_);
       const double bot = sec_drho(L.w1, cr_);
       double Po;
-      if (top > 0.0 || R.k + L.k == 2)
+      if (top > 0.0 || Rt.k + L.k == 2)
         Po = 0.0;
       else if (top >= bot)
         Po = 1.0;
       else
         Po = interpolate_position<double>(top, L.p0, bot, L.p1);
-      int K = L.m;
+      int Kn = L.m;
       double pa = L.p0, pb = L.p1;
-      if (Po >= 1.0 && K < nk) {
-        K += 1;
+      if (Po >= 1.0 && Kn < nk) {
+        Kn += 1;
         Po = Po - 1.0;
         pa = L.p1;
         pb = L.p2;
       }
-      if ((double)(K - lastK_l) + (Po - lastP_l) < 0.0) {  // monotonic backstop
+      if ((double)(Kn - lastK_l) + (Po - lastP_l) < 0.0) {  // monotonic backstop
         PoL = lastP_l; KoL = lastK_l; posL = lastPos_l;
       } else {
-        PoL = Po; KoL = K; posL = pa + Po * (pb - pa);
+        PoL = Po; KoL = Kn; posL = pa + Po * (pb - pa);
       }
-      if (R.k <= nk) {
-        PoR = 0.0; KoR = R.k;
-        const double q = R.pk();
-        posR = q + 0.0 * (R.pk1() - q);
-        R.advance(nk);
+      if (Rt.k <= nk) {
+        PoR = 0.0; KoR = Rt.k;
+        const double q = Rt.pk();
+        posR = q + 0.0 * (Rt.pk1() - q);
+        Rt.advance(nk);
       } else {
         PoR = 1.0; KoR = nk;
-        posR = R.p0 + 1.0 * (R.p1 - R.p0);
+        posR = Rt.p0 + 1.0 * (Rt.p1 - Rt.p0);
         reached_bottom = true; search_l = false;
       }
     } else {
       const ColRec cl_ = L.cur();
-      const double top = sec_drho(R.w0, cl_);
-      const double bot = sec_drho(R.w1, cl_);
+      const double top = sec_drho(Rt.w0, cl_);
+      const double bot = sec_drho(Rt.w1, cl_);
       double Po;
-      if (top >= 0.0 || R.k + L.k == 2)
+      if (top >= 0.0 || Rt.k + L.k == 2)
         Po = 0.0;
       else if (top >= bot)
         Po = 1.0;
       else
-        Po = interpolate_position<double>(top, R.p0, bot, R.p1);
-      int K = R.m;
-      double pa = R.p0, pb = R.p1;
-      if (Po >= 1.0 && K < nk) {
-        K += 1;
+        Po = interpolate_position<double>(top, Rt.p0, bot, Rt.p1);
+      int Kn = Rt.m;
+      double pa = Rt.p0, pb = Rt.p1;
+      if (Po >= 1.0 && Kn < nk) {
+        Kn += 1;
         Po = Po - 1.0;
-        pa = R.p1;
-        pb = R.p2;
+        pa = Rt.p1;
+        pb = Rt.p2;
       }
-      if ((double)(K - lastK_r) + (Po - lastP_r) < 0.0) {
+      if ((double)(Kn - lastK_r) + (Po - lastP_r) < 0.0) {
         PoR = lastP_r; KoR = lastK_r; posR = lastPos_r;
       } else {
-        PoR = Po; KoR = K; posR = pa + Po * (pb - pa);
+        PoR = Po; KoR = Kn; posR = pa + Po * (pb - pa);
       }
       if (L.k <= nk) {
         PoL = 0.0; KoL = L.k;
@@ -285,20 +379,20 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
       const double hR = posR - lastPos_r;
       double he = 0.0;
       if (hL + hR > 0.0) he = ((2.0 * hL) * hR) / (hL + hR);
-      pend.hEff = he * a.pa_to_h;
+      pend.hEff = Face::stored_heff(he * a.pa_to_h);
       if (pend.hEff != 0.0) {
         mono_l = mono_l && pend.KoL >= contK_l;
         mono_r = mono_r && pend.KoR >= contK_r;
         contK_l = pend.KoL;
         contK_r = pend.KoR;
       }
-      o[(size_t)(ks - 2) * fst] = pend;
+      o[(size_t)(ks - 2) * fst] = Face::put(pend);
     }
-    pend = FaceRec{PoL, PoR, 0.0, KoL, KoR};
+    pend = SurfVals{PoL, PoR, 0.0, KoL, KoR};
     lastPos_l = posL;
     lastPos_r = posR;
   }
-  o[(size_t)(ns - 1) * fst] = pend;
+  o[(size_t)(ns - 1) * fst] = Face::put(pend);
   a.mono[fo] = (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0));
 }
 
@@ -309,11 +403,12 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs a, TileMap tm) 
 //   per column (ca = NT*nz, cb = 1, ct = nz): each tracer's column is one
 //     run of records, which a face's lanes, walking down the column, read in
 //     contiguous stretches;
-//   layer-major (ca = NT, cb = N2*NT, ct = 1): the lanes of this kernel,
-//     adjacent columns, write contiguous rows.
+//   layer-major (ca = NT, cb = N2*NT, ct = 1): the lanes of the streaming
+//     kernel, adjacent columns, write contiguous rows.
 // ---------------------------------------------------------------------------
-struct TracerSet {
-  const double* q[4];  // thickness-weighted tracers of the group
+template <class R>
+struct TracerSetT {
+  const R* q[4];  // thickness-weighted tracers of the group
 };
 
 // (32-bit element offsets from uniform bases: the host checks that
@@ -322,54 +417,57 @@ struct ReconLayout {
   uint32_t ca, cb, ct;
 };
 
-template <int NT>
-__global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz, const double* __restrict__ h,
-                                                      TracerSet tr, ReconRec* __restrict__ rec,
+// The reconstruction is computed in FP64 in both forms, from the stored
+// fields.
+// Streaming form: one lane per column walks it top-down with a register
+// window (stream_edges).
+template <class F, int NT>
+__global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz,
+                                                      const typename F::R* __restrict__ h,
+                                                      TracerSetT<typename F::R> tr,
+                                                      ReconRec* __restrict__ rec,
                                                       ReconLayout y) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
-    hh = h[o];
+    hh = (double)h[o];
     const double he = hh > kHDivEps ? hh : kHDivEps;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) s[t] = tr.q[t][o] / he;
+    for (int t = 0; t < NT; ++t) s[t] = (double)tr.q[t][o] / he;
   };
   double e_prev[NT];
   ReconRec* rc = rec + (size_t)c * y.ca;
   auto emit = [&](int k, const double* e, const double* s_above) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (k > 0) {
-        const double tl = s_above[t];
-        double al = e_prev[t], ar = e[t];
-        ppm_limit<double>(tl, al, ar);
-        rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec{tl, al, ar, e_prev[t], e[t], 0.0};
-      }
+      if (k > 0) rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec{s_above[t], e_prev[t], e[t]};
       e_prev[t] = e[t];
     }
   };
   stream_edges<double, NT>(nz, load, emit);
 }
 
-// The same reconstruction, tiled: a workgroup takes kReconCols adjacent
-// columns x kReconLayers layers.  Layer values, PLM slopes and PPM edges are
-// staged in LDS stage by stage (every value from the same functions on the
-// same operands as stream_edges, including its clamped windows at the top
-// and bottom), so global reads run along columns and the per-column record
-// runs are written along layers.  Per-column layout only (cb = 1).
+// Tiled form: a workgroup takes kReconCols adjacent columns x kReconLayers
+// layers.  Layer values, PLM slopes and PPM edges are staged in LDS stage by
+// stage (every value from the same functions on the same operands as
+// stream_edges, including its clamped windows at the top and bottom), so
+// global reads run along columns and the per-column record runs are written
+// along layers.  Per-column layout only (cb = 1).
 constexpr int kReconCols = 16, kReconLayers = 16;
 
-template <int NT>
+template <class F, int NT>
 __global__ void __launch_bounds__(kReconCols * kReconLayers)
-    k_tracer_recon_tiled(int N2, int nz, const double* __restrict__ h, TracerSet tr,
-                         ReconRec* __restrict__ rec, ReconLayout y) {
+    k_tracer_recon_tiled(int N2, int nz, const typename F::R* __restrict__ h,
+                         TracerSetT<typename F::R> tr, ReconRec* __restrict__ rec,
+                         ReconLayout y) {
+  using R = double;
   constexpr int CT = kReconCols, KT = kReconLayers, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
-  __shared__ double sh[NL][CT];        // h, layers k0-2 .. k0+KT+1 (clamped)
-  __shared__ double sv[NT][NL][CT];    // layer values hTr / max(h, eps), same layers
-  __shared__ double sd[NT][NSL][CT];   // PLM slopes, layers k0-1 .. k0+KT
-  __shared__ double se[NT][NE][CT];    // PPM edges, interfaces k0 .. k0+KT
+  __shared__ R sh[NL][CT];        // h, layers k0-2 .. k0+KT+1 (clamped)
+  __shared__ R sv[NT][NL][CT];    // layer values hTr / max(h, eps), same layers
+  __shared__ R sd[NT][NSL][CT];   // PLM slopes, layers k0-1 .. k0+KT
+  __shared__ R se[NT][NE][CT];    // PPM edges, interfaces k0 .. k0+KT
   const int nkb = (nz + KT - 1) / KT;
   const int b = xcd_band((int)blockIdx.x, (int)gridDim.x);
   const int c0 = (b / nkb) * CT, k0 = (b % nkb) * KT;
@@ -382,11 +480,11 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
     kl = kl < 0 ? 0 : (kl > nz - 1 ? nz - 1 : kl);
     const int c = c0 + cc < N2 ? c0 + cc : N2 - 1;
     const size_t o = (size_t)(nz - 1 - kl) * n2 + c;  // top-down layer kl
-    const double hh = h[o];
-    const double he = hh > kHDivEps ? hh : kHDivEps;
+    const R hh = (R)h[o];
+    const R he = hh > R(kHDivEps) ? hh : R(kHDivEps);
     sh[r][cc] = hh;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) sv[t][r][cc] = tr.q[t][o] / he;
+    for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)tr.q[t][o] / he;
   }
   __syncthreads();
   // 2: slopes of layers kl = k0-1+r; 0 for the first and last layer (the
@@ -397,9 +495,9 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
     const bool in = kl >= 1 && kl <= nz - 2;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      sd[t][r][cc] = in ? plm_slope<double>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
-                                            sv[t][r + 1][cc], sv[t][r + 2][cc])
-                        : 0.0;
+      sd[t][r][cc] = in ? plm_slope<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
+                                       sv[t][r + 1][cc], sv[t][r + 2][cc])
+                        : R(0);
   }
   __syncthreads();
   // 3: edges at interfaces ki = k0+r
@@ -409,14 +507,14 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
                             // sd rows r, r+1: slopes of layers ki-1, ki
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      double e = 0.0;
+      R e = R(0);
       if (ki == 0) {
-        e = sv[t][r + 2][cc] - 0.5 * 0.0;  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
+        e = sv[t][r + 2][cc] - R(0.5) * R(0);  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
       } else if (ki == nz) {
-        e = sv[t][r + 1][cc] + 0.5 * 0.0;  // diff(nk) = 0
+        e = sv[t][r + 1][cc] + R(0.5) * R(0);  // diff(nk) = 0
       } else if (ki < nz) {
-        e = ppm_edge<double>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sh[r + 3][cc],
-                             sv[t][r + 1][cc], sv[t][r + 2][cc], sd[t][r][cc], sd[t][r + 1][cc]);
+        e = ppm_edge<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sh[r + 3][cc], sv[t][r + 1][cc],
+                        sv[t][r + 2][cc], sd[t][r][cc], sd[t][r + 1][cc]);
       }
       se[t][r][cc] = e;
     }
@@ -428,13 +526,8 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
   if (kl < nz && c < N2) {
     ReconRec* rc = rec + (size_t)c * y.ca + (size_t)kl * y.cb;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const double tl = sv[t][kk + 2][cc];
-      const double et = se[t][kk][cc], eb = se[t][kk + 1][cc];
-      double al = et, ar = eb;
-      ppm_limit<double>(tl, al, ar);
-      rc[(size_t)t * y.ct] = ReconRec{tl, al, ar, et, eb, 0.0};
-    }
+    for (int t = 0; t < NT; ++t)
+      rc[(size_t)t * y.ct] = ReconRec{sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]};
   }
 }
 
@@ -446,21 +539,21 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
 // Pieces of redi_sublayer_dT (:893-925).  The interface value at position
 // Po in layer K of a column, (1 - Po) * T(K-1) + Po * T(K), from layer K's
 // record (the tlt / tlb / trt / trb expressions, :905-912):
-__device__ __forceinline__ double edge_value(const ReconRec& r, double Po) {
+__device__ __forceinline__ double edge_value(const LayerVals& r, double Po) {
   return (1.0 - Po) * r.Tt + Po * r.Tb;
 }
 
 // The down-gradient guard (:913-924): the layer difference where the
 // top, bottom, mean and layer differences agree in sign, else 0.
-__device__ __forceinline__ double guarded_dT(double tlt, double tlb, double trt, double trb,
-                                             double tlay, double trlay) {
-  const double dT_top = trt - tlt;
-  const double dT_bot = trb - tlb;
-  const double dT_ave = 0.5 * (dT_top + dT_bot);
-  const double dT_layer = trlay - tlay;
-  if (signum1<double>(dT_top) * signum1<double>(dT_bot) <= 0.0 ||
-      signum1<double>(dT_ave) * signum1<double>(dT_layer) <= 0.0)
-    return 0.0;
+template <class R>
+__device__ __forceinline__ R guarded_dT(R tlt, R tlb, R trt, R trb, R tlay, R trlay) {
+  const R dT_top = trt - tlt;
+  const R dT_bot = trb - tlb;
+  const R dT_ave = R(0.5) * (dT_top + dT_bot);
+  const R dT_layer = trlay - tlay;
+  if (signum1<R>(dT_top) * signum1<R>(dT_bot) <= R(0) ||
+      signum1<R>(dT_ave) * signum1<R>(dT_layer) <= R(0))
+    return R(0);
   return dT_layer;
 }
 
@@ -477,6 +570,7 @@ constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
 constexpr int kFaceGroup = 4;
 constexpr int kChunk = 63;  // sublayers per pass of a wavefront (64 surfaces)
 
+template <class F>
 struct FaceFluxArgs {
   int nfa;                // faces per row of the face array
   int ilo, ihi, jlo, jhi; // faces computed: i in [ilo, ihi), j in [jlo, jhi) (0-based)
@@ -489,16 +583,16 @@ struct FaceFluxArgs {
   uint32_t nfp;           // faces per sublayer row of tgt / flx (nf rounded up to kFaceGroup)
   double dt;
   const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
-  const FaceRec* rec;
+  const FaceRecF<F>* rec;
   ReconLayout ry;
-  const ReconRec* recon;  // tracer t's records from recon + t*ry.ct
-  uint32_t* tgt;          // packed targets, (s, face) at s*nfp + face
-  double* flx;            // fluxes, (s, face, t) at (s*nfp + face)*NT + t
+  const ReconRec* recon;      // tracer t's records from recon + t*ry.ct
+  uint32_t* tgt;              // packed targets, (s, face) at s*nfp + face
+  typename F::R* flx;         // fluxes, (s, face, t) at (s*nfp + face)*NT + t
 };
 
 // flx = dT * hEff * Coef with Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
 // for every contributing sublayer, in the reference's rounding order
-// (:1060-1090).  The reference evaluates this twice per face, once from
+// (:1060-1090), computed in FP64 in both forms and stored in the form's type.  The reference evaluates this twice per face, once from
 // each bounding cell; both evaluations are the same expression on the same
 // operands, so one evaluation serves both cells.
 // Lane l of a face's wavefront takes neutral surface s = c0 + l: it loads
@@ -511,9 +605,10 @@ struct FaceFluxArgs {
 // each column.  The kFaceGroup waves of a workgroup read one line of each
 // surface-major face-record row between them; their fluxes are staged in
 // LDS and written back as whole rows (kFaceGroup faces x NT tracers).
-template <int NT>
-__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
-  __shared__ double sx[kFaceGroup * 64 * NT];
+template <class F, int NT>
+__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
+  using R = typename F::R;
+  __shared__ R sx[kFaceGroup * 64 * NT];  // fluxes, stored type
   __shared__ uint32_t stg[kFaceGroup * 64];
   const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const size_t f0 = (size_t)xcd_band((int)blockIdx.x, (int)gridDim.x) * kFaceGroup;
@@ -529,7 +624,7 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
     cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
     coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
   }
-  const FaceRec* r = a.rec + (live ? fo * a.fa : 0);
+  const FaceRecF<F>* r = a.rec + (live ? fo * a.fa : 0);
   const ReconRec* recL = a.recon + cl * a.ry.ca;
   const ReconRec* recR = a.recon + cr * a.ry.ca;
   const uint32_t cb = a.ry.cb, ct = a.ry.ct;
@@ -538,24 +633,26 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
   for (int c0 = 0; c0 < nsub; c0 += kChunk) {
     const int s = c0 + lane;
     const bool surf = live && s < ns;
-    FaceRec F{0.0, 0.0, 0.0, 1, 1};
-    if (surf) F = r[(size_t)s * fb];
+    SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
+    if (surf) Fs = r[(size_t)s * fb].get();
+    const int KoL = Fs.KoL, KoR = Fs.KoR;
+    const double PoL = Fs.PoL, PoR = Fs.PoR;
     double eL[NT], eR[NT];
-    ReconRec qL[NT], qR[NT];
+    LayerVals qL[NT], qR[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      qL[t] = ReconRec{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      qL[t] = LayerVals{0.0, 0.0, 0.0, 0.0, 0.0};
       qR[t] = qL[t];
       if (surf) {
-        qL[t] = recL[(uint32_t)(F.KoL - 1) * cb + (uint32_t)t * ct];
-        qR[t] = recR[(uint32_t)(F.KoR - 1) * cb + (uint32_t)t * ct];
+        qL[t] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
+        qR[t] = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
       }
-      eL[t] = edge_value(qL[t], F.PoL);
-      eR[t] = edge_value(qR[t], F.PoR);
+      eL[t] = edge_value(qL[t], PoL);
+      eR[t] = edge_value(qR[t], PoR);
     }
     // surface s+1, the bottom of sublayer s
-    const double PoLb = __shfl_down(F.PoL, 1, 64), PoRb = __shfl_down(F.PoR, 1, 64);
-    const int KoLb = __shfl_down(F.KoL, 1, 64), KoRb = __shfl_down(F.KoR, 1, 64);
+    const double PoLb = __shfl_down(PoL, 1, 64), PoRb = __shfl_down(PoR, 1, 64);
+    const int KoLb = __shfl_down(KoL, 1, 64), KoRb = __shfl_down(KoR, 1, 64);
     double eLb[NT], eRb[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -563,19 +660,18 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
       eRb[t] = __shfl_down(eR[t], 1, 64);
     }
     uint32_t packed = 0;
-    double x[NT];
+    R x[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) x[t] = 0.0;
-    if (surf && lane < kChunk && s < nsub && F.hEff != 0.0) {
-      packed = ((uint32_t)F.KoL << kTargetShiftL) | ((uint32_t)F.KoR << kTargetShiftR);
+    for (int t = 0; t < NT; ++t) x[t] = R(0);
+    if (surf && lane < kChunk && s < nsub && Fs.hEff != 0.0) {
+      packed = ((uint32_t)KoL << kTargetShiftL) | ((uint32_t)KoR << kTargetShiftR);
+      const double hEff = Fs.hEff;
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const double tlay = ppm_ave<double>(F.PoL, PoLb + (double)(KoLb - F.KoL), qL[t].aL,
-                                            qL[t].aR, qL[t].Tl);
-        const double trlay = ppm_ave<double>(F.PoR, PoRb + (double)(KoRb - F.KoR), qR[t].aL,
-                                             qR[t].aR, qR[t].Tl);
-        const double d = guarded_dT(eL[t], eLb[t], eR[t], eRb[t], tlay, trlay);
-        x[t] = (d * F.hEff) * coef;
+        const double tlay = ppm_ave<double>(PoL, PoLb + (double)(KoLb - KoL), qL[t].aL, qL[t].aR, qL[t].Tl);
+        const double trlay = ppm_ave<double>(PoR, PoRb + (double)(KoRb - KoR), qR[t].aL, qR[t].aR, qR[t].Tl);
+        const double d = guarded_dT<double>(eL[t], eLb[t], eR[t], eRb[t], tlay, trlay);
+        x[t] = (R)((d * hEff) * coef);
       }
     }
 #pragma unroll
@@ -598,30 +694,33 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs a) {
 // ---------------------------------------------------------------------------
 // K5: flux divergence per cell (redi_apply_flux_impl, :1094-1181)
 // ---------------------------------------------------------------------------
+template <class F>
 struct CellFluxArgs {
   int nx, ny, nz;
   int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
   int wall_w, wall_e, wall_s, wall_n;
   const double* areaT;
   const uint32_t *tgt_u, *tgt_v;
-  const double *flx_u, *flx_v;
+  const typename F::R *flx_u, *flx_v;
   uint32_t nu, nv;                 // faces per sublayer row of the u / v arrays
-  double* htr[kMaxNT];
+  typename F::R* htr[kMaxNT];
+  double* tend[kMaxNT];            // optional: the FP64 tendency dTr / areaT per cell
 };
 
 // Layers an accumulator column of k_cell_flux holds.
 constexpr int kCellKCap[4] = {32, 76, 128, 256};
 
 // One wavefront per workgroup; lane = (cell, tracer), 64 / NT consecutive
-// cells of one row.  dTr(1:nz) of each lane lives in LDS, one column per lane
-// (acc[K][lane]), so a lane can add into any layer: every open face of the
-// cell (W, E, S, N) is streamed sublayer by sublayer (s ascending) and each
-// contributing sublayer's flux is added into dTr(nz+1-Ko(s)), exactly the
-// order of the reference's loops, whatever the pattern of Ko.  The lanes of
-// a wavefront read one contiguous stretch of each surface-major sublayer
-// row.  Then hTr += dTr * (1 / areaT).
-template <int NT, int KCAP>
-__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
+// cells of one row.  dTr(1:nz) of each lane lives in LDS in FP64 (both
+// forms), one column per lane (acc[K][lane]), so a lane can add into any
+// layer: every open face of the cell (W, E, S, N) is streamed sublayer by
+// sublayer (s ascending) and each contributing sublayer's flux is added
+// into dTr(nz+1-Ko(s)), exactly the order of the reference's loops,
+// whatever the pattern of Ko.  The lanes of a wavefront read one contiguous
+// stretch of each surface-major sublayer row.  Then hTr += dTr * (1/areaT).
+template <class F, int NT, int KCAP>
+__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs<F> a) {
+  using R = typename F::R;
   __shared__ double acc[KCAP * 64];
   constexpr int kCells = 64 / NT;  // cells per wavefront
   constexpr int kBatch = 32;       // sublayer rows in flight per face
@@ -654,7 +753,7 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
     const int shift = left ? kTargetShiftL : kTargetShiftR;
     const size_t nf = f < 2 ? a.nu : a.nv;
     const uint32_t* tg = (f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
-    const double* fx = (f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
+    const R* fx = (f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
     // dTr(K) of the run of consecutive contributions to one layer K stays
     // in a register; it is read from / written back to LDS when K changes,
     // so every addition still lands in reference order
@@ -662,25 +761,27 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
     double cur = 0.0;
     for (int s0 = 0; s0 < nsub; s0 += kBatch) {
       uint32_t pk[kBatch];
-      double x[kBatch];
+      R x[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        const int s = s0 + u;
         // every (sublayer, face) slot of a computed face holds a flux (0 where
-        // nothing contributes), so both loads issue without waiting
-        pk[u] = s < nsub ? tg[(size_t)s * nf] : 0u;
-        x[u] = s < nsub ? fx[(size_t)s * nf * NT] : 0.0;
+        // nothing contributes), so both loads issue without waiting; rows past
+        // the last sublayer re-read the last one and are skipped below, which
+        // keeps the loads free of branches
+        const int s = s0 + u < nsub ? s0 + u : nsub - 1;
+        pk[u] = tg[(size_t)s * nf];
+        x[u] = fx[(size_t)s * nf * NT];
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        if (pk[u] == 0) continue;
+        if (s0 + u >= nsub || pk[u] == 0) continue;
         const int K = (int)((pk[u] >> shift) & 0xffffu);
         if (K != curK) {
           if (curK) acc[(curK - 1) * 64 + lane] = cur;
           cur = acc[(K - 1) * 64 + lane];
           curK = K;
         }
-        cur = left ? cur + x[u] : cur - x[u];
+        cur = left ? cur + (double)x[u] : cur - (double)x[u];
       }
     }
     if (curK) acc[(curK - 1) * 64 + lane] = cur;
@@ -688,13 +789,16 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs a) {
   const size_t n2 = (size_t)nx * ny;
   const size_t c = (size_t)ci + (size_t)nx * cj;
   const double iaij = 1.0 / a.areaT[c];
-  double* hp = a.htr[0];
+  R* hp = a.htr[0];
+  double* tp = a.tend[0];
 #pragma unroll
   for (int q = 1; q < NT; ++q)
-    if (t == q) hp = a.htr[q];
+    if (t == q) { hp = a.htr[q]; tp = a.tend[q]; }
   for (int K = 1; K <= nz; ++K) {
     const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
-    hp[o] = hp[o] + acc[(K - 1) * 64 + lane] * iaij;
+    const double d = acc[(K - 1) * 64 + lane] * iaij;
+    hp[o] = (R)((double)hp[o] + d);
+    if (tp) tp[o] = d;
   }
 }
 
@@ -707,15 +811,17 @@ struct CoeffArrays {  // the reference layout: [s][face] (Fortran (nfa,nfb,ns))
 };
 
 // Faces (i, j), i in [i0, i1), j in [j0, j1) of an nfa-wide face array.
-__global__ void k_export_coeffs(const FaceRec* __restrict__ rec, uint32_t fa, uint32_t fb, int nfa,
-                                int nfb, int ns, int i0, int i1, int j0, int j1, CoeffArrays out) {
+template <class F>
+__global__ void k_export_coeffs(const FaceRecF<F>* __restrict__ rec, uint32_t fa, uint32_t fb,
+                                int nfa, int nfb, int ns, int i0, int i1, int j0, int j1,
+                                CoeffArrays out) {
   const size_t nf = (size_t)nfa * nfb;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nf * ns) return;
   const size_t f = t % nf, s = t / nf;  // writes coalesced along the face index
   const int i = (int)(f % nfa), j = (int)(f / nfa);
   if (i < i0 || i >= i1 || j < j0 || j >= j1) return;
-  const FaceRec r = rec[f * fa + s * fb];
+  const SurfVals r = rec[f * fa + s * fb].get();
   out.PoL[t] = r.PoL;
   out.PoR[t] = r.PoR;
   out.KoL[t] = r.KoL;
@@ -735,10 +841,11 @@ __global__ void k_face_khtr(int nu, int nv, const double* __restrict__ ext_u,
 }
 
 // ocean_redi_init fill (:493-502): Po = hEff = 0, Ko = 1
-__global__ void k_fill_faces(size_t nrec, FaceRec* __restrict__ rec, size_t nface,
+template <class F>
+__global__ void k_fill_faces(size_t nrec, FaceRecF<F>* __restrict__ rec, size_t nface,
                              uint8_t* __restrict__ mono) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nrec) rec[t] = FaceRec{0.0, 0.0, 0.0, 1, 1};
+  if (t < nrec) rec[t] = FaceRecF<F>::put(SurfVals{0.0, 0.0, 0.0, 1, 1});
   if (t < nface) mono[t] = kMonoL | kMonoR;
 }
 

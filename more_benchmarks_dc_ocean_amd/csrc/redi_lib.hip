@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "../../include/redi_hip.h"
-#include "redi_kernels_fp64.hpp"
+#include "redi_kernels.hpp"
 
 using namespace redi;
 
@@ -84,26 +84,38 @@ int recon_slots(int nt) { return std::min(std::max(nt, 2), kTracerGroup); }
 // rounded up to whole face-flux workgroups.
 size_t padded_faces(size_t nf) { return (nf + kFaceGroup - 1) / kFaceGroup * kFaceGroup; }
 
+// Bytes of the per-form records and values (the column state is FP64 in
+// both forms).
+struct FormSizes {
+  size_t face, recon, val;  // FaceRec, ReconRec, stored value
+};
+template <class F>
+constexpr FormSizes form_sizes() {
+  return FormSizes{sizeof(FaceRecF<F>), sizeof(ReconRec), sizeof(typename F::R)};
+}
+FormSizes sizes_of(int form) {
+  return form == REDI_FORM_FP32_STORE ? form_sizes<Fp32>() : form_sizes<Fp64>();
+}
+
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
-//   Phase A: ColRec + P per column interface (40 B).
-//   Phase B, per tracer slot: a ReconRec (48 B) per layer of every column
-//   and a flux (8 B) per sublayer of every (padded) u and v face; once: the
-//   packed targets (4 B) per sublayer and face, and a discard layer field
-//   (8 B per cell) that takes the second output of a two-tracer launch run
-//   for a lone tracer.
+//   Phase A: ColRec + P per column interface.
+//   Phase B, per tracer slot: a ReconRec per layer of every column and a
+//   flux per sublayer of every (padded) u and v face; once: the packed
+//   targets (4 B) per sublayer and face, and a discard layer field that
+//   takes the second output of a two-tracer launch run for a lone tracer.
 // Every array starts on a 256-byte boundary.
 struct PhaseBScratch {
-  ReconRec* recon;   // N2 * nz * slots records (ReconLayout)
-  double* discard;   // N2 * nz
-  double* flx_u;     // (ns - 1) * Nu' * slots, Nu' = padded u faces
-  double* flx_v;     // (ns - 1) * Nv' * slots
+  void* recon;       // N2 * nz * slots records (ReconLayout)
+  void* discard;     // N2 * nz values
+  void* flx_u;       // (ns - 1) * Nu' * slots values, Nu' = padded u faces
+  void* flx_v;       // (ns - 1) * Nv' * slots values
   uint32_t* tgt_u;   // (ns - 1) * Nu'
   uint32_t* tgt_v;   // (ns - 1) * Nv'
   size_t bytes;
 };
 
-PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots) {
+PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const FormSizes& z) {
   const size_t N2 = (size_t)nx * ny, nsub = 2 * (size_t)nz + 1;
   const size_t nu = padded_faces((size_t)(nx + 1) * ny), nv = padded_faces((size_t)nx * (ny + 1));
   char* b = (char*)base;
@@ -114,27 +126,28 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots) {
     return p;
   };
   PhaseBScratch s;
-  s.recon = (ReconRec*)take(sizeof(ReconRec) * N2 * nz * slots);
-  s.discard = (double*)take(8 * N2 * nz);
-  s.flx_u = (double*)take(8 * nsub * nu * slots);
-  s.flx_v = (double*)take(8 * nsub * nv * slots);
+  s.recon = take(z.recon * N2 * nz * slots);
+  s.discard = take(z.val * N2 * nz);
+  s.flx_u = take(z.val * nsub * nu * slots);
+  s.flx_v = take(z.val * nsub * nv * slots);
   s.tgt_u = (uint32_t*)take(4 * nsub * nu);
   s.tgt_v = (uint32_t*)take(4 * nsub * nv);
   s.bytes = off;
   return s;
 }
 
-size_t workspace_bytes(int nx, int ny, int nz, int nt) {
-  const size_t phase_a = (size_t)40 * (nz + 1) * nx * ny;
-  return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt)).bytes);
+size_t workspace_bytes(int nx, int ny, int nz, int nt, int form) {
+  const FormSizes z = sizes_of(form);
+  const size_t phase_a = (sizeof(ColRec) + sizeof(double)) * (nz + 1) * (size_t)nx * ny;
+  return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt), z).bytes);
 }
 
 // The per-face Phase-A state: FaceRec (face, s) at s*nup + face
 // (surface-major, rows padded) for u and v faces, plus the monotonicity
 // flags and the resolved face diffusivities.
 struct FaceState {
-  FaceRec* u = nullptr;
-  FaceRec* v = nullptr;
+  void* u = nullptr;
+  void* v = nullptr;
   uint8_t* mono_u = nullptr;
   uint8_t* mono_v = nullptr;
   double* khtr_u = nullptr;
@@ -143,20 +156,20 @@ struct FaceState {
   size_t nup = 0, nvp = 0;  // faces per surface row
 };
 
-size_t face_state_bytes(int nx, int ny, int nz) {
+size_t face_state_bytes(int nx, int ny, int nz, int form) {
   const size_t ns = 2 * (size_t)nz + 2, nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
-  return (padded_faces(nu) + padded_faces(nv)) * ns * sizeof(FaceRec) +
+  return (padded_faces(nu) + padded_faces(nv)) * ns * sizes_of(form).face +
          (nu + nv) * (1 + sizeof(double));
 }
 
-int alloc_face_state(FaceState* fs, int nx, int ny, int nz, bool own_khtr) {
-  const size_t ns = 2 * (size_t)nz + 2;
+int alloc_face_state(FaceState* fs, int nx, int ny, int nz, bool own_khtr, int form) {
+  const size_t ns = 2 * (size_t)nz + 2, rb = sizes_of(form).face;
   fs->nu = (size_t)(nx + 1) * ny;
   fs->nv = (size_t)nx * (ny + 1);
   fs->nup = padded_faces(fs->nu);
   fs->nvp = padded_faces(fs->nv);
-  HIP_TRY(hipMalloc(&fs->u, fs->nup * ns * sizeof(FaceRec)));
-  HIP_TRY(hipMalloc(&fs->v, fs->nvp * ns * sizeof(FaceRec)));
+  HIP_TRY(hipMalloc(&fs->u, fs->nup * ns * rb));
+  HIP_TRY(hipMalloc(&fs->v, fs->nvp * ns * rb));
   HIP_TRY(hipMalloc(&fs->mono_u, fs->nu));
   HIP_TRY(hipMalloc(&fs->mono_v, fs->nv));
   if (own_khtr) {
@@ -173,12 +186,20 @@ void free_face_state(FaceState* fs, bool own_khtr) {
   *fs = FaceState{};
 }
 
+template <class F>
 int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
   const size_t ns = 2 * (size_t)nz + 2;
-  k_fill_faces<<<blocks_for(fs.nup * ns, 256), 256, 0, st>>>(fs.nup * ns, fs.u, fs.nu, fs.mono_u);
-  k_fill_faces<<<blocks_for(fs.nvp * ns, 256), 256, 0, st>>>(fs.nvp * ns, fs.v, fs.nv, fs.mono_v);
+  k_fill_faces<F><<<blocks_for(fs.nup * ns, 256), 256, 0, st>>>(fs.nup * ns, (FaceRecF<F>*)fs.u,
+                                                                 fs.nu, fs.mono_u);
+  k_fill_faces<F><<<blocks_for(fs.nvp * ns, 256), 256, 0, st>>>(fs.nvp * ns, (FaceRecF<F>*)fs.v,
+                                                                 fs.nv, fs.mono_v);
   HIP_TRY(hipGetLastError());
   return REDI_OK;
+}
+
+int fill_face_state_form(const FaceState& fs, int nz, int form, hipStream_t st) {
+  return form == REDI_FORM_FP32_STORE ? fill_face_state<Fp32>(fs, nz, st)
+                                      : fill_face_state<Fp64>(fs, nz, st);
 }
 
 struct Tiles {
@@ -196,33 +217,35 @@ TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
 }
 
 // Phase A over both face grids (redi_calc_coeffs_x / _y).
-int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const double* h, const double* t,
-                   const double* s, const double* wet_u, const double* wet_v, const FaceState& fs,
-                   void* ws, const Tiles& tiles, hipStream_t st, Timer& tm) {
+template <class F>
+int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename F::R* h,
+                   const typename F::R* t, const typename F::R* s, const double* wet_u,
+                   const double* wet_v, const FaceState& fs, void* ws, const Tiles& tiles,
+                   hipStream_t st, Timer& tm) {
   const int N2 = nx * ny;
   ColRec* rec = (ColRec*)ws;
   double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
-  k_column_state<<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, t, s, eos, rec, P);
+  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, t, s, eos, rec, P);
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
-    SweepArgs a{nx, nz, nx + 1, 0, 0, 1, 0, 1, 0, wet_u, rec, P, fs.u, fs.mono_u, pa_to_h};
-    a.fa = 1; a.fb = (uint32_t)fs.nup;
+    SweepArgs<F> a{nx, nz, nx + 1, 1, (uint32_t)fs.nup, 1, 0, 1, 0, wet_u, rec, P,
+                   (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h};
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
-    k_neutral_sweep<<<nb, bs, 0, st>>>(a, m);
+    k_neutral_sweep<F><<<nb, bs, 0, st>>>(a, m);
     tm.end(slot, st);
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
-    SweepArgs a{nx, nz, nx, 0, 0, 0, 1, 0, 1, wet_v, rec, P, fs.v, fs.mono_v, pa_to_h};
-    a.fa = 1; a.fb = (uint32_t)fs.nvp;
+    SweepArgs<F> a{nx, nz, nx, 1, (uint32_t)fs.nvp, 0, 1, 0, 1, wet_v, rec, P,
+                   (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h};
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
-    k_neutral_sweep<<<nb, bs, 0, st>>>(a, m);
+    k_neutral_sweep<F><<<nb, bs, 0, st>>>(a, m);
     tm.end(slot, st);
   }
   HIP_TRY(hipGetLastError());
@@ -235,11 +258,59 @@ struct PhaseB {
   int ww, we, ws, wn;
   const double *khtr_u_ext, *khtr_v_ext;
   double khtr;
-  const double *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT, *h;
+  const double *dy_cu, *dx_cv, *idxCu, *idyCv, *areaT;
 };
 
-int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceState& fs, void* ws,
-                   const Tiles& tiles, hipStream_t st, Timer& tm) {
+template <class F, int NT>
+void launch_recon(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
+                  const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
+                  hipStream_t st) {
+  const int N2 = p.nx * p.ny, nz = p.nz;
+  if (tiles.recon_layer_major) {
+    k_tracer_recon<F, NT><<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, ts, recon, ry);
+  } else {
+    const unsigned nbr = blocks_for(N2, kReconCols) * (unsigned)((nz + kReconLayers - 1) / kReconLayers);
+    k_tracer_recon_tiled<F, NT><<<nbr, kReconCols * kReconLayers, 0, st>>>(N2, nz, h, ts, recon, ry);
+  }
+}
+
+template <class F, int NT>
+void launch_cell(int kcap, unsigned nbc, const CellFluxArgs<F>& ca, hipStream_t st) {
+  switch (kcap) {
+    case 0: k_cell_flux<F, NT, kCellKCap[0]><<<nbc, 64, 0, st>>>(ca); break;
+    case 1: k_cell_flux<F, NT, kCellKCap[1]><<<nbc, 64, 0, st>>>(ca); break;
+    case 2: k_cell_flux<F, NT, kCellKCap[2]><<<nbc, 64, 0, st>>>(ca); break;
+    default: k_cell_flux<F, NT, kCellKCap[3]><<<nbc, 64, 0, st>>>(ca); break;
+  }
+}
+
+// One tracer group of Phase B: reconstruction, the face fluxes of both
+// grids, the divergence.
+template <class F, int NT>
+void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
+                  const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
+                  FaceFluxArgs<F>& fu, FaceFluxArgs<F>& fv, unsigned nbu, unsigned nbv,
+                  const CellFluxArgs<F>& ca, int kcap, unsigned nbc, hipStream_t st, Timer& tm) {
+  int slot = tm.begin(REDI_K_TRACER_RECON, st);
+  launch_recon<F, NT>(p, tiles, h, ts, recon, ry, st);
+  tm.end(slot, st);
+  for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
+    const unsigned nb = grid ? nbv : nbu;
+    if (!nb) continue;
+    slot = tm.begin(REDI_K_FACE_FLUX, st);
+    k_face_flux<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
+    tm.end(slot, st);
+  }
+  slot = tm.begin(REDI_K_CELL_FLUX, st);
+  launch_cell<F, NT>(kcap, nbc, ca, st);
+  tm.end(slot, st);
+}
+
+template <class F>
+int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::R* const* tracers,
+                   const FaceState& fs, void* ws, const Tiles& tiles, hipStream_t st, Timer& tm,
+                   double* const* tend = nullptr) {
+  using R = typename F::R;
   const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny, ns = 2 * nz + 2;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   int slot = tm.begin(REDI_K_FACE_KHTR, st);
@@ -247,32 +318,36 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
       nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v);
   tm.end(slot, st);
   const int slots = recon_slots(nt);
-  const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots);
+  const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
+  ReconRec* recon = (ReconRec*)sc.recon;
 
-  FaceFluxArgs fu{};  // u faces i = 2..nx (1-based), every row
+  FaceFluxArgs<F> fu{};  // u faces i = 2..nx (1-based), every row
   fu.nfa = nx + 1; fu.ilo = 1; fu.ihi = nx; fu.jlo = 0; fu.jhi = ny;
   fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
   fu.wall0 = p.ww ? p.nghost : -1;
   fu.wall1 = p.we ? p.nghost + p.nxp : -1;
   fu.nf = (uint32_t)nu; fu.fa = 1; fu.fb = (uint32_t)fs.nup; fu.nfp = (uint32_t)fs.nup;
-  fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu; fu.rec = fs.u;
-  fu.tgt = sc.tgt_u; fu.flx = sc.flx_u;
-  FaceFluxArgs fv{};  // v faces j = 2..ny (1-based), every column
+  fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu;
+  fu.rec = (const FaceRecF<F>*)fs.u; fu.recon = recon;
+  fu.tgt = sc.tgt_u; fu.flx = (R*)sc.flx_u;
+  FaceFluxArgs<F> fv{};  // v faces j = 2..ny (1-based), every column
   fv.nfa = nx; fv.ilo = 0; fv.ihi = nx; fv.jlo = 1; fv.jhi = ny;
   fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
   fv.wall0 = p.ws ? p.nghost : -1;
   fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
   fv.nf = (uint32_t)nv; fv.fa = 1; fv.fb = (uint32_t)fs.nvp; fv.nfp = (uint32_t)fs.nvp;
-  fv.dt = p.dt; fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv; fv.rec = fs.v;
-  fv.tgt = sc.tgt_v; fv.flx = sc.flx_v;
+  fv.dt = p.dt; fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv;
+  fv.rec = (const FaceRecF<F>*)fs.v; fv.recon = recon;
+  fv.tgt = sc.tgt_v; fv.flx = (R*)sc.flx_v;
 
-  CellFluxArgs ca{};
+  CellFluxArgs<F> ca{};
   ca.nx = nx; ca.ny = ny; ca.nz = nz;
   ca.wuf_w = p.nghost + 1; ca.wuf_e = p.nghost + p.nxp + 1;
   ca.wvf_s = p.nghost + 1; ca.wvf_n = p.nghost + p.nyp + 1;
   ca.wall_w = p.ww; ca.wall_e = p.we; ca.wall_s = p.ws; ca.wall_n = p.wn;
   ca.areaT = p.areaT;
-  ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v; ca.flx_u = sc.flx_u; ca.flx_v = sc.flx_v;
+  ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v;
+  ca.flx_u = (const R*)sc.flx_u; ca.flx_v = (const R*)sc.flx_v;
   ca.nu = (uint32_t)fs.nup; ca.nv = (uint32_t)fs.nvp;
 
   int kcap = 0;  // smallest accumulator depth that holds nz layers
@@ -288,65 +363,28 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
     // above while its source is exact in a thread-by-thread CPU build and
     // at -O0, so none is built.)
     const int kn = gn == 1 ? 2 : gn;
-    TracerSet ts{};
+    TracerSetT<R> ts{};
     for (int t = 0; t < kMaxNT; ++t) {
       const int tt = std::min(t, gn - 1);
       ts.q[t] = tracers[g0 + tt];
       ca.htr[t] = tracers[g0 + tt];
+      ca.tend[t] = tend ? tend[g0 + tt] : nullptr;
     }
-    fu.recon = fv.recon = sc.recon;
-    if (gn == 1) ca.htr[1] = sc.discard;
+    if (gn == 1) {
+      ca.htr[1] = (R*)sc.discard;
+      ca.tend[1] = nullptr;
+    }
     const uint32_t k32 = (uint32_t)kn;
     const ReconLayout ry = tiles.recon_layer_major
                                ? ReconLayout{k32, (uint32_t)N2 * k32, 1}
                                : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
     fu.ry = fv.ry = ry;
-    slot = tm.begin(REDI_K_TRACER_RECON, st);
-    if (tiles.recon_layer_major) {
-      const unsigned nbr = blocks_for(N2, 128);
-      switch (kn) {
-        case 2: k_tracer_recon<2><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-        case 3: k_tracer_recon<3><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-        default: k_tracer_recon<4><<<nbr, 128, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-      }
-    } else {
-      const unsigned nbr = blocks_for(N2, kReconCols) * (unsigned)((nz + kReconLayers - 1) / kReconLayers);
-      const unsigned bsr = kReconCols * kReconLayers;
-      switch (kn) {
-        case 2: k_tracer_recon_tiled<2><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-        case 3: k_tracer_recon_tiled<3><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-        default: k_tracer_recon_tiled<4><<<nbr, bsr, 0, st>>>(N2, nz, p.h, ts, sc.recon, ry); break;
-      }
-    }
-    tm.end(slot, st);
-    for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
-      const unsigned nb = grid ? nbv : nbu;
-      if (!nb) continue;
-      const FaceFluxArgs& fa = grid ? fv : fu;
-      slot = tm.begin(REDI_K_FACE_FLUX, st);
-      switch (kn) {
-        case 2: k_face_flux<2><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
-        case 3: k_face_flux<3><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
-        default: k_face_flux<4><<<nb, 64 * kFaceGroup, 0, st>>>(fa); break;
-      }
-      tm.end(slot, st);
-    }
-    slot = tm.begin(REDI_K_CELL_FLUX, st);
     const unsigned nbc = (unsigned)ny * (unsigned)((nx + 64 / kn - 1) / (64 / kn));
-#define REDI_CELL_FLUX(NT)                                                           \
-  switch (kcap) {                                                                    \
-    case 0: k_cell_flux<NT, kCellKCap[0]><<<nbc, 64, 0, st>>>(ca); break;            \
-    case 1: k_cell_flux<NT, kCellKCap[1]><<<nbc, 64, 0, st>>>(ca); break;            \
-    case 2: k_cell_flux<NT, kCellKCap[2]><<<nbc, 64, 0, st>>>(ca); break;            \
-    default: k_cell_flux<NT, kCellKCap[3]><<<nbc, 64, 0, st>>>(ca); break;           \
-  }
     switch (kn) {
-      case 2: REDI_CELL_FLUX(2) break;
-      case 3: REDI_CELL_FLUX(3) break;
-      default: REDI_CELL_FLUX(4) break;
+      case 2: launch_group<F, 2>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
+      case 3: launch_group<F, 3>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
+      default: launch_group<F, 4>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
     }
-#undef REDI_CELL_FLUX
-    tm.end(slot, st);
   }
   HIP_TRY(hipGetLastError());
   return REDI_OK;
@@ -355,19 +393,22 @@ int launch_phase_b(const PhaseB& p, int nt, double* const* tracers, const FaceSt
 // Write the reference-layout coefficient arrays (and khtr) into `d`.
 // all_faces: every face (the reference's init values where Phase A never
 // writes); otherwise only the faces redi_calc_coeffs_x/_y compute.
+template <class F>
 int export_coeffs(const FaceState& fs, int nx, int ny, int nz, const redi_coeffs_t& d,
                   bool all_faces, bool with_khtr, hipStream_t st) {
   const int ns = 2 * nz + 2;
   const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
   const uint32_t ua = 1, ub = (uint32_t)fs.nup, va = 1, vb = (uint32_t)fs.nvp;
+  const FaceRecF<F>* ru = (const FaceRecF<F>*)fs.u;
+  const FaceRecF<F>* rv = (const FaceRecF<F>*)fs.v;
   CoeffArrays cu{d.uPoL, d.uPoR, d.uhEff, d.uKoL, d.uKoR};
   CoeffArrays cv{d.vPoL, d.vPoR, d.vhEff, d.vKoL, d.vKoR};
   if (all_faces) {
-    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, ua, ub, nx + 1, ny, ns, 0, nx + 1, 0, ny, cu);
-    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, va, vb, nx, ny + 1, ns, 0, nx, 0, ny + 1, cv);
+    k_export_coeffs<F><<<blocks_for(nu * ns, 256), 256, 0, st>>>(ru, ua, ub, nx + 1, ny, ns, 0, nx + 1, 0, ny, cu);
+    k_export_coeffs<F><<<blocks_for(nv * ns, 256), 256, 0, st>>>(rv, va, vb, nx, ny + 1, ns, 0, nx, 0, ny + 1, cv);
   } else {
-    k_export_coeffs<<<blocks_for(nu * ns, 256), 256, 0, st>>>(fs.u, ua, ub, nx + 1, ny, ns, 1, nx, 0, ny, cu);
-    k_export_coeffs<<<blocks_for(nv * ns, 256), 256, 0, st>>>(fs.v, va, vb, nx, ny + 1, ns, 0, nx, 1, ny, cv);
+    k_export_coeffs<F><<<blocks_for(nu * ns, 256), 256, 0, st>>>(ru, ua, ub, nx + 1, ny, ns, 1, nx, 0, ny, cu);
+    k_export_coeffs<F><<<blocks_for(nv * ns, 256), 256, 0, st>>>(rv, va, vb, nx, ny + 1, ns, 0, nx, 1, ny, cv);
   }
   if (with_khtr) {
     if (d.khtr_u) HIP_TRY(hipMemcpyAsync(d.khtr_u, fs.khtr_u, nu * 8, hipMemcpyDeviceToDevice, st));
@@ -386,7 +427,7 @@ bool valid_eos(int v) {
 }
 
 // Per-device grow-only state for the flat entries (which, like the
-// reference launchers, receive no state object).
+// reference launchers, receive no state object).  FP64 form.
 struct FlatSlot {
   int nx = 0, ny = 0, nz = 0;
   FaceState fs;
@@ -405,14 +446,14 @@ int flat_state(int nx, int ny, int nz, int nt, FlatSlot** out) {
   if (s.nx != nx || s.ny != ny || s.nz != nz) {
     if (s.fs.u) free_face_state(&s.fs, false);
     s.nx = s.ny = s.nz = 0;
-    int rc = alloc_face_state(&s.fs, nx, ny, nz, false);
+    int rc = alloc_face_state(&s.fs, nx, ny, nz, false, REDI_FORM_FP64);
     if (rc) return rc;
-    rc = fill_face_state(s.fs, nz, nullptr);
+    rc = fill_face_state<Fp64>(s.fs, nz, nullptr);
     if (rc) return rc;
     HIP_TRY(hipDeviceSynchronize());
     s.nx = nx; s.ny = ny; s.nz = nz;
   }
-  const size_t wb = workspace_bytes(nx, ny, nz, nt);
+  const size_t wb = workspace_bytes(nx, ny, nz, nt, REDI_FORM_FP64);
   if (s.ws_bytes < wb) {
     if (s.ws) HIP_TRY(hipFree(s.ws));
     s.ws = nullptr; s.ws_bytes = 0;
@@ -427,6 +468,7 @@ int flat_state(int nx, int ny, int nz, int nt, FlatSlot** out) {
 
 struct redi_s {
   int device = 0;
+  int form = REDI_FORM_FP64;
   int nx = 0, ny = 0, nz = 0, ns = 0;
   int enable = 0, continuous = 1;
   double khtr = 0.0, ref_pres = -1.0;
@@ -435,28 +477,123 @@ struct redi_s {
   size_t ws_bytes = 0;
   Tiles tiles;
   Timer timer;
+  std::vector<double*> tend;  // optional tendency outputs, one per tracer
   bool is_init = false;
 };
+
+namespace {
+
+// The model state of either form, reduced to what the launches need.
+template <class R>
+struct StateView {
+  int nz_ml, idx_t, idx_s, nt;
+  const R* h;
+  R* const* tracers;
+};
+
+template <class F>
+int calc_coeffs_impl(const char* who, const redi_hgrid_t* grid, const redi_metrics_t* m,
+                     const redi_eos_t* eos, redi_t* r, const StateView<typename F::R>& ms,
+                     void* stream) {
+  // silent guards (:585-593)
+  if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
+  if (ms.idx_t <= 0 || ms.idx_s <= 0) return REDI_SKIPPED;
+  if (!ms.h) return REDI_SKIPPED;
+  if (r->nz != ms.nz_ml) return REDI_SKIPPED;
+  if (grid->nx_total != r->nx || grid->ny_total != r->ny)
+    return fail(REDI_ERR_ARG, "%s: grid %dx%d does not match the state %dx%d", who,
+                grid->nx_total, grid->ny_total, r->nx, r->ny);
+  if (ms.idx_t > ms.nt || ms.idx_s > ms.nt || !ms.tracers)
+    return fail(REDI_ERR_ARG, "%s: tracer index out of range", who);
+  if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "%s: unknown EOS variant %d", who, eos->variant);
+  if (!m->wet_u || !m->wet_v) return fail(REDI_ERR_ARG, "%s: null wet mask", who);
+  return launch_phase_a<F>(r->nx, r->ny, r->nz, to_eos(*eos), ms.h, ms.tracers[ms.idx_t - 1],
+                           ms.tracers[ms.idx_s - 1], m->wet_u, m->wet_v, r->fs, r->ws, r->tiles,
+                           (hipStream_t)stream, r->timer);
+}
+
+template <class F>
+int apply_flux_impl(const char* who, const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
+                    const StateView<typename F::R>& ms, double dt, const double* kue,
+                    const double* kve, const redi_bc_t* bc, void* stream) {
+  // silent guards (:951-959, :978)
+  if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
+  if (!ms.h || !ms.tracers || ms.nt < 1) return REDI_SKIPPED;
+  if (r->nz != ms.nz_ml) return REDI_SKIPPED;
+  const bool use_ext = kue && kve;
+  if (!use_ext && r->khtr <= 0.0) return REDI_SKIPPED;
+  if (grid->nx_total != r->nx || grid->ny_total != r->ny)
+    return fail(REDI_ERR_ARG, "%s: grid %dx%d does not match the state %dx%d", who,
+                grid->nx_total, grid->ny_total, r->nx, r->ny);
+  if (!m->dy_cu || !m->dx_cv || !m->idxCu || !m->idyCv || !m->areaT)
+    return fail(REDI_ERR_ARG, "%s: null metric", who);
+  for (int t = 0; t < ms.nt; ++t)
+    if (!ms.tracers[t]) return fail(REDI_ERR_ARG, "%s: tracer %d is null", who, t + 1);
+  PhaseB p;
+  p.nx = r->nx; p.ny = r->ny; p.nz = r->nz;
+  p.nghost = grid->nghost; p.nxp = grid->nx_phys; p.nyp = grid->ny_phys;
+  p.dt = dt;
+  p.ww = bc ? (bc->west == REDI_OBC_WALL) : 1;
+  p.we = bc ? (bc->east == REDI_OBC_WALL) : 1;
+  p.ws = bc ? (bc->south == REDI_OBC_WALL) : 1;
+  p.wn = bc ? (bc->north == REDI_OBC_WALL) : 1;
+  p.khtr_u_ext = use_ext ? kue : nullptr;
+  p.khtr_v_ext = use_ext ? kve : nullptr;
+  p.khtr = r->khtr;
+  p.dy_cu = m->dy_cu; p.dx_cv = m->dx_cv; p.idxCu = m->idxCu; p.idyCv = m->idyCv;
+  p.areaT = m->areaT;
+  if (!r->tend.empty() && (int)r->tend.size() != ms.nt)
+    return fail(REDI_ERR_ARG, "%s: %zu tendency outputs for %d tracers", who, r->tend.size(), ms.nt);
+  return launch_phase_b<F>(p, ms.h, ms.nt, ms.tracers, r->fs, r->ws, r->tiles,
+                           (hipStream_t)stream, r->timer, r->tend.empty() ? nullptr : r->tend.data());
+}
+
+StateView<double> view64(const redi_state_t& s) {
+  return StateView<double>{s.nz_ml, s.idx_temperature, s.idx_salinity, s.ntracers, s.h_layer,
+                           s.tracers};
+}
+StateView<float> view32(const redi_state32_t& s) {
+  return StateView<float>{s.nz_ml, s.idx_temperature, s.idx_salinity, s.ntracers, s.h_layer,
+                          s.tracers};
+}
+
+int wrong_form(const char* who, const redi_t* r, int want) {
+  return fail(REDI_ERR_ARG, "%s: the state was created for form %d, not %d", who, r->form, want);
+}
+
+}  // namespace
 
 extern "C" {
 
 const char* redi_last_error(void) { return g_err.c_str(); }
 
-int redi_version(void) { return 200; }
+int redi_version(void) { return 300; }
 
 long long redi_workspace_bytes(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return 0;
-  return (long long)workspace_bytes(nx, ny, nz, kTracerGroup);
+  return (long long)workspace_bytes(nx, ny, nz, kTracerGroup, REDI_FORM_FP64);
 }
 
 long long redi_state_bytes(int nx, int ny, int nz) {
+  return redi_state_bytes_form(nx, ny, nz, REDI_FORM_FP64);
+}
+
+long long redi_state_bytes_form(int nx, int ny, int nz, int form) {
   if (nx < 1 || ny < 1 || nz < 1) return 0;
-  return (long long)(face_state_bytes(nx, ny, nz) + workspace_bytes(nx, ny, nz, kTracerGroup));
+  if (form != REDI_FORM_FP64 && form != REDI_FORM_FP32_STORE) return 0;
+  return (long long)(face_state_bytes(nx, ny, nz, form) +
+                     workspace_bytes(nx, ny, nz, kTracerGroup, form));
 }
 
 int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
+  return redi_init_form(out, grid, nz_ml, REDI_FORM_FP64);
+}
+
+int redi_init_form(redi_t** out, const redi_hgrid_t* grid, int nz_ml, int form) {
   if (!out || !grid) return fail(REDI_ERR_ARG, "redi_init: null argument");
   *out = nullptr;
+  if (form != REDI_FORM_FP64 && form != REDI_FORM_FP32_STORE)
+    return fail(REDI_ERR_ARG, "redi_init: unknown form %d", form);
   const int nx = grid->nx_total, ny = grid->ny_total;
   if (nx < 1 || ny < 1) return fail(REDI_ERR_ARG, "redi_init: bad grid %d x %d", nx, ny);
   const int nz = nz_ml < 1 ? 1 : nz_ml;
@@ -467,6 +604,7 @@ int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
     return fail(REDI_ERR_ARG, "redi_init: %d x %d x %d exceeds 2^29 cell interfaces per domain",
                 nx, ny, nz);
   redi_t* r = new redi_t();
+  r->form = form;
   auto cleanup = [&](int code) {
     free_face_state(&r->fs, true);
     (void)hipFree(r->ws);
@@ -475,12 +613,12 @@ int redi_init(redi_t** out, const redi_hgrid_t* grid, int nz_ml) {
   };
   if (hipGetDevice(&r->device) != hipSuccess) return cleanup(fail(REDI_ERR_HIP, "redi_init: no device"));
   r->nx = nx; r->ny = ny; r->nz = nz; r->ns = 2 * nz + 2;
-  int rc = alloc_face_state(&r->fs, nx, ny, nz, true);
+  int rc = alloc_face_state(&r->fs, nx, ny, nz, true, form);
   if (rc) return cleanup(rc);
-  r->ws_bytes = workspace_bytes(nx, ny, nz, kTracerGroup);
+  r->ws_bytes = workspace_bytes(nx, ny, nz, kTracerGroup, form);
   if (hipMalloc(&r->ws, r->ws_bytes) != hipSuccess)
     return cleanup(fail(REDI_ERR_HIP, "redi_init: workspace hipMalloc(%zu B) failed", r->ws_bytes));
-  rc = fill_face_state(r->fs, nz, nullptr);
+  rc = fill_face_state_form(r->fs, nz, form, nullptr);
   if (rc) return cleanup(rc);
   if (hipMemset(r->fs.khtr_u, 0, r->fs.nu * 8) != hipSuccess ||
       hipMemset(r->fs.khtr_v, 0, r->fs.nv * 8) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
@@ -527,6 +665,14 @@ int redi_set_tracer_group(redi_t* r, int n) {
   return REDI_OK;
 }
 
+int redi_set_tendency_output(redi_t* r, int ntracers, double* const* tend) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_tendency_output: null handle");
+  if (ntracers < 0 || (ntracers > 0 && !tend))
+    return fail(REDI_ERR_ARG, "redi_set_tendency_output: bad arguments");
+  r->tend.assign(tend, tend + ntracers);
+  return REDI_OK;
+}
+
 int redi_set_recon_order(redi_t* r, int layer_major) {
   if (!r) return fail(REDI_ERR_ARG, "redi_set_recon_order: null handle");
   r->tiles.recon_layer_major = layer_major ? 1 : 0;
@@ -550,61 +696,43 @@ int redi_export_coeffs(const redi_t* r, const redi_coeffs_t* dst, void* stream) 
   if (!dst->uPoL || !dst->uPoR || !dst->uKoL || !dst->uKoR || !dst->uhEff || !dst->vPoL ||
       !dst->vPoR || !dst->vKoL || !dst->vKoR || !dst->vhEff)
     return fail(REDI_ERR_ARG, "redi_export_coeffs: incomplete destination arrays");
-  return export_coeffs(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
+  if (r->form == REDI_FORM_FP32_STORE)
+    return export_coeffs<Fp32>(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
+  return export_coeffs<Fp64>(r->fs, r->nx, r->ny, r->nz, *dst, true, true, (hipStream_t)stream);
 }
 
 int redi_calc_coeffs(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_t* eos,
                      redi_t* r, const redi_state_t* ms, void* stream) {
   if (!grid || !m || !eos || !ms) return fail(REDI_ERR_ARG, "redi_calc_coeffs: null argument");
-  // silent guards (:585-593)
-  if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
-  if (ms->idx_temperature <= 0 || ms->idx_salinity <= 0) return REDI_SKIPPED;
-  if (!ms->h_layer) return REDI_SKIPPED;
-  if (r->nz != ms->nz_ml) return REDI_SKIPPED;
-  if (grid->nx_total != r->nx || grid->ny_total != r->ny)
-    return fail(REDI_ERR_ARG, "redi_calc_coeffs: grid %dx%d does not match the state %dx%d",
-                grid->nx_total, grid->ny_total, r->nx, r->ny);
-  if (ms->idx_temperature > ms->ntracers || ms->idx_salinity > ms->ntracers || !ms->tracers)
-    return fail(REDI_ERR_ARG, "redi_calc_coeffs: tracer index out of range");
-  if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "redi_calc_coeffs: unknown EOS variant %d", eos->variant);
-  if (!m->wet_u || !m->wet_v) return fail(REDI_ERR_ARG, "redi_calc_coeffs: null wet mask");
-  return launch_phase_a(r->nx, r->ny, r->nz, to_eos(*eos), ms->h_layer,
-                        ms->tracers[ms->idx_temperature - 1], ms->tracers[ms->idx_salinity - 1],
-                        m->wet_u, m->wet_v, r->fs, r->ws, r->tiles, (hipStream_t)stream, r->timer);
+  if (r && r->form != REDI_FORM_FP64) return wrong_form("redi_calc_coeffs", r, REDI_FORM_FP64);
+  return calc_coeffs_impl<Fp64>("redi_calc_coeffs", grid, m, eos, r, view64(*ms), stream);
+}
+
+int redi_calc_coeffs_fp32(const redi_hgrid_t* grid, const redi_metrics_t* m,
+                          const redi_eos_t* eos, redi_t* r, const redi_state32_t* ms,
+                          void* stream) {
+  if (!grid || !m || !eos || !ms) return fail(REDI_ERR_ARG, "redi_calc_coeffs_fp32: null argument");
+  if (r && r->form != REDI_FORM_FP32_STORE)
+    return wrong_form("redi_calc_coeffs_fp32", r, REDI_FORM_FP32_STORE);
+  return calc_coeffs_impl<Fp32>("redi_calc_coeffs_fp32", grid, m, eos, r, view32(*ms), stream);
 }
 
 int redi_apply_flux(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
                     const redi_state_t* ms, double dt, const double* kue, const double* kve,
                     const redi_bc_t* bc, void* stream) {
   if (!grid || !m || !ms) return fail(REDI_ERR_ARG, "redi_apply_flux: null argument");
-  // silent guards (:951-959, :978)
-  if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
-  if (!ms->h_layer || !ms->tracers || ms->ntracers < 1) return REDI_SKIPPED;
-  if (r->nz != ms->nz_ml) return REDI_SKIPPED;
-  const bool use_ext = kue && kve;
-  if (!use_ext && r->khtr <= 0.0) return REDI_SKIPPED;
-  if (grid->nx_total != r->nx || grid->ny_total != r->ny)
-    return fail(REDI_ERR_ARG, "redi_apply_flux: grid %dx%d does not match the state %dx%d",
-                grid->nx_total, grid->ny_total, r->nx, r->ny);
-  if (!m->dy_cu || !m->dx_cv || !m->idxCu || !m->idyCv || !m->areaT)
-    return fail(REDI_ERR_ARG, "redi_apply_flux: null metric");
-  for (int t = 0; t < ms->ntracers; ++t)
-    if (!ms->tracers[t]) return fail(REDI_ERR_ARG, "redi_apply_flux: tracer %d is null", t + 1);
-  PhaseB p;
-  p.nx = r->nx; p.ny = r->ny; p.nz = r->nz;
-  p.nghost = grid->nghost; p.nxp = grid->nx_phys; p.nyp = grid->ny_phys;
-  p.dt = dt;
-  p.ww = bc ? (bc->west == REDI_OBC_WALL) : 1;
-  p.we = bc ? (bc->east == REDI_OBC_WALL) : 1;
-  p.ws = bc ? (bc->south == REDI_OBC_WALL) : 1;
-  p.wn = bc ? (bc->north == REDI_OBC_WALL) : 1;
-  p.khtr_u_ext = use_ext ? kue : nullptr;
-  p.khtr_v_ext = use_ext ? kve : nullptr;
-  p.khtr = r->khtr;
-  p.dy_cu = m->dy_cu; p.dx_cv = m->dx_cv; p.idxCu = m->idxCu; p.idyCv = m->idyCv;
-  p.areaT = m->areaT; p.h = ms->h_layer;
-  return launch_phase_b(p, ms->ntracers, ms->tracers, r->fs, r->ws, r->tiles, (hipStream_t)stream,
-                        r->timer);
+  if (r && r->form != REDI_FORM_FP64) return wrong_form("redi_apply_flux", r, REDI_FORM_FP64);
+  return apply_flux_impl<Fp64>("redi_apply_flux", grid, m, r, view64(*ms), dt, kue, kve, bc, stream);
+}
+
+int redi_apply_flux_fp32(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
+                         const redi_state32_t* ms, double dt, const double* kue,
+                         const double* kve, const redi_bc_t* bc, void* stream) {
+  if (!grid || !m || !ms) return fail(REDI_ERR_ARG, "redi_apply_flux_fp32: null argument");
+  if (r && r->form != REDI_FORM_FP32_STORE)
+    return wrong_form("redi_apply_flux_fp32", r, REDI_FORM_FP32_STORE);
+  return apply_flux_impl<Fp32>("redi_apply_flux_fp32", grid, m, r, view32(*ms), dt, kue, kve, bc,
+                               stream);
 }
 
 int redi_apply_flux_hoist(const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
@@ -621,6 +749,20 @@ int redi_step(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_
     const int a = redi_calc_coeffs(grid, m, eos, r, ms, stream);
     if (a < 0) return a;
     const int b = redi_apply_flux(grid, m, r, ms, dt, kue, kve, bc, stream);
+    if (b < 0) return b;
+    if (a == REDI_SKIPPED && b == REDI_SKIPPED) rc = REDI_SKIPPED;
+  }
+  return rc;
+}
+
+int redi_step_fp32(const redi_hgrid_t* grid, const redi_metrics_t* m, const redi_eos_t* eos,
+                   redi_t* r, const redi_state32_t* ms, double dt, const double* kue,
+                   const double* kve, const redi_bc_t* bc, int nsteps, void* stream) {
+  int rc = REDI_OK;
+  for (int n = 0; n < nsteps; ++n) {
+    const int a = redi_calc_coeffs_fp32(grid, m, eos, r, ms, stream);
+    if (a < 0) return a;
+    const int b = redi_apply_flux_fp32(grid, m, r, ms, dt, kue, kve, bc, stream);
     if (b < 0) return b;
     if (a == REDI_SKIPPED && b == REDI_SKIPPED) rc = REDI_SKIPPED;
   }
@@ -650,18 +792,17 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
   fs.khtr_v = cf->khtr_v;
   const hipStream_t st = (hipStream_t)stream;
   const Tiles tiles;
-  rc = launch_phase_a(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1], tracers[idx_S - 1],
-                      wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
+  rc = launch_phase_a<Fp64>(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1],
+                            tracers[idx_S - 1], wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
   if (rc) return rc;
-  rc = export_coeffs(fs, nx, ny, nz, *cf, false, false, st);
+  rc = export_coeffs<Fp64>(fs, nx, ny, nz, *cf, false, false, st);
   if (rc) return rc;
   PhaseB p;
   p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;
   p.ww = ww; p.we = we; p.ws = wsz; p.wn = wn;
   p.khtr_u_ext = kue; p.khtr_v_ext = kve; p.khtr = 0.0;
   p.dy_cu = dy_cu; p.dx_cv = dx_cv; p.idxCu = idxCu; p.idyCv = idyCv; p.areaT = areaT;
-  p.h = h_layer;
-  return launch_phase_b(p, nt, tracers, fs, fsl->ws, tiles, st, g_no_timer);
+  return launch_phase_b<Fp64>(p, h_layer, nt, tracers, fs, fsl->ws, tiles, st, g_no_timer);
 }
 
 int redi_timing_enable(redi_t* r, int on) {

@@ -75,7 +75,10 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_calc_coeffs", "redi_apply_flux", "redi_apply_flux_hoist", "redi_step",
             "redi_workspace_bytes", "redi_last_error", "redi_version", "redi_opt_launch",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
-            "redi_kernel_name")
+            "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
+            "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
+            "redi_set_tendency_output")
+FORMS = {"fp64": 0, "fp32_store": 1}  # REDI_FORM_*
 NKERNELS = 7  # REDI_NKERNELS
 
 
@@ -97,11 +100,13 @@ def lib() -> ctypes.CDLL:
             raise RediError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         L.redi_init.argtypes = [ctypes.POINTER(_P), ctypes.POINTER(HGrid), _I]
+        L.redi_init_form.argtypes = [ctypes.POINTER(_P), ctypes.POINTER(HGrid), _I, _I]
         L.redi_destroy.argtypes = [_P]
         L.redi_configure.argtypes = [_P, _I, _D, _I, _D]
         L.redi_export_coeffs.argtypes = [_P, ctypes.POINTER(Coeffs), _P]
         L.redi_set_tiles.argtypes = [_P, _I, _I]
         L.redi_set_recon_order.argtypes = [_P, _I]
+        L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
                                              ctypes.POINTER(ctypes.c_longlong)]
@@ -112,8 +117,14 @@ def lib() -> ctypes.CDLL:
         L.redi_apply_flux.argtypes = [ctypes.POINTER(HGrid), ctypes.POINTER(Metrics), _P,
                                       ctypes.POINTER(State), _D, _P, _P, ctypes.POINTER(Bc), _P]
         L.redi_apply_flux_hoist.argtypes = L.redi_apply_flux.argtypes
+        # the FP32-store entries: redi_state32_t has redi_state_t's layout
+        L.redi_calc_coeffs_fp32.argtypes = L.redi_calc_coeffs.argtypes
+        L.redi_apply_flux_fp32.argtypes = L.redi_apply_flux.argtypes
         L.redi_step.argtypes = [ctypes.POINTER(HGrid), ctypes.POINTER(Metrics), ctypes.POINTER(Eos),
                                 _P, ctypes.POINTER(State), _D, _P, _P, ctypes.POINTER(Bc), _I, _P]
+        L.redi_step_fp32.argtypes = L.redi_step.argtypes
+        L.redi_state_bytes_form.restype = ctypes.c_longlong
+        L.redi_state_bytes_form.argtypes = [_I, _I, _I, _I]
         L.redi_workspace_bytes.restype = ctypes.c_longlong
         L.redi_workspace_bytes.argtypes = [_I, _I, _I]
         L.redi_last_error.restype = ctypes.c_char_p
@@ -148,16 +159,22 @@ def _stream(device) -> _P:
 
 
 class DeviceDomain:
-    """One ``RediCase`` resident in HBM plus its ctypes descriptors."""
+    """One ``RediCase`` resident in HBM plus its ctypes descriptors.  In the
+    FP32-store form h and the tracers are held in FP32 (rounded from the
+    case's FP64 fields); the metrics stay FP64."""
 
-    def __init__(self, case: RediCase, device: str = "cuda"):
+    def __init__(self, case: RediCase, device: str = "cuda", form: str = "fp64"):
         import torch
+        if form not in FORMS:
+            raise ValueError(f"unknown form {form!r}")
         self.case = case
+        self.form = form
         self.device = torch.device(device)
         dev = self.device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-        self.h = t(case.h)
-        self.hTr = t(case.hTr)  # (nt, nz, ny, nx)
+        fdt = torch.float32 if form == "fp32_store" else torch.float64
+        self.h = t(case.h).to(fdt)
+        self.hTr = t(case.hTr).to(fdt)  # (nt, nz, ny, nx)
         self.dy_cu, self.dx_cv = t(case.dy_cu), t(case.dx_cv)
         self.idxCu, self.idyCv = t(case.idxCu), t(case.idyCv)
         self.areaT, self.wet_u, self.wet_v = t(case.areaT), t(case.wet_u), t(case.wet_v)
@@ -179,10 +196,11 @@ class DeviceDomain:
                            ctypes.cast(self._trp, ctypes.POINTER(_P)))
 
     def set_tracers(self, hTr) -> None:
-        """Overwrite the tracers in place (host array or tensor)."""
+        """Overwrite the tracers in place (host array or tensor; converted to
+        the domain's storage type)."""
         import torch
         src = hTr if isinstance(hTr, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(hTr))
-        self.hTr.copy_(src)
+        self.hTr.copy_(src.to(self.hTr.dtype))
 
     def tracers_host(self) -> np.ndarray:
         return self.hTr.cpu().numpy()
@@ -195,9 +213,13 @@ class OceanRedi:
     it in the reference layout.
     """
 
-    def __init__(self, case_or_grid, nz: Optional[int] = None, device: str = "cuda"):
+    def __init__(self, case_or_grid, nz: Optional[int] = None, device: str = "cuda",
+                 form: str = "fp64"):
         import torch
         L = lib()
+        if form not in FORMS:
+            raise ValueError(f"unknown form {form!r}")
+        self.form = form
         if isinstance(case_or_grid, RediCase):
             c = case_or_grid
             grid = HGrid(c.nx, c.ny, c.nx_phys, c.ny_phys, c.nghost, 0, 0.0, 0.0)
@@ -210,7 +232,8 @@ class OceanRedi:
             self.nz = max(1, int(nz))
             self.ns = 2 * self.nz + 2
             h = _P()
-            _check(L.redi_init(ctypes.byref(h), ctypes.byref(grid), int(nz)), "redi_init")
+            _check(L.redi_init_form(ctypes.byref(h), ctypes.byref(grid), int(nz), FORMS[form]),
+                   "redi_init")
             self.handle = h
         self.enable = False
         self.khtr = 0.0
@@ -234,11 +257,14 @@ class OceanRedi:
             pass
 
     # -- public entries --------------------------------------------------------
+    def _fp32(self) -> bool:
+        return self.form == "fp32_store"
+
     def calc_coeffs(self, dom: DeviceDomain) -> int:
         """redi_calc_coeffs (redi/ocean_redi.F90:572-607)."""
-        return _check(lib().redi_calc_coeffs(ctypes.byref(dom.grid), ctypes.byref(dom.metrics),
-                                             ctypes.byref(dom.eos), self.handle,
-                                             ctypes.byref(dom.state), _stream(self.device)),
+        fn = lib().redi_calc_coeffs_fp32 if self._fp32() else lib().redi_calc_coeffs
+        return _check(fn(ctypes.byref(dom.grid), ctypes.byref(dom.metrics), ctypes.byref(dom.eos),
+                         self.handle, ctypes.byref(dom.state), _stream(self.device)),
                       "redi_calc_coeffs")
 
     def apply_flux(self, dom: DeviceDomain, dt: Optional[float] = None,
@@ -258,7 +284,10 @@ class OceanRedi:
             bcp = ctypes.byref(bc)
         ku = _ptr(dom.khtr_u_ext) if use_ext else _P(0)
         kv = _ptr(dom.khtr_v_ext) if use_ext else _P(0)
-        fn = lib().redi_apply_flux_hoist if hoist else lib().redi_apply_flux
+        if self._fp32():
+            fn = lib().redi_apply_flux_fp32
+        else:
+            fn = lib().redi_apply_flux_hoist if hoist else lib().redi_apply_flux
         return _check(fn(ctypes.byref(dom.grid), ctypes.byref(dom.metrics), self.handle,
                          ctypes.byref(dom.state), float(dt), ku, kv, bcp,
                          _stream(self.device)), "redi_apply_flux")
@@ -267,10 +296,10 @@ class OceanRedi:
         c = dom.case
         ku = _ptr(dom.khtr_u_ext) if c.use_ext else _P(0)
         kv = _ptr(dom.khtr_v_ext) if c.use_ext else _P(0)
-        return _check(lib().redi_step(ctypes.byref(dom.grid), ctypes.byref(dom.metrics),
-                                      ctypes.byref(dom.eos), self.handle, ctypes.byref(dom.state),
-                                      float(c.dt), ku, kv, ctypes.byref(dom.bc), int(nsteps),
-                                      _stream(self.device)), "redi_step")
+        fn = lib().redi_step_fp32 if self._fp32() else lib().redi_step
+        return _check(fn(ctypes.byref(dom.grid), ctypes.byref(dom.metrics), ctypes.byref(dom.eos),
+                         self.handle, ctypes.byref(dom.state), float(c.dt), ku, kv,
+                         ctypes.byref(dom.bc), int(nsteps), _stream(self.device)), "redi_step")
 
     def timing(self, on: bool = True) -> None:
         """Bracket every kernel launch of this object with HIP events."""
@@ -285,6 +314,20 @@ class OceanRedi:
             if kt.launches[k]:
                 out[lib().redi_kernel_name(k).decode()] = (kt.ms[k], int(kt.launches[k]))
         return out
+
+    def set_tendency_output(self, tend) -> None:
+        """Write the FP64 tendency dTr / areaT of every later apply_flux into
+        ``tend`` (a float64 tensor (nt, nz, ny, nx) on the device), or stop
+        with ``None``."""
+        if tend is None:
+            self._tend_ptrs = None
+            _check(lib().redi_set_tendency_output(self.handle, 0, None), "redi_set_tendency_output")
+            return
+        nt = tend.shape[0]
+        self._tend_ptrs = (_P * nt)(*(tend[t].data_ptr() for t in range(nt)))
+        _check(lib().redi_set_tendency_output(self.handle, nt,
+                                              ctypes.cast(self._tend_ptrs, ctypes.POINTER(_P))),
+               "redi_set_tendency_output")
 
     def set_recon_order(self, layer_major: bool) -> None:
         _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
@@ -331,9 +374,10 @@ class OceanRedi:
         return t
 
 
-def make_redi(case: RediCase, device: str = "cuda"):
-    """(OceanRedi, DeviceDomain) configured from the case's knobs."""
-    dom = DeviceDomain(case, device)
-    r = OceanRedi(case, device=device)
+def make_redi(case: RediCase, device: str = "cuda", form: str = "fp64"):
+    """(OceanRedi, DeviceDomain) configured from the case's knobs, in the
+    given precision form ("fp64" or "fp32_store")."""
+    dom = DeviceDomain(case, device, form)
+    r = OceanRedi(case, device=device, form=form)
     r.configure(enable=bool(case.enable), khtr=case.khtr)
     return r, dom

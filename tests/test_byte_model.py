@@ -35,13 +35,14 @@ def test_hand_count_tiny_domain():
     assert bm.step_bytes(c) == 8 + 16 + 2 * 4 * (4 * 24 + 3 * 8) + 8 * 17
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32_store"])
 @pytest.mark.parametrize("group", [1, 2, 3, 4])
 @pytest.mark.parametrize("nt", [1, 2, 3, 4])
-def test_launch_split_adds_up(group, nt):
+def test_launch_split_adds_up(group, nt, form):
     c = _Sizes(100, 70, 30, nt)
-    launches = bm.step_launches(c, group)
+    launches = bm.step_launches(c, group, form)
     total = sum(sum(v) for v in launches.values())
-    assert total == pytest.approx(bm.step_bytes(c), rel=1e-12)
+    assert total == pytest.approx(bm.step_bytes(c, form), rel=1e-12)
     ngroups = -(-nt // max(1, min(group, nt)))
     assert len(launches["k_tracer_recon"]) == ngroups
     assert len(launches["k_cell_flux"]) == ngroups
