@@ -114,19 +114,24 @@ struct LayerVals {
 };
 
 // One layer of a tracer column (K3 output), FP64 in both forms: the layer
-// mean and the PPM interface values at its top and bottom.  The limited
-// edge values are not stored: each branch of ppm_limit yields them from
-// exactly these three values (unchanged, flattened to the mean, or one
-// edge rebuilt from the mean and the other), so the reader re-runs the
-// limiter on the same operands and obtains the same bits.  Every value a
-// guard compares across a face is then exact in both forms -- the sweep
-// places neutral surfaces where the T edges of the two columns nearly
-// agree, so rounding them would flip guards.
-struct ReconRec {
+// mean, the PPM interface values at its top and bottom, and the branch the
+// PPM limiter took.  Every branch yields the limited edges from exactly
+// these three values (kept, flattened to the mean, or one edge rebuilt from
+// the mean and the other), so the reader replays the branch on the same
+// operands and obtains the same bits.  Every value a guard compares across
+// a face is then exact in both forms -- the sweep places neutral surfaces
+// where the T edges of the two columns nearly agree, so rounding them
+// would flip guards.
+struct alignas(16) ReconRec {
   double Tl, Tt, Tb;
+  int32_t branch, pad;
+  static __device__ __forceinline__ ReconRec make(double tl, double et, double eb) {
+    double al = et, ar = eb;
+    return ReconRec{tl, et, eb, ppm_limit<double>(tl, al, ar), 0};
+  }
   __device__ __forceinline__ LayerVals get() const {
-    double al = Tt, ar = Tb;
-    ppm_limit<double>(Tl, al, ar);
+    double al, ar;
+    ppm_limited_edges<double>(branch, Tl, Tt, Tb, al, ar);
     return LayerVals{Tl, al, ar, Tt, Tb};
   }
 };
@@ -442,7 +447,7 @@ __global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz,
   auto emit = [&](int k, const double* e, const double* s_above) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (k > 0) rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec{s_above[t], e_prev[t], e[t]};
+      if (k > 0) rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec::make(s_above[t], e_prev[t], e[t]);
       e_prev[t] = e[t];
     }
   };
@@ -527,7 +532,7 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
     ReconRec* rc = rec + (size_t)c * y.ca + (size_t)kl * y.cb;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      rc[(size_t)t * y.ct] = ReconRec{sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]};
+      rc[(size_t)t * y.ct] = ReconRec::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
   }
 }
 
