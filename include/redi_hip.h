@@ -166,6 +166,11 @@ int redi_set_recon_order(redi_t *self, int layer_major);
  * into the FP32 state. */
 int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
 
+/* Layers per workgroup of the per-column tiled reconstruction (8, 16 or 32,
+ * default 16; the workgroup takes 256 / kt columns).  Changes only the
+ * schedule, never the result. */
+int redi_set_recon_chunk(redi_t *self, int kt);
+
 /* Faces of the last redi_calc_coeffs whose contributing sublayers are not
  * monotone in layer index (u and v faces).  Synchronous. */
 int redi_nonmonotone_faces(const redi_t *self, long long *n_u, long long *n_v);

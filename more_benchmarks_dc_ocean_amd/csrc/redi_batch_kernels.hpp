@@ -1,0 +1,112 @@
+// redi_batch_kernels.hpp -- the Redi step over many independent domains of
+// uneven size, one persistent launch per kernel kind.
+//
+// Each batched kernel reads a work table from device memory: item e (one
+// domain's launch of that kernel kind -- one of its two face grids for the
+// sweep and the face fluxes) owns the virtual blocks [start[e], start[e] +
+// nblk[e]) of a virtual grid of count[1] blocks.  A fixed grid of workgroups
+// (sized to fill the machine once) strides over the virtual blocks and runs,
+// for each, the same kernel body the single-domain launch runs for that
+// block, with the item's arguments.  Every result is therefore the
+// single-domain result, bit for bit.
+//
+// The grid and the table addresses are the only launch arguments, so one
+// captured hipGraph serves every batch composition: adding or retiring a
+// domain rewrites the table contents, never the graph.
+//
+// Item starts are multiples of 8 and the persistent grid is a multiple of 8,
+// so virtual block b runs on XCD b % 8 and the single-domain kernels' XCD
+// banding (xcd_band over the item's own block index) keeps holding.
+#pragma once
+
+#include "redi_kernels.hpp"
+
+namespace redi {
+
+template <class Item>
+struct BatchView {
+  const uint32_t* __restrict__ count;  // [0] items, [1] virtual blocks
+  const uint32_t* __restrict__ start;  // first virtual block of item e (count[0] + 1 entries)
+  const uint32_t* __restrict__ nblk;   // blocks of item e (<= start[e+1] - start[e])
+  const Item* __restrict__ items;
+};
+
+// Runs body(item, local block, item blocks) for every virtual block this
+// workgroup owns.  kBarrier: the body uses LDS across waves, so the next
+// block must wait until every wave is done with it.
+template <bool kBarrier, class Item, class Body>
+__device__ __forceinline__ void for_each_block(const BatchView<Item>& v, Body&& body) {
+  const uint32_t total = v.count[1];
+  uint32_t e = 0;
+  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    while (v.start[e + 1] <= b) ++e;  // b only grows, so the search only moves forward
+    const uint32_t local = b - v.start[e], nb = v.nblk[e];
+    if (local < nb) {
+      body(v.items[e], (int)local, (int)nb);
+      if (kBarrier) __syncthreads();
+    }
+  }
+}
+
+template <class F>
+struct SweepItem {
+  SweepArgs<F> a;
+  TileMap tm;  // tm.tx * tm.ty == kBatchSweepLanes
+};
+constexpr int kBatchSweepLanes = 64;
+
+template <class F, int NT>
+struct ReconItem {
+  ReconArgs<F, NT> a;
+  int kt;  // layer chunk: 8, 16 or 32
+  int pad_;
+};
+
+template <class F>
+__global__ void __launch_bounds__(128) kb_column_state(BatchView<ColumnArgs<F>> v) {
+  for_each_block<false>(v, [](const ColumnArgs<F>& a, int bid, int) { column_state_body<F>(a, bid); });
+}
+
+template <class F>
+__global__ void __launch_bounds__(kBatchSweepLanes) kb_neutral_sweep(BatchView<SweepItem<F>> v) {
+  for_each_block<false>(v, [](const SweepItem<F>& it, int bid, int nb) {
+    neutral_sweep_body<F>(it.a, it.tm, bid, nb);
+  });
+}
+
+__global__ void __launch_bounds__(256) kb_face_khtr(BatchView<KhtrArgs> v) {
+  for_each_block<false>(v, [](const KhtrArgs& a, int bid, int) { face_khtr_body(a, bid); });
+}
+
+template <class F, int NT>
+__global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconItem<F, NT>> v) {
+  __shared__ double lds[recon_lds_max<NT>()];
+  for_each_block<true>(v, [&](const ReconItem<F, NT>& it, int bid, int nb) {
+    switch (it.kt) {
+      case 8: tracer_recon_tiled_body<F, NT, 8>(it.a, bid, nb, lds); break;
+      case 32: tracer_recon_tiled_body<F, NT, 32>(it.a, bid, nb, lds); break;
+      default: tracer_recon_tiled_body<F, NT, 16>(it.a, bid, nb, lds); break;
+    }
+  });
+}
+
+template <class F, int NT>
+__global__ void __launch_bounds__(64 * kFaceGroup) kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
+  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ uint32_t stg[kFaceGroup * 64];
+  for_each_block<true>(v, [&](const FaceFluxArgs<F>& a, int bid, int nb) {
+    face_flux_body<F, NT>(a, bid, nb, sx, stg);
+  });
+}
+
+// One wavefront per workgroup and per-lane accumulator columns: no barrier
+// between blocks.
+template <class F, int NT, int KCAP>
+__global__ void __launch_bounds__(64) kb_cell_flux(BatchView<CellFluxArgs<F>> v) {
+  __shared__ double acc[KCAP * 64];
+  for_each_block<false>(v, [&](const CellFluxArgs<F>& a, int bid, int nb) {
+    cell_flux_body<F, NT, KCAP>(a, bid, nb, acc);
+  });
+}
+
+}  // namespace redi

@@ -165,8 +165,9 @@ struct TileMap {
   int n0, n1;    // index space extent (fast, slow)
   int tx, ty;    // tile shape (tx * ty == blockDim.x)
   int tiles0;    // tiles along the fast axis
-  __device__ __forceinline__ bool map(int& a, int& b) const {
-    const int tile = xcd_band((int)blockIdx.x, (int)gridDim.x);
+  // lanes of block `bid` of `nb` (tiles beyond the index space map outside it)
+  __device__ __forceinline__ bool map(int bid, int nb, int& a, int& b) const {
+    const int tile = xcd_band(bid, nb);
     const int t0 = tile % tiles0, t1 = tile / tiles0;
     a = t0 * tx + (int)(threadIdx.x % (unsigned)tx);
     b = t1 * ty + (int)(threadIdx.x / (unsigned)tx);
@@ -182,18 +183,30 @@ struct TileMap {
 // derivatives' products, and FP32 derivatives move neutral surfaces into
 // other layers.
 template <class F>
-__global__ void __launch_bounds__(128) k_column_state(int N2, int nz,
-                                                      const typename F::R* __restrict__ h,
-                                                      const typename F::R* __restrict__ thtr,
-                                                      const typename F::R* __restrict__ shtr,
-                                                      EosParams eos, ColRec* __restrict__ rec,
-                                                      double* __restrict__ Pout) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N2) return;
-  const size_t n2 = (size_t)N2;
+struct ColumnArgs {
+  int N2, nz;
+  const typename F::R* h;
+  const typename F::R* thtr;
+  const typename F::R* shtr;
+  EosParams eos;
+  ColRec* rec;
+  double* P;
+};
+
+// Lane = column; `bid` is the workgroup's block index (blockDim.x lanes).
+template <class F>
+__device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bid) {
+  const int c = bid * (int)blockDim.x + (int)threadIdx.x;
+  if (c >= a.N2) return;
+  const int nz = a.nz;
+  const size_t n2 = (size_t)a.N2;
+  const typename F::R* __restrict__ h = a.h;
+  const typename F::R* __restrict__ thtr = a.thtr;
+  const typename F::R* __restrict__ shtr = a.shtr;
+  const EosParams eos = a.eos;
   const double g_rho0 = kGravity * eos.rho0;
-  ColRec* r = rec + (size_t)c * (nz + 1);
-  double* pc = Pout + (size_t)c * (nz + 1);
+  ColRec* r = a.rec + (size_t)c * (nz + 1);
+  double* pc = a.P + (size_t)c * (nz + 1);
   double P = 0.0;  // Pint(1) = 0
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
@@ -210,6 +223,11 @@ __global__ void __launch_bounds__(128) k_column_state(int N2, int nz,
     pc[k] = P;
   };
   stream_edges<double, 2>(nz, load, emit);
+}
+
+template <class F>
+__global__ void __launch_bounds__(128) k_column_state(ColumnArgs<F> a) {
+  column_state_body<F>(a, (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,10 +289,11 @@ struct SweepArgs {
 // FP64 in both forms; positions and thicknesses are stored in the form's
 // type.
 template <class F>
-__global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap tm) {
+__device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const TileMap& tm,
+                                                   int bid, int nb) {
   using Face = FaceRecF<F>;
   int fi, fj;
-  if (!tm.map(fi, fj)) return;
+  if (!tm.map(bid, nb, fi, fj)) return;
   const int i = a.i0 + fi, j = a.j0 + fj;
   const size_t fo = (size_t)i + (size_t)a.nfa * j;
   const int nk = a.nz, ns = 2 * nk + 2;
@@ -401,6 +420,11 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap t
   a.mono[fo] = (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0));
 }
 
+template <class F>
+__global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap tm) {
+  neutral_sweep_body<F>(a, tm, (int)blockIdx.x, (int)gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // K3: tracer column reconstruction (redi_tracer_column, :852-891), the NT
 // tracers of a group per launch.  ReconRec (k, col, t) at col*ca + k*cb +
@@ -422,64 +446,90 @@ struct ReconLayout {
   uint32_t ca, cb, ct;
 };
 
+template <class F, int NT>
+struct ReconArgs {
+  int N2, nz;
+  const typename F::R* h;
+  TracerSetT<typename F::R> tr;
+  ReconRec* rec;
+  ReconLayout y;
+};
+
 // The reconstruction is computed in FP64 in both forms, from the stored
 // fields.
 // Streaming form: one lane per column walks it top-down with a register
 // window (stream_edges).
 template <class F, int NT>
-__global__ void __launch_bounds__(128) k_tracer_recon(int N2, int nz,
-                                                      const typename F::R* __restrict__ h,
-                                                      TracerSetT<typename F::R> tr,
-                                                      ReconRec* __restrict__ rec,
-                                                      ReconLayout y) {
+__global__ void __launch_bounds__(128) k_tracer_recon(ReconArgs<F, NT> a) {
+  const int N2 = a.N2, nz = a.nz;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
+  const typename F::R* __restrict__ h = a.h;
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
     hh = (double)h[o];
     const double he = hh > kHDivEps ? hh : kHDivEps;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) s[t] = (double)tr.q[t][o] / he;
+    for (int t = 0; t < NT; ++t) s[t] = (double)a.tr.q[t][o] / he;
   };
   double e_prev[NT];
-  ReconRec* rc = rec + (size_t)c * y.ca;
+  ReconRec* rc = a.rec + (size_t)c * a.y.ca;
   auto emit = [&](int k, const double* e, const double* s_above) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (k > 0) rc[(size_t)(k - 1) * y.cb + (size_t)t * y.ct] = ReconRec::make(s_above[t], e_prev[t], e[t]);
+      if (k > 0) rc[(size_t)(k - 1) * a.y.cb + (size_t)t * a.y.ct] = ReconRec::make(s_above[t], e_prev[t], e[t]);
       e_prev[t] = e[t];
     }
   };
   stream_edges<double, NT>(nz, load, emit);
 }
 
-// Tiled form: a workgroup takes kReconCols adjacent columns x kReconLayers
-// layers.  Layer values, PLM slopes and PPM edges are staged in LDS stage by
-// stage (every value from the same functions on the same operands as
-// stream_edges, including its clamped windows at the top and bottom), so
-// global reads run along columns and the per-column record runs are written
-// along layers.  Per-column layout only (cb = 1).
-constexpr int kReconCols = 16, kReconLayers = 16;
+// Tiled form: a workgroup of kReconLanes lanes takes CT = kReconLanes / KT
+// adjacent columns x KT layers (KT, the layer chunk, is 8, 16 or 32).  Layer
+// values, PLM slopes and PPM edges are staged in LDS stage by stage (every
+// value from the same functions on the same operands as stream_edges,
+// including its clamped windows at the top and bottom), so global reads run
+// along columns and the per-column record runs are written along layers.
+// The chunk changes only the schedule, never a result.  Per-column layout
+// only (cb = 1).
+constexpr int kReconLanes = 256;
+constexpr int kReconChunkDefault = 16;
 
-template <class F, int NT>
-__global__ void __launch_bounds__(kReconCols * kReconLayers)
-    k_tracer_recon_tiled(int N2, int nz, const typename F::R* __restrict__ h,
-                         TracerSetT<typename F::R> tr, ReconRec* __restrict__ rec,
-                         ReconLayout y) {
+
+// LDS doubles of one tiled-reconstruction workgroup
+template <int NT, int KT>
+constexpr int recon_lds_doubles() {
+  constexpr int CT = kReconLanes / KT;
+  return (KT + 4) * CT * (1 + NT) + NT * (KT + 2) * CT + NT * (KT + 1) * CT;
+}
+template <int NT>
+constexpr int recon_lds_max() {
+  constexpr int a = recon_lds_doubles<NT, 8>(), b = recon_lds_doubles<NT, 16>(),
+                c = recon_lds_doubles<NT, 32>();
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+template <class F, int NT, int KT>
+__device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& a, int bid, int nb,
+                                                        double* lds) {
   using R = double;
-  constexpr int CT = kReconCols, KT = kReconLayers, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
-  __shared__ R sh[NL][CT];        // h, layers k0-2 .. k0+KT+1 (clamped)
-  __shared__ R sv[NT][NL][CT];    // layer values hTr / max(h, eps), same layers
-  __shared__ R sd[NT][NSL][CT];   // PLM slopes, layers k0-1 .. k0+KT
-  __shared__ R se[NT][NE][CT];    // PPM edges, interfaces k0 .. k0+KT
+  constexpr int CT = kReconLanes / KT, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
+  // h, layers k0-2 .. k0+KT+1 (clamped); layer values hTr / max(h, eps), same
+  // layers; PLM slopes, layers k0-1 .. k0+KT; PPM edges, interfaces k0 .. k0+KT
+  R(*sh)[CT] = reinterpret_cast<R(*)[CT]>(lds);
+  R(*sv)[NL][CT] = reinterpret_cast<R(*)[NL][CT]>(lds + NL * CT);
+  R(*sd)[NSL][CT] = reinterpret_cast<R(*)[NSL][CT]>(lds + NL * CT * (1 + NT));
+  R(*se)[NE][CT] = reinterpret_cast<R(*)[NE][CT]>(lds + NL * CT * (1 + NT) + NT * NSL * CT);
+  const int N2 = a.N2, nz = a.nz;
+  const typename F::R* __restrict__ h = a.h;
   const int nkb = (nz + KT - 1) / KT;
-  const int b = xcd_band((int)blockIdx.x, (int)gridDim.x);
+  const int b = xcd_band(bid, nb);
   const int c0 = (b / nkb) * CT, k0 = (b % nkb) * KT;
   const int tid = (int)threadIdx.x;
   const size_t n2 = (size_t)N2;
   // 1: layer values, lanes along columns
-  for (int q = tid; q < NL * CT; q += CT * KT) {
+  for (int q = tid; q < NL * CT; q += kReconLanes) {
     const int r = q / CT, cc = q % CT;
     int kl = k0 - 2 + r;
     kl = kl < 0 ? 0 : (kl > nz - 1 ? nz - 1 : kl);
@@ -489,12 +539,12 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
     const R he = hh > R(kHDivEps) ? hh : R(kHDivEps);
     sh[r][cc] = hh;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)tr.q[t][o] / he;
+    for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)a.tr.q[t][o] / he;
   }
   __syncthreads();
   // 2: slopes of layers kl = k0-1+r; 0 for the first and last layer (the
   // slope stream_edges starts from and the one it skips)
-  for (int q = tid; q < NSL * CT; q += CT * KT) {
+  for (int q = tid; q < NSL * CT; q += kReconLanes) {
     const int r = q / CT, cc = q % CT;
     const int kl = k0 - 1 + r;  // rows r, r+1, r+2 of sh / sv hold layers kl-1, kl, kl+1
     const bool in = kl >= 1 && kl <= nz - 2;
@@ -506,7 +556,7 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
   }
   __syncthreads();
   // 3: edges at interfaces ki = k0+r
-  for (int q = tid; q < NE * CT; q += CT * KT) {
+  for (int q = tid; q < NE * CT; q += kReconLanes) {
     const int r = q / CT, cc = q % CT;
     const int ki = k0 + r;  // sh / sv rows r..r+3: layers ki-2 .. ki+1 (clamped);
                             // sd rows r, r+1: slopes of layers ki-1, ki
@@ -529,11 +579,23 @@ __global__ void __launch_bounds__(kReconCols * kReconLayers)
   const int kk = tid % KT, cc = tid / KT;
   const int kl = k0 + kk, c = c0 + cc;
   if (kl < nz && c < N2) {
-    ReconRec* rc = rec + (size_t)c * y.ca + (size_t)kl * y.cb;
+    ReconRec* rc = a.rec + (size_t)c * a.y.ca + (size_t)kl * a.y.cb;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      rc[(size_t)t * y.ct] = ReconRec::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
+      rc[(size_t)t * a.y.ct] = ReconRec::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
   }
+}
+
+template <class F, int NT, int KT>
+__global__ void __launch_bounds__(kReconLanes) k_tracer_recon_tiled(ReconArgs<F, NT> a) {
+  __shared__ double lds[recon_lds_doubles<NT, KT>()];
+  tracer_recon_tiled_body<F, NT, KT>(a, (int)blockIdx.x, (int)gridDim.x, lds);
+}
+
+// Workgroups of the tiled reconstruction of one domain.
+inline unsigned recon_tiled_blocks(int N2, int nz, int kt) {
+  const int ct = kReconLanes / kt;
+  return (unsigned)((N2 + ct - 1) / ct) * (unsigned)((nz + kt - 1) / kt);
 }
 
 // ---------------------------------------------------------------------------
@@ -610,13 +672,14 @@ struct FaceFluxArgs {
 // each column.  The kFaceGroup waves of a workgroup read one line of each
 // surface-major face-record row between them; their fluxes are staged in
 // LDS and written back as whole rows (kFaceGroup faces x NT tracers).
+// sx: kFaceGroup * 64 * NT fluxes (stored type), stg: kFaceGroup * 64
+// packed targets, both in LDS.
 template <class F, int NT>
-__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
+__device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
+                                               typename F::R* sx, uint32_t* stg) {
   using R = typename F::R;
-  __shared__ R sx[kFaceGroup * 64 * NT];  // fluxes, stored type
-  __shared__ uint32_t stg[kFaceGroup * 64];
   const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const size_t f0 = (size_t)xcd_band((int)blockIdx.x, (int)gridDim.x) * kFaceGroup;
+  const size_t f0 = (size_t)xcd_band(bid, nb) * kFaceGroup;
   const size_t fo = f0 + wv;
   const int i = (int)(fo % (size_t)a.nfa), j = (int)(fo / (size_t)a.nfa);
   const int across = a.di ? i : j;
@@ -696,6 +759,13 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a
   }
 }
 
+template <class F, int NT>
+__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
+  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ uint32_t stg[kFaceGroup * 64];
+  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
+}
+
 // ---------------------------------------------------------------------------
 // K5: flux divergence per cell (redi_apply_flux_impl, :1094-1181)
 // ---------------------------------------------------------------------------
@@ -723,16 +793,17 @@ constexpr int kCellKCap[4] = {32, 76, 128, 256};
 // into dTr(nz+1-Ko(s)), exactly the order of the reference's loops,
 // whatever the pattern of Ko.  The lanes of a wavefront read one contiguous
 // stretch of each surface-major sublayer row.  Then hTr += dTr * (1/areaT).
+// acc: KCAP * 64 doubles of LDS.
 template <class F, int NT, int KCAP>
-__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs<F> a) {
+__device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid, int nb,
+                                               double* acc) {
   using R = typename F::R;
-  __shared__ double acc[KCAP * 64];
   constexpr int kCells = 64 / NT;  // cells per wavefront
   constexpr int kBatch = 32;       // sublayer rows in flight per face
   const int lane = (int)threadIdx.x;
   const int cw = lane / NT, t = lane % NT;
   const int segs = (a.nx + kCells - 1) / kCells;
-  const int w = xcd_band((int)blockIdx.x, (int)gridDim.x);
+  const int w = xcd_band(bid, nb);
   const int cj = w / segs, ci = (w % segs) * kCells + cw;
   const bool cell = cw < kCells && ci < a.nx && cj < a.ny;
   const int nz = a.nz, nsub = 2 * nz + 1;
@@ -807,6 +878,19 @@ __global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs<F> a) {
   }
 }
 
+template <class F, int NT, int KCAP>
+__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs<F> a) {
+  __shared__ double acc[KCAP * 64];
+  cell_flux_body<F, NT, KCAP>(a, (int)blockIdx.x, (int)gridDim.x, acc);
+}
+
+// Workgroups of k_cell_flux for an nx x ny domain.
+template <int NT>
+inline unsigned cell_flux_blocks(int nx, int ny) {
+  constexpr int kCells = 64 / NT;
+  return (unsigned)ny * (unsigned)((nx + kCells - 1) / kCells);
+}
+
 // ---------------------------------------------------------------------------
 // Face records <-> reference arrays, khtr faces, initial fill
 // ---------------------------------------------------------------------------
@@ -834,16 +918,25 @@ __global__ void k_export_coeffs(const FaceRecF<F>* __restrict__ rec, uint32_t fa
   if ((int)s < ns - 1) out.hEff[t] = r.hEff;
 }
 
-__global__ void k_face_khtr(int nu, int nv, const double* __restrict__ ext_u,
-                            const double* __restrict__ ext_v, double khtr, double* __restrict__ ku,
-                            double* __restrict__ kv) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nu) {
-    ku[t] = ext_u ? ext_u[t] : khtr;  // redi_face_copy / redi_face_const (:1015-1035)
-  } else if (t < nu + nv) {
-    kv[t - nu] = ext_v ? ext_v[t - nu] : khtr;
+struct KhtrArgs {
+  int nu, nv;
+  const double* ext_u;  // NULL: the scalar khtr
+  const double* ext_v;
+  double khtr;
+  double* ku;
+  double* kv;
+};
+
+__device__ __forceinline__ void face_khtr_body(const KhtrArgs& a, int bid) {
+  const int t = bid * (int)blockDim.x + (int)threadIdx.x;
+  if (t < a.nu) {
+    a.ku[t] = a.ext_u ? a.ext_u[t] : a.khtr;  // redi_face_copy / redi_face_const (:1015-1035)
+  } else if (t < a.nu + a.nv) {
+    a.kv[t - a.nu] = a.ext_v ? a.ext_v[t - a.nu] : a.khtr;
   }
 }
+
+__global__ void k_face_khtr(KhtrArgs a) { face_khtr_body(a, (int)blockIdx.x); }
 
 // ocean_redi_init fill (:493-502): Po = hEff = 0, Ko = 1
 template <class F>

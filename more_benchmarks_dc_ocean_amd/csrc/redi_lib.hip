@@ -206,6 +206,7 @@ struct Tiles {
   int tx = 64, ty = 1;       // one 64-lane wavefront per tile
   int recon_layer_major = 0; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
+  int recon_kt = kReconChunkDefault;  // layers per tiled-reconstruction workgroup
 };
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
@@ -226,7 +227,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   ColRec* rec = (ColRec*)ws;
   double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
-  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, t, s, eos, rec, P);
+  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, h, t, s, eos, rec, P});
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
@@ -266,11 +267,16 @@ void launch_recon(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
                   const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
                   hipStream_t st) {
   const int N2 = p.nx * p.ny, nz = p.nz;
+  const ReconArgs<F, NT> ra{N2, nz, h, ts, recon, ry};
   if (tiles.recon_layer_major) {
-    k_tracer_recon<F, NT><<<blocks_for(N2, 128), 128, 0, st>>>(N2, nz, h, ts, recon, ry);
+    k_tracer_recon<F, NT><<<blocks_for(N2, 128), 128, 0, st>>>(ra);
   } else {
-    const unsigned nbr = blocks_for(N2, kReconCols) * (unsigned)((nz + kReconLayers - 1) / kReconLayers);
-    k_tracer_recon_tiled<F, NT><<<nbr, kReconCols * kReconLayers, 0, st>>>(N2, nz, h, ts, recon, ry);
+    const unsigned nbr = recon_tiled_blocks(N2, nz, tiles.recon_kt);
+    switch (tiles.recon_kt) {
+      case 8: k_tracer_recon_tiled<F, NT, 8><<<nbr, kReconLanes, 0, st>>>(ra); break;
+      case 32: k_tracer_recon_tiled<F, NT, 32><<<nbr, kReconLanes, 0, st>>>(ra); break;
+      default: k_tracer_recon_tiled<F, NT, 16><<<nbr, kReconLanes, 0, st>>>(ra); break;
+    }
   }
 }
 
@@ -315,7 +321,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   int slot = tm.begin(REDI_K_FACE_KHTR, st);
   k_face_khtr<<<blocks_for((size_t)nu + nv, 256), 256, 0, st>>>(
-      nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v);
+      KhtrArgs{nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v});
   tm.end(slot, st);
   const int slots = recon_slots(nt);
   const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
@@ -670,6 +676,14 @@ int redi_set_tendency_output(redi_t* r, int ntracers, double* const* tend) {
   if (ntracers < 0 || (ntracers > 0 && !tend))
     return fail(REDI_ERR_ARG, "redi_set_tendency_output: bad arguments");
   r->tend.assign(tend, tend + ntracers);
+  return REDI_OK;
+}
+
+int redi_set_recon_chunk(redi_t* r, int kt) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_recon_chunk: null handle");
+  if (kt != 8 && kt != 16 && kt != 32)
+    return fail(REDI_ERR_ARG, "redi_set_recon_chunk: layer chunk %d is not 8, 16 or 32", kt);
+  r->tiles.recon_kt = kt;
   return REDI_OK;
 }
 

@@ -77,7 +77,7 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
-            "redi_set_tendency_output")
+            "redi_set_tendency_output", "redi_set_recon_chunk")
 FORMS = {"fp64": 0, "fp32_store": 1}  # REDI_FORM_*
 NKERNELS = 7  # REDI_NKERNELS
 
@@ -106,6 +106,7 @@ def lib() -> ctypes.CDLL:
         L.redi_export_coeffs.argtypes = [_P, ctypes.POINTER(Coeffs), _P]
         L.redi_set_tiles.argtypes = [_P, _I, _I]
         L.redi_set_recon_order.argtypes = [_P, _I]
+        L.redi_set_recon_chunk.argtypes = [_P, _I]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
@@ -327,11 +328,15 @@ class OceanRedi:
         self._tend_ptrs = (_P * nt)(*(tend[t].data_ptr() for t in range(nt)))
         _check(lib().redi_set_tendency_output(self.handle, nt,
                                               ctypes.cast(self._tend_ptrs, ctypes.POINTER(_P))),
-               "redi_set_tendency_output")
+               "redi_set_tendency_output", "redi_set_recon_chunk")
 
     def set_recon_order(self, layer_major: bool) -> None:
         _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
                "redi_set_recon_order")
+
+    def set_recon_chunk(self, kt: int) -> None:
+        """Layers per tiled-reconstruction workgroup (8, 16 or 32)."""
+        _check(lib().redi_set_recon_chunk(self.handle, int(kt)), "redi_set_recon_chunk")
 
     def nonmonotone_faces(self) -> tuple:
         nu, nv = ctypes.c_longlong(), ctypes.c_longlong()
