@@ -167,6 +167,118 @@ def time_form(case, form: str, steps: int, warmup: int, workload: str) -> dict:
             "launch_shape": shape, "kernel_ms_per_launch": kernels}
 
 
+def time_batched(cases, form: str, steps: int, warmup: int, persist: int = 1,
+                 tuned: bool = True) -> dict:
+    """Batched workload: every domain of ``cases`` in one RediBatch, timed
+    per step (one hipGraph launch, persistent kernels over all domains) and,
+    on the same resident state, as one launch sequence per domain."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
+    from more_benchmarks_dc_ocean_amd.batch import RediBatch, domain_bytes
+    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain
+
+    dev = "cuda:0"
+    cache = TuneCache(TUNED)
+    arena = sum(domain_bytes(c.nx, c.ny, c.nz, form, 4) for c in cases)
+    b = RediBatch(form=form, group=4, max_domains=len(cases), arena_bytes=arena, persist=persist,
+                  device=dev)
+    doms, shapes = [], []
+    for c in cases:
+        d = DeviceDomain(c, dev, form)
+        shape = cache.batch_shape(c.nx, c.ny, c.nz, form) if tuned else None
+        b.add(d, shape)
+        doms.append(d)
+        shapes.append(shape)
+    cells = sum(c.interior_cells for c in cases)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn) -> float:
+        for _ in range(warmup):
+            fn(1)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn(1)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_batched = timed(b.step)
+    log(f"[batched {form}] {len(cases)} domains, {cells / 1e6:.1f} M cells: {ms_batched:.3f} ms/step")
+    ms_serial = timed(b.step_serial)
+    log(f"[batched {form}] one launch sequence per domain: {ms_serial:.3f} ms/step")
+    from more_benchmarks_dc_ocean_amd import byte_model
+    alg = sum(byte_model.step_bytes(c, form) for c in cases)
+    b.close()
+    del doms
+    torch.cuda.empty_cache()
+    hist = {}
+    for sh in shapes:
+        k = "default" if sh is None else f"{sh['tile_i']}x{sh['tile_j']}/k{sh['kchunk']}"
+        hist[k] = hist.get(k, 0) + 1
+    return {"domains": len(cases), "cells": cells, "form": form, "persist": persist,
+            "value": round(cells / (ms_batched * 1e-3) / 1e6, 3), "unit": "Mcell-steps/s",
+            "ms_per_step": round(ms_batched, 4),
+            "serial_value": round(cells / (ms_serial * 1e-3) / 1e6, 3),
+            "serial_ms_per_step": round(ms_serial, 4),
+            "speedup_vs_serial": round(ms_serial / ms_batched, 4),
+            "step_alg_GBps": round(alg / (ms_batched * 1e-3) / 1e9, 2),
+            "step_frac": round(alg / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "shapes": hist, "arena_GiB": round(arena / 2**30, 2)}
+
+
+def run_served(rate: float, duration: float, npool: int, budget_gb: float, seed: int = 0) -> dict:
+    """Served workload: Poisson step requests against the continuous-batching
+    server (FP32-store), and the same requests as serial plain-FP64 solves."""
+    import torch
+    from more_benchmarks_dc_ocean_amd import serve
+    from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
+    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain, make_redi
+
+    dev = "cuda:0"
+    t0 = time.time()
+    cases = serve.pool_cases(npool, seed)
+    log(f"[served] pool of {npool} domains generated in {time.time() - t0:.1f}s")
+    # serial plain-FP64 step time of every pool domain (the baseline's service times)
+    step_s = []
+    for c in cases:
+        r, d = make_redi(c, dev, "fp64")
+        r.step(d, 1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r.step(d, 3)
+        torch.cuda.synchronize()
+        step_s.append((time.perf_counter() - t1) / 3)
+        r.close()
+        del d
+    torch.cuda.empty_cache()
+    cache = TuneCache(TUNED)
+    pool = [DeviceDomain(c, dev, "fp32_store") for c in cases]
+    shapes = [cache.batch_shape(c.nx, c.ny, c.nz, "fp32_store") for c in cases]
+    srv = serve.RediServer(pool, int(budget_gb * 1e9), "fp32_store", max_slots=256, shapes=shapes,
+                           device=dev)
+    # warm the graph and the kernels on one request of each domain
+    warm = [serve.Request(rid=i, arrival=0.0, nsteps=1, domain=i) for i in range(npool)]
+    srv.run(warm)
+    reqs = serve.poisson_requests(rate, duration, npool, seed)
+    log(f"[served] {len(reqs)} requests over {duration:.0f}s at {rate:.0f}/s")
+    st = srv.run(reqs, log=log)
+    st.update({"rate_per_s": rate, "duration_s": duration, "pool_domains": npool,
+               "budget_GB": budget_gb, "iterations": srv.iterations, "max_active": srv.max_active,
+               "form": "fp32_store"})
+    srv.close()
+    del pool
+    torch.cuda.empty_cache()
+    base = serve.fifo_serial(reqs, step_s, cases)
+    base["form"] = "fp64"
+    log(f"[served] batched p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
+        f"{st['value']:.1f} Mcell-steps/s; serial FP64 p50 {base['p50_ms']:.1f} ms "
+        f"p99 {base['p99_ms']:.1f} ms {base['value']:.1f} Mcell-steps/s")
+    return {**{k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
+            "serial_fp64": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in base.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +287,13 @@ def main() -> int:
     ap.add_argument("--workload", default="two_fronts_1440x1080x75", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "fp64", "fp32_store"])
+    ap.add_argument("--batched", type=int, default=96,
+                    help="domains of the batched leg (0 = skip it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="arrival window of the served leg (0 = skip it)")
+    ap.add_argument("--serve-pool", type=int, default=24)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the Redi step of one "
@@ -216,6 +335,16 @@ def main() -> int:
         f32 = dict(res["fp32_store"])
         f32["ratio_vs_fp64"] = round(f32["value"] / res["fp64"]["value"], 4)
         line["forms"] = {"fp32_store": f32}
+    if args.batched > 0:
+        from more_benchmarks_dc_ocean_amd import inputs as _inp
+        t0 = time.time()
+        bcases = _inp.case_batch(args.batched, seed=0, nsteps=args.steps)
+        log(f"[batched] generated {len(bcases)} domains in {time.time() - t0:.1f}s")
+        line["batched"] = time_batched(bcases, "fp32_store", args.steps, args.warmup)
+        del bcases
+    if args.serve_seconds > 0:
+        line["served"] = run_served(args.serve_rate, args.serve_seconds, args.serve_pool,
+                                    args.serve_budget_gb)
     print(json.dumps(line), flush=True)
     return 0
 

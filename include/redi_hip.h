@@ -166,6 +166,14 @@ int redi_set_recon_order(redi_t *self, int layer_major);
  * into the FP32 state. */
 int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
 
+/* Opt-in neutral-slope taper (no counterpart in the reference, which does
+ * not taper): in every later redi_calc_coeffs, a sublayer whose neutral
+ * slope |S| = |zR - zL| / dx (z the sublayer's mid-depth in the right / left
+ * column, dx from idxCu / idyCv) exceeds slope_max has its hEff scaled by
+ * (slope_max / |S|)^2.  slope_max = 0 (the default) turns it off; every
+ * parity result with the reference is for the default. */
+int redi_set_slope_taper(redi_t *self, double slope_max);
+
 /* Layers per workgroup of the per-column tiled reconstruction (8, 16 or 32,
  * default 16; the workgroup takes 256 / kt columns).  Changes only the
  * schedule, never the result. */
@@ -302,6 +310,78 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp,
                    const double *khtr_v_ext, const double *dy_cu,
                    const double *dx_cv, const double *idxCu,
                    const double *idyCv, const double *areaT, void *stream);
+
+/* ---- batched form: many independent domains in one launch per kernel -----
+ * No counterpart in the reference, which steps one domain per call.  A batch
+ * holds up to max_domains independent domains of any size, each with its own
+ * grid, layer count, EOS, walls, dt and diffusivity, all in one precision
+ * form.  redi_batch_step advances every domain in the batch by one Redi step
+ * (redi_calc_coeffs then redi_apply_flux, as redi_step) with one hipGraph
+ * launch: one persistent kernel per kernel kind walks a device-resident work
+ * table that covers every domain.  Each domain's result is bit-identical to
+ * redi_step on that domain alone, in the same form.
+ *
+ * The Phase-A state and scratch of every domain are carved from one device
+ * arena of arena_bytes allocated at redi_batch_create; adding a domain
+ * allocates nothing on the device, and a domain that does not fit returns
+ * REDI_ERR_BUDGET.  The caller owns h, the tracers and the metrics of each
+ * domain (device pointers, which must stay valid while the domain is in the
+ * batch).  All calls on one batch must be made from one host thread and on
+ * one stream (or the caller orders the streams). */
+#define REDI_ERR_BUDGET (-6)
+
+typedef struct redi_batch_s redi_batch_t;
+
+typedef struct {
+    redi_hgrid_t grid;
+    int nz_ml;
+    int ntracers;         /* 1..group of the batch */
+    int idx_temperature;  /* 1-based */
+    int idx_salinity;
+    redi_metrics_t metrics;
+    redi_eos_t eos;
+    const void *h_layer;  /* (nx, ny, nz), device, the batch form's type */
+    void *const *tracers; /* host array of ntracers device pointers (nx, ny, nz) */
+    double dt;
+    double khtr;          /* used where khtr_u_ext / khtr_v_ext are NULL; must be > 0 then */
+    const double *khtr_u_ext, *khtr_v_ext; /* both non-NULL = the use_ext path */
+    redi_bc_t bc;
+    int tile_i, tile_j;   /* sweep tile, tile_i * tile_j == 64 (0, 0 = 64 x 1) */
+    int kchunk;           /* layers per reconstruction workgroup: 8, 16, 32 (0 = 16) */
+    int pad_;
+    double slope_max;     /* opt-in slope taper (redi_set_slope_taper), 0 = off */
+} redi_domain_t;
+
+/* group: tracer slots of the batch's flux kernels (2..4); every domain has at
+ * most that many tracers.  persist: workgroups of each persistent kernel per
+ * CU per occupancy slot (1 = exactly fill the GPU once; 0 = one workgroup per
+ * work block, no persistence). */
+int redi_batch_create(redi_batch_t **out, int form, int group, int max_domains,
+                      long long arena_bytes, int persist);
+int redi_batch_destroy(redi_batch_t *b);
+
+/* Arena bytes one domain of this size takes in a batch of this form / group. */
+long long redi_batch_domain_bytes(int nx_total, int ny_total, int nz, int form, int group);
+
+/* Admit a domain: returns its slot (>= 0) or a negative error; enqueues the
+ * initial fill of its Phase-A state on `stream`. */
+int redi_batch_add(redi_batch_t *b, const redi_domain_t *d, void *stream);
+/* Retire a slot; its arena range is reused by later adds (in stream order). */
+int redi_batch_remove(redi_batch_t *b, int slot);
+
+/* nsteps Redi steps of every domain in the batch, one graph launch per step. */
+int redi_batch_step(redi_batch_t *b, int nsteps, void *stream);
+/* The same steps as one launch sequence per domain (the unbatched schedule,
+ * for comparison); same bits. */
+int redi_batch_step_serial(redi_batch_t *b, int nsteps, void *stream);
+
+/* Phase-A state of one slot in the reference layout (as redi_export_coeffs). */
+int redi_batch_export_coeffs(const redi_batch_t *b, int slot, const redi_coeffs_t *dst,
+                             void *stream);
+
+/* Domains in the batch, arena bytes in use and free (largest free range). */
+int redi_batch_info(const redi_batch_t *b, int *ndomains, long long *used_bytes,
+                    long long *largest_free_bytes);
 
 #ifdef __cplusplus
 }

@@ -40,5 +40,42 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+FBUILD = os.path.join(FORTRAN, "_build")
+DROPIN_SRC = os.path.join(ROOT, "tests", "fortran", "redi_dropin_driver.F90")
+DROPIN_BIN = os.path.join(ROOT, "tests", "fortran", "_build", "redi_dropin_hip")
+FC = os.environ.get("FC", "/opt/rocm/llvm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran drop-in modules (fortran/*.F90) and the reference-style
+    test driver linked against them and libredi_hip.so; returns the driver
+    path, or "" where no Fortran compiler is installed."""
+    if not os.path.exists(FC):
+        return ""
+    os.makedirs(FBUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(DROPIN_BIN), exist_ok=True)
+    flags = ["-O2", "-module-dir", FBUILD]
+    objs = []
+    for src in ("redi_hip_c.F90", "ocean_redi_hip.F90"):
+        obj = os.path.join(FBUILD, src.replace(".F90", ".o"))
+        cmd = [FC, *flags, "-c", os.path.join(FORTRAN, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, capture_output=not verbose)
+        objs.append(obj)
+    drv = os.path.join(FBUILD, "redi_dropin_driver.o")
+    subprocess.run([FC, *flags, "-c", DROPIN_SRC, "-o", drv], check=True, capture_output=not verbose)
+    rocm_lib = os.environ.get("ROCM_LIB", "/opt/rocm/lib")
+    cmd = [FC, "-o", DROPIN_BIN, *objs, drv, "-L" + PKG, "-lredi_hip", "-L" + rocm_lib,
+           "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../../more_benchmarks_dc_ocean_amd",
+           "-Wl,-rpath," + rocm_lib]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, capture_output=not verbose)
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

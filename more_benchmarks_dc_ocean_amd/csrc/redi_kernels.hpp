@@ -283,7 +283,27 @@ struct SweepArgs {
   FaceRecF<F>* out;        // [s][face]
   uint8_t* mono;           // [face]
   double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
+  const double* inv_dx;    // 1 / face spacing per face (idxCu / idyCv): the slope taper's
+  double slope_max;        // opt-in slope taper: 0 = off (the reference has none)
 };
+
+// Opt-in neutral-slope taper (no counterpart in the reference): sublayer
+// between surfaces with absolute positions (pressures) top / bottom on the
+// left and right columns has slope S = (zR - zL) / dx, z the mid-depth
+// (mean of its bounding positions, Pa -> m); where |S| > slope_max its hEff
+// is scaled by (slope_max / |S|)^2, the quadratic taper of Gerdes, Koberle
+// and Willebrand (1991).  Applied to hEff in m before it is stored.
+__device__ __forceinline__ double slope_taper(double hm, double topL, double botL, double topR,
+                                              double botR, double pa_to_h, double inv_dx,
+                                              double smax) {
+  const double dz = ((0.5 * (topR + botR)) - (0.5 * (topL + botL))) * pa_to_h;
+  const double aS = fabs(dz * inv_dx);
+  if (aS > smax) {
+    const double q = smax / aS;
+    hm = hm * (q * q);
+  }
+  return hm;
+}
 
 // Density differences, positions, pressures and thicknesses are carried in
 // FP64 in both forms; positions and thicknesses are stored in the form's
@@ -403,7 +423,10 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
       const double hR = posR - lastPos_r;
       double he = 0.0;
       if (hL + hR > 0.0) he = ((2.0 * hL) * hR) / (hL + hR);
-      pend.hEff = Face::stored_heff(he * a.pa_to_h);
+      double hm = he * a.pa_to_h;
+      if (a.slope_max > 0.0 && hm != 0.0)
+        hm = slope_taper(hm, lastPos_l, posL, lastPos_r, posR, a.pa_to_h, a.inv_dx[fo], a.slope_max);
+      pend.hEff = Face::stored_heff(hm);
       if (pend.hEff != 0.0) {
         mono_l = mono_l && pend.KoL >= contK_l;
         mono_r = mono_r && pend.KoR >= contK_r;

@@ -218,11 +218,19 @@ TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
 }
 
 // Phase A over both face grids (redi_calc_coeffs_x / _y).
+// Opt-in slope taper of Phase A: 1 / face spacing of u and v faces and the
+// slope limit (0 = off).
+struct Taper {
+  const double* inv_u = nullptr;
+  const double* inv_v = nullptr;
+  double slope_max = 0.0;
+};
+
 template <class F>
 int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename F::R* h,
                    const typename F::R* t, const typename F::R* s, const double* wet_u,
                    const double* wet_v, const FaceState& fs, void* ws, const Tiles& tiles,
-                   hipStream_t st, Timer& tm) {
+                   hipStream_t st, Timer& tm, const Taper& tp = Taper{}) {
   const int N2 = nx * ny;
   ColRec* rec = (ColRec*)ws;
   double* P = (double*)(rec + (size_t)N2 * (nz + 1));
@@ -233,7 +241,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
     SweepArgs<F> a{nx, nz, nx + 1, 1, (uint32_t)fs.nup, 1, 0, 1, 0, wet_u, rec, P,
-                   (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h};
+                   (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
@@ -242,7 +250,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
     SweepArgs<F> a{nx, nz, nx, 1, (uint32_t)fs.nvp, 0, 1, 0, 1, wet_v, rec, P,
-                   (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h};
+                   (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
@@ -312,22 +320,17 @@ void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
   tm.end(slot, st);
 }
 
+// Face-flux arguments of both face grids and the cell-flux arguments of one
+// domain's Phase B over the scratch `sc` (tracer pointers and the
+// reconstruction layout are set per tracer group).
 template <class F>
-int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::R* const* tracers,
-                   const FaceState& fs, void* ws, const Tiles& tiles, hipStream_t st, Timer& tm,
-                   double* const* tend = nullptr) {
+void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc, FaceFluxArgs<F>& fu,
+                  FaceFluxArgs<F>& fv, CellFluxArgs<F>& ca) {
   using R = typename F::R;
-  const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny, ns = 2 * nz + 2;
+  const int nx = p.nx, ny = p.ny, nz = p.nz, ns = 2 * nz + 2;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
-  int slot = tm.begin(REDI_K_FACE_KHTR, st);
-  k_face_khtr<<<blocks_for((size_t)nu + nv, 256), 256, 0, st>>>(
-      KhtrArgs{nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v});
-  tm.end(slot, st);
-  const int slots = recon_slots(nt);
-  const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
   ReconRec* recon = (ReconRec*)sc.recon;
-
-  FaceFluxArgs<F> fu{};  // u faces i = 2..nx (1-based), every row
+  fu = FaceFluxArgs<F>{};  // u faces i = 2..nx (1-based), every row
   fu.nfa = nx + 1; fu.ilo = 1; fu.ihi = nx; fu.jlo = 0; fu.jhi = ny;
   fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
   fu.wall0 = p.ww ? p.nghost : -1;
@@ -336,7 +339,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu;
   fu.rec = (const FaceRecF<F>*)fs.u; fu.recon = recon;
   fu.tgt = sc.tgt_u; fu.flx = (R*)sc.flx_u;
-  FaceFluxArgs<F> fv{};  // v faces j = 2..ny (1-based), every column
+  fv = FaceFluxArgs<F>{};  // v faces j = 2..ny (1-based), every column
   fv.nfa = nx; fv.ilo = 0; fv.ihi = nx; fv.jlo = 1; fv.jhi = ny;
   fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
   fv.wall0 = p.ws ? p.nghost : -1;
@@ -346,7 +349,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   fv.rec = (const FaceRecF<F>*)fs.v; fv.recon = recon;
   fv.tgt = sc.tgt_v; fv.flx = (R*)sc.flx_v;
 
-  CellFluxArgs<F> ca{};
+  ca = CellFluxArgs<F>{};
   ca.nx = nx; ca.ny = ny; ca.nz = nz;
   ca.wuf_w = p.nghost + 1; ca.wuf_e = p.nghost + p.nxp + 1;
   ca.wvf_s = p.nghost + 1; ca.wvf_n = p.nghost + p.nyp + 1;
@@ -355,9 +358,41 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v;
   ca.flx_u = (const R*)sc.flx_u; ca.flx_v = (const R*)sc.flx_v;
   ca.nu = (uint32_t)fs.nup; ca.nv = (uint32_t)fs.nvp;
+}
 
-  int kcap = 0;  // smallest accumulator depth that holds nz layers
+// Smallest accumulator depth class of k_cell_flux that holds nz layers.
+int cell_kcap(int nz) {
+  int kcap = 0;
   while (kCellKCap[kcap] < nz) ++kcap;
+  return kcap;
+}
+
+// Reconstruction layout of a group of k tracers.
+ReconLayout recon_layout(int layer_major, int N2, int nz, int k) {
+  const uint32_t k32 = (uint32_t)k;
+  return layer_major ? ReconLayout{k32, (uint32_t)N2 * k32, 1}
+                     : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
+}
+
+template <class F>
+int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::R* const* tracers,
+                   const FaceState& fs, void* ws, const Tiles& tiles, hipStream_t st, Timer& tm,
+                   double* const* tend = nullptr) {
+  using R = typename F::R;
+  const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny;
+  const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
+  int slot = tm.begin(REDI_K_FACE_KHTR, st);
+  k_face_khtr<<<blocks_for((size_t)nu + nv, 256), 256, 0, st>>>(
+      KhtrArgs{nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, fs.khtr_u, fs.khtr_v});
+  tm.end(slot, st);
+  const int slots = recon_slots(nt);
+  const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
+  ReconRec* recon = (ReconRec*)sc.recon;
+  FaceFluxArgs<F> fu, fv;
+  CellFluxArgs<F> ca;
+  phase_b_args<F>(p, fs, sc, fu, fv, ca);
+
+  const int kcap = cell_kcap(nz);
   const unsigned nbu = nx > 1 ? (unsigned)(fs.nup / kFaceGroup) : 0;
   const unsigned nbv = ny > 1 ? (unsigned)(fs.nvp / kFaceGroup) : 0;
   const int G = std::max(1, std::min(tiles.group, slots));
@@ -380,12 +415,10 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
       ca.htr[1] = (R*)sc.discard;
       ca.tend[1] = nullptr;
     }
-    const uint32_t k32 = (uint32_t)kn;
-    const ReconLayout ry = tiles.recon_layer_major
-                               ? ReconLayout{k32, (uint32_t)N2 * k32, 1}
-                               : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
+    const ReconLayout ry = recon_layout(tiles.recon_layer_major, N2, nz, kn);
     fu.ry = fv.ry = ry;
-    const unsigned nbc = (unsigned)ny * (unsigned)((nx + 64 / kn - 1) / (64 / kn));
+    const unsigned nbc = kn == 2 ? cell_flux_blocks<2>(nx, ny)
+                                 : (kn == 3 ? cell_flux_blocks<3>(nx, ny) : cell_flux_blocks<4>(nx, ny));
     switch (kn) {
       case 2: launch_group<F, 2>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
       case 3: launch_group<F, 3>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
@@ -484,6 +517,7 @@ struct redi_s {
   Tiles tiles;
   Timer timer;
   std::vector<double*> tend;  // optional tendency outputs, one per tracer
+  double slope_max = 0.0;     // opt-in slope taper (0 = off)
   bool is_init = false;
 };
 
@@ -513,9 +547,14 @@ int calc_coeffs_impl(const char* who, const redi_hgrid_t* grid, const redi_metri
     return fail(REDI_ERR_ARG, "%s: tracer index out of range", who);
   if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "%s: unknown EOS variant %d", who, eos->variant);
   if (!m->wet_u || !m->wet_v) return fail(REDI_ERR_ARG, "%s: null wet mask", who);
+  Taper tp;
+  if (r->slope_max > 0.0) {
+    if (!m->idxCu || !m->idyCv) return fail(REDI_ERR_ARG, "%s: the slope taper needs idxCu and idyCv", who);
+    tp = Taper{m->idxCu, m->idyCv, r->slope_max};
+  }
   return launch_phase_a<F>(r->nx, r->ny, r->nz, to_eos(*eos), ms.h, ms.tracers[ms.idx_t - 1],
                            ms.tracers[ms.idx_s - 1], m->wet_u, m->wet_v, r->fs, r->ws, r->tiles,
-                           (hipStream_t)stream, r->timer);
+                           (hipStream_t)stream, r->timer, tp);
 }
 
 template <class F>
@@ -676,6 +715,13 @@ int redi_set_tendency_output(redi_t* r, int ntracers, double* const* tend) {
   if (ntracers < 0 || (ntracers > 0 && !tend))
     return fail(REDI_ERR_ARG, "redi_set_tendency_output: bad arguments");
   r->tend.assign(tend, tend + ntracers);
+  return REDI_OK;
+}
+
+int redi_set_slope_taper(redi_t* r, double slope_max) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_slope_taper: null handle");
+  if (!(slope_max >= 0.0)) return fail(REDI_ERR_ARG, "redi_set_slope_taper: slope limit %g < 0", slope_max);
+  r->slope_max = slope_max;
   return REDI_OK;
 }
 
@@ -906,3 +952,6 @@ void redi_cuda_launch_(const int* nx, const int* ny, const int* nz, const int* n
 }
 
 }  // extern "C"
+
+// The batched form (redi_batch_*).
+#include "redi_batch_host.hpp"

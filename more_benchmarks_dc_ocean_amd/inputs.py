@@ -27,7 +27,7 @@ import dataclasses
 import math
 import os
 import struct
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -431,3 +431,21 @@ def crop(c: RediCase, i0: int, j0: int, nxp: int, nyp: int) -> RediCase:
         khtr_u_ext=c.khtr_u_ext[slu].copy(), dx_cv=c.dx_cv[slv].copy(),
         idyCv=c.idyCv[slv].copy(), wet_v=wet_v, khtr_v_ext=c.khtr_v_ext[slv].copy(),
         areaT=c.areaT[sl].copy(), name=c.name + f"_crop{i0}_{j0}_{nxp}x{nyp}")
+
+
+def batch_sizes(n: int = 96, seed: int = 0, lo: int = 48, hi: int = 512,
+                nzs: Sequence[int] = (25, 50, 75), cfg: int = 3) -> List[tuple]:
+    """(side, nz) of the domains of a batched workload: square interior side
+    log-uniform in [lo, hi] (area log-uniform in [lo^2, hi^2]), nz uniform
+    over ``nzs``."""
+    rng = _rng(cfg, 100_000 + seed, 0)
+    sides = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(int)
+    nz = rng.choice(np.asarray(nzs), n)
+    return [(int(s), int(k)) for s, k in zip(sides, nz)]
+
+
+def case_batch(n: int = 96, seed: int = 0, nsteps: int = 1, nt: int = 4) -> List[RediCase]:
+    """The batched workload: ``n`` independent domains of uneven size, each with
+    its own seeded bathymetry and T/S (case_domain physics)."""
+    return [case_domain(3, seed * 1000 + i, s, s, nz, nt=nt, nsteps=nsteps)
+            for i, (s, nz) in enumerate(batch_sizes(n, seed))]

@@ -77,7 +77,11 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
-            "redi_set_tendency_output", "redi_set_recon_chunk")
+            "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
+            # the batched form (bound in batch.py)
+            "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
+            "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
+            "redi_batch_export_coeffs", "redi_batch_info")
 FORMS = {"fp64": 0, "fp32_store": 1}  # REDI_FORM_*
 NKERNELS = 7  # REDI_NKERNELS
 
@@ -107,6 +111,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_tiles.argtypes = [_P, _I, _I]
         L.redi_set_recon_order.argtypes = [_P, _I]
         L.redi_set_recon_chunk.argtypes = [_P, _I]
+        L.redi_set_slope_taper.argtypes = [_P, _D]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
@@ -195,6 +200,16 @@ class DeviceDomain:
         self._trp = (_P * nt)(*(self.hTr[t].data_ptr() for t in range(nt)))
         self.state = State(c.nz, c.idx_T, c.idx_S, nt, _P(self.h.data_ptr()),
                            ctypes.cast(self._trp, ctypes.POINTER(_P)))
+
+    def clone_tracers(self) -> "DeviceDomain":
+        """A domain that shares this one's h, metrics and face diffusivities
+        (a Redi step only reads them) and owns a device copy of its tracers
+        (the state a step updates)."""
+        import copy
+        d = copy.copy(self)
+        d.hTr = self.hTr.clone()
+        d._refresh_state()
+        return d
 
     def set_tracers(self, hTr) -> None:
         """Overwrite the tracers in place (host array or tensor; converted to
@@ -328,11 +343,20 @@ class OceanRedi:
         self._tend_ptrs = (_P * nt)(*(tend[t].data_ptr() for t in range(nt)))
         _check(lib().redi_set_tendency_output(self.handle, nt,
                                               ctypes.cast(self._tend_ptrs, ctypes.POINTER(_P))),
-               "redi_set_tendency_output", "redi_set_recon_chunk")
+               "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
+            # the batched form (bound in batch.py)
+            "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
+            "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
+            "redi_batch_export_coeffs", "redi_batch_info")
 
     def set_recon_order(self, layer_major: bool) -> None:
         _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
                "redi_set_recon_order")
+
+    def set_slope_taper(self, slope_max: float) -> None:
+        """Opt-in neutral-slope taper of later calc_coeffs (0 = off, the
+        reference's behaviour)."""
+        _check(lib().redi_set_slope_taper(self.handle, float(slope_max)), "redi_set_slope_taper")
 
     def set_recon_chunk(self, kt: int) -> None:
         """Layers per tiled-reconstruction workgroup (8, 16 or 32)."""

@@ -609,10 +609,62 @@ void oracle_build_column(int nz, const double *h_col, const double *thtr_col,
     free(htd);
 }
 
+/* Opt-in neutral-slope taper -- NOT part of the reference, which tapers
+ * nothing; the plain restatement never calls it (slope_max = 0).  Sublayer s
+ * lies between surfaces s and s+1; its mid-depth in each column is the mean
+ * of the absolute positions (redi_absolute_position, ocean_redi.F90:289-298)
+ * of its bounding surfaces, converted Pa -> m; slope S = (zR - zL) * inv_dx.
+ * Where |S| > slope_max, hEff (m) is scaled by (slope_max/|S|)^2, the
+ * quadratic taper of Gerdes, Koberle and Willebrand (1991). */
+static void slope_taper(int nz, const double *Pl, const double *Pr,
+                        const double *PoL, const double *PoR, const int *KoL,
+                        const int *KoR, double pa_to_h, double inv_dx,
+                        double slope_max, double *hEff) {
+    const int ns = 2 * nz + 2;
+    for (int s = 0; s < ns - 1; ++s) {
+        const double hm = hEff[s];
+        if (hm == 0.0) continue;
+        const double topL = abs_pos(Pl, KoL[s], PoL[s]), botL = abs_pos(Pl, KoL[s + 1], PoL[s + 1]);
+        const double topR = abs_pos(Pr, KoR[s], PoR[s]), botR = abs_pos(Pr, KoR[s + 1], PoR[s + 1]);
+        const double dz = ((0.5 * (topR + botR)) - (0.5 * (topL + botL))) * pa_to_h;
+        const double aS = fabs(dz * inv_dx);
+        if (aS > slope_max) {
+            const double q = slope_max / aS;
+            hEff[s] = hm * (q * q);
+        }
+    }
+}
+
+static void face_coeffs(int nz, const double *hL, const double *tL,
+                        const double *sL, const double *hR, const double *tR,
+                        const double *sR, const oracle_eos_t *eos,
+                        double inv_dx, double slope_max,
+                        double *PoL, double *PoR, int *KoL, int *KoR,
+                        double *hEff);
+
 /* redi_face_coeffs, ocean_redi.F90:656-692 (hEff converted Pa -> m). */
 void oracle_face_coeffs(int nz, const double *hL, const double *tL,
                         const double *sL, const double *hR, const double *tR,
                         const double *sR, const oracle_eos_t *eos,
+                        double *PoL, double *PoR, int *KoL, int *KoR,
+                        double *hEff) {
+    face_coeffs(nz, hL, tL, sL, hR, tR, sR, eos, 0.0, 0.0, PoL, PoR, KoL, KoR, hEff);
+}
+
+/* redi_face_coeffs followed by the opt-in slope taper (slope_max > 0). */
+void oracle_face_coeffs_taper(int nz, const double *hL, const double *tL,
+                              const double *sL, const double *hR, const double *tR,
+                              const double *sR, const oracle_eos_t *eos,
+                              double inv_dx, double slope_max,
+                              double *PoL, double *PoR, int *KoL, int *KoR,
+                              double *hEff) {
+    face_coeffs(nz, hL, tL, sL, hR, tR, sR, eos, inv_dx, slope_max, PoL, PoR, KoL, KoR, hEff);
+}
+
+static void face_coeffs(int nz, const double *hL, const double *tL,
+                        const double *sL, const double *hR, const double *tR,
+                        const double *sR, const oracle_eos_t *eos,
+                        double inv_dx, double slope_max,
                         double *PoL, double *PoR, int *KoL, int *KoR,
                         double *hEff) {
     const int n1 = nz + 1, ns = 2 * nz + 2;
@@ -625,12 +677,15 @@ void oracle_face_coeffs(int nz, const double *hL, const double *tL,
                                         dTr, dSr, PoL, PoR, KoL, KoR, hEff);
     const double pa_to_h = 1.0 / (GRAVITY * eos->rho0);
     for (int s = 0; s < ns - 1; ++s) hEff[s] = hEff[s] * pa_to_h;
+    if (slope_max > 0.0)
+        slope_taper(nz, Pl, Pr, PoL, PoR, KoL, KoR, pa_to_h, inv_dx, slope_max, hEff);
     free(w);
 }
 
 static void calc_faces(int nx, int ny, int nz, const oracle_eos_t *eos,
                        const double *h, const double *t, const double *sa,
                        const double *wet, int nfa, int nfb, int is_u,
+                       const double *inv_dx, double slope_max,
                        double *PoL, double *PoR, int *KoL, int *KoR,
                        double *hEff) {
     const int ns = 2 * nz + 2;
@@ -665,8 +720,8 @@ static void calc_faces(int nx, int ny, int nz, const oracle_eos_t *eos,
                 hL[k] = h[cl + k * cstride]; tL[k] = t[cl + k * cstride]; sL[k] = sa[cl + k * cstride];
                 hR[k] = h[cr + k * cstride]; tR[k] = t[cr + k * cstride]; sR[k] = sa[cr + k * cstride];
             }
-            oracle_face_coeffs(nz, hL, tL, sL, hR, tR, sR, eos, po, po + ns,
-                               ko, ko + ns, po + 2 * ns);
+            face_coeffs(nz, hL, tL, sL, hR, tR, sR, eos, slope_max > 0.0 ? inv_dx[fo] : 0.0,
+                        slope_max, po, po + ns, ko, ko + ns, po + 2 * ns);
             for (int s = 0; s < ns; ++s) {
                 PoL[fo + s * fstride] = po[s];
                 PoR[fo + s * fstride] = po[ns + s];
@@ -686,10 +741,25 @@ void oracle_calc_coeffs(int nx, int ny, int nz, const oracle_eos_t *eos,
                         const double *wet_v, double *uPoL, double *uPoR,
                         int *uKoL, int *uKoR, double *uhEff, double *vPoL,
                         double *vPoR, int *vKoL, int *vKoR, double *vhEff) {
+    oracle_calc_coeffs_taper(nx, ny, nz, eos, h_layer, t_htr, s_htr, wet_u, wet_v,
+                             NULL, NULL, 0.0, uPoL, uPoR, uKoL, uKoR, uhEff,
+                             vPoL, vPoR, vKoL, vKoR, vhEff);
+}
+
+/* oracle_calc_coeffs with the opt-in slope taper (idxCu / idyCv give the
+ * face spacings; slope_max = 0 is the plain restatement). */
+void oracle_calc_coeffs_taper(int nx, int ny, int nz, const oracle_eos_t *eos,
+                              const double *h_layer, const double *t_htr,
+                              const double *s_htr, const double *wet_u,
+                              const double *wet_v, const double *idxCu,
+                              const double *idyCv, double slope_max,
+                              double *uPoL, double *uPoR, int *uKoL, int *uKoR,
+                              double *uhEff, double *vPoL, double *vPoR, int *vKoL,
+                              int *vKoR, double *vhEff) {
     calc_faces(nx, ny, nz, eos, h_layer, t_htr, s_htr, wet_u, nx + 1, ny, 1,
-               uPoL, uPoR, uKoL, uKoR, uhEff);
+               idxCu, slope_max, uPoL, uPoR, uKoL, uKoR, uhEff);
     calc_faces(nx, ny, nz, eos, h_layer, t_htr, s_htr, wet_v, nx, ny + 1, 0,
-               vPoL, vPoR, vKoL, vKoR, vhEff);
+               idyCv, slope_max, vPoL, vPoR, vKoL, vKoR, vhEff);
 }
 
 /* ---------------------------------------------------------------------- */

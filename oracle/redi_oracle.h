@@ -84,6 +84,25 @@ void oracle_calc_coeffs(int nx, int ny, int nz, const oracle_eos_t *eos,
                         int *uKoL, int *uKoR, double *uhEff, double *vPoL,
                         double *vPoR, int *vKoL, int *vKoR, double *vhEff);
 
+/* Opt-in neutral-slope taper -- not part of the reference (it tapers
+ * nothing): redi_face_coeffs / redi_calc_coeffs followed by scaling hEff by
+ * (slope_max/|S|)^2 where the sublayer's neutral slope |S| > slope_max;
+ * slope_max = 0 is the plain restatement. */
+void oracle_face_coeffs_taper(int nz, const double *hL, const double *tL,
+                              const double *sL, const double *hR, const double *tR,
+                              const double *sR, const oracle_eos_t *eos,
+                              double inv_dx, double slope_max,
+                              double *PoL, double *PoR, int *KoL, int *KoR,
+                              double *hEff);
+void oracle_calc_coeffs_taper(int nx, int ny, int nz, const oracle_eos_t *eos,
+                              const double *h_layer, const double *t_htr,
+                              const double *s_htr, const double *wet_u,
+                              const double *wet_v, const double *idxCu,
+                              const double *idyCv, double slope_max,
+                              double *uPoL, double *uPoR, int *uKoL, int *uKoR,
+                              double *uhEff, double *vPoL, double *vPoR, int *vKoL,
+                              int *vKoR, double *vhEff);
+
 /* redi_apply_flux_impl for ONE tracer: hTr += dTr/areaT over all cells,
  * reading the neighbour columns from hTr_in (the snapshot). */
 void oracle_apply_flux_tracer(

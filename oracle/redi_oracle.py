@@ -73,6 +73,10 @@ def lib() -> ctypes.CDLL:
         _lib.oracle_sublayer_dT.restype = D
         _lib.oracle_sublayer_dT.argtypes = [I, I, I, I, D, D, D, D] + [P] * 8
         _lib.oracle_calc_coeffs.argtypes = [I, I, I, ctypes.POINTER(OracleEos)] + [P] * 15
+        _lib.oracle_calc_coeffs_taper.argtypes = ([I, I, I, ctypes.POINTER(OracleEos)] + [P] * 7
+                                                  + [D] + [P] * 10)
+        _lib.oracle_face_coeffs_taper.argtypes = ([I, P, P, P, P, P, P, ctypes.POINTER(OracleEos),
+                                                   D, D] + [P] * 5)
         _lib.oracle_apply_flux_tracer.argtypes = [I, I, I, D, I, I, I, I, I, I, I] + [P] * 20
         _lib.oracle_set_threads.restype = I
         _lib.oracle_set_threads.argtypes = [I]
@@ -101,8 +105,10 @@ class OracleRedi:
     ``ocean_redi_init`` (Po = hEff = 0, Ko = 1; redi/ocean_redi.F90:493-505).
     """
 
-    def __init__(self, case):
+    def __init__(self, case, slope_max: float = 0.0):
         self.case = case
+        # opt-in slope taper, not part of the reference (0 = the plain restatement)
+        self.slope_max = float(slope_max)
         nx, ny, ns = case.nx, case.ny, case.ns
         self.uPoL = np.zeros((ns, ny, nx + 1)); self.uPoR = np.zeros((ns, ny, nx + 1))
         self.uKoL = np.ones((ns, ny, nx + 1), np.int32); self.uKoR = np.ones((ns, ny, nx + 1), np.int32)
@@ -122,6 +128,14 @@ class OracleRedi:
         if not c.enable or c.idx_T <= 0 or c.idx_S <= 0:
             return
         e = eos_of(c)
+        if self.slope_max > 0.0:
+            lib().oracle_calc_coeffs_taper(
+                c.nx, c.ny, c.nz, ctypes.byref(e), ptr(h), ptr(hTr[c.idx_T - 1]),
+                ptr(hTr[c.idx_S - 1]), ptr(c.wet_u), ptr(c.wet_v), ptr(c.idxCu), ptr(c.idyCv),
+                self.slope_max,
+                ptr(self.uPoL), ptr(self.uPoR), ptr(self.uKoL), ptr(self.uKoR), ptr(self.uhEff),
+                ptr(self.vPoL), ptr(self.vPoR), ptr(self.vKoL), ptr(self.vKoR), ptr(self.vhEff))
+            return
         lib().oracle_calc_coeffs(
             c.nx, c.ny, c.nz, ctypes.byref(e), ptr(h), ptr(hTr[c.idx_T - 1]),
             ptr(hTr[c.idx_S - 1]), ptr(c.wet_u), ptr(c.wet_v),
@@ -156,10 +170,10 @@ class OracleRedi:
         self.apply_flux(h, hTr)
 
 
-def run_oracle(case, nsteps: int | None = None) -> dict:
+def run_oracle(case, nsteps: int | None = None, slope_max: float = 0.0) -> dict:
     """Run the C restatement from a cold start; outputs mirror read_output."""
     n = case.nsteps if nsteps is None else nsteps
-    o = OracleRedi(case)
+    o = OracleRedi(case, slope_max)
     h = np.ascontiguousarray(case.h)
     hTr = np.ascontiguousarray(case.hTr.copy())
     out = {}
@@ -181,9 +195,12 @@ def build_reference() -> bool:
     reference tree; returns False where it is absent)."""
     if not os.path.isdir("/root/reference/redi"):
         return have_reference()
-    subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True,
+    subprocess.run(["make", "-s", "-C", HERE, "ref", "dropin-ref"], check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     return have_reference()
+
+
+DROPIN_REF_BIN = os.path.join(HERE, "_ref", "redi_dropin_ref")
 
 
 def run_reference(case, nsteps: int | None = None, mode: int = 0,

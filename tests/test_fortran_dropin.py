@@ -1,0 +1,68 @@
+"""The Fortran drop-in: one caller source, two implementations.
+
+tests/fortran/redi_dropin_driver.F90 uses only the reference's public
+Fortran surface (the modules constants, grid, ocean_metrics,
+multilayer_cgrid_state, ocean_boundary_types, ocean_eos, ocean_redi and the
+calls of redi/drivers/dc_main.F90:124-136).  It is compiled against the
+reference's own modules (oracle/_ref/redi_dropin_ref, CPU) and against the
+HIP-backed modules of more_benchmarks_dc_ocean_amd/fortran
+(tests/fortran/_build/redi_dropin_hip, GPU).  Both must write the oracle's
+bytes: the first pins the driver, the second is the drop-in.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import OUTS, ROOT
+
+import redi_oracle
+from more_benchmarks_dc_ocean_amd import inputs
+
+HIP_BIN = os.path.join(ROOT, "tests", "fortran", "_build", "redi_dropin_hip")
+
+
+def _case(eos: int, use_ext: int, walls=(1, 1, 1, 1)):
+    import dataclasses
+    c = inputs.case_domain(2, 31 + eos, 22, 17, 25, nt=3, nsteps=3)
+    return dataclasses.replace(c, eos_variant=eos, use_ext=use_ext, walls=walls)
+
+
+def _run(binary, case, tmp_path, hoist=False):
+    src, dst = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    inputs.write_input(src, case)
+    cmd = [binary, src, dst] + (["hoist"] if hoist else [])
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return inputs.read_output(dst)
+
+
+def _same_as_oracle(out, case):
+    ref = redi_oracle.run_oracle(case, nsteps=case.nsteps)
+    for k in OUTS:
+        assert np.array_equal(out[k], ref[k]), f"{k} differs from the oracle"
+
+
+CASES = [(1, 1, (1, 1, 1, 1)), (4, 0, (1, 0, 1, 1)), (2, 1, (0, 1, 1, 0))]
+
+
+@pytest.mark.parametrize("eos,use_ext,walls", CASES)
+def test_driver_against_reference_modules(tmp_path, eos, use_ext, walls):
+    if not os.path.exists(redi_oracle.DROPIN_REF_BIN):
+        pytest.skip("the reference build (oracle/_ref) is absent here")
+    c = _case(eos, use_ext, walls)
+    _same_as_oracle(_run(redi_oracle.DROPIN_REF_BIN, c, tmp_path), c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("eos,use_ext,walls", CASES)
+@pytest.mark.parametrize("hoist", [False, True])
+def test_driver_against_hip_dropin_modules(tmp_path, eos, use_ext, walls, hoist, gpu_device):
+    assert os.path.exists(HIP_BIN), f"{HIP_BIN} is not built (__graft_entry__.build())"
+    print(f"[gpu] fortran drop-in eos={eos} use_ext={use_ext} walls={walls} hoist={hoist}",
+          flush=True)
+    c = _case(eos, use_ext, walls)
+    _same_as_oracle(_run(HIP_BIN, c, tmp_path, hoist), c)
