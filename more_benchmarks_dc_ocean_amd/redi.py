@@ -343,11 +343,7 @@ class OceanRedi:
         self._tend_ptrs = (_P * nt)(*(tend[t].data_ptr() for t in range(nt)))
         _check(lib().redi_set_tendency_output(self.handle, nt,
                                               ctypes.cast(self._tend_ptrs, ctypes.POINTER(_P))),
-               "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
-            # the batched form (bound in batch.py)
-            "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
-            "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
-            "redi_batch_export_coeffs", "redi_batch_info")
+               "redi_set_tendency_output")
 
     def set_recon_order(self, layer_major: bool) -> None:
         _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
