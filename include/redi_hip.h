@@ -174,6 +174,11 @@ int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
  * parity result with the reference is for the default. */
 int redi_set_slope_taper(redi_t *self, double slope_max);
 
+/* Face-flux record reads staged through LDS per chunk of surfaces (1) or
+ * gathered lane by lane (0, default).  Applies to the per-column
+ * reconstruction order.  Changes only the schedule, never the result. */
+int redi_set_face_staging(redi_t *self, int on);
+
 /* Layers per workgroup of the per-column tiled reconstruction (8, 16 or 32,
  * default 16; the workgroup takes 256 / kt columns).  Changes only the
  * schedule, never the result. */

@@ -275,7 +275,8 @@ struct BatchOps {
     b->grid[kNodeSweep] = persistent_grid(b, kb_neutral_sweep<F>, kBatchSweepLanes, kNodeSweep);
     b->grid[kNodeKhtr] = persistent_grid(b, kb_face_khtr, 256, kNodeKhtr);
     b->grid[kNodeRecon] = persistent_grid(b, kb_tracer_recon<F, NT>, kReconLanes, kNodeRecon);
-    b->grid[kNodeFace] = persistent_grid(b, kb_face_flux<F, NT>, 64 * kFaceGroup, kNodeFace);
+    b->grid[kNodeFace] = persistent_grid(b, kb_face_flux<F, NT, kFaceStageDefault>, 64 * kFaceGroup,
+                                         kNodeFace);
     b->grid[kNodeCell0 + 0] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[0]>, 64, kNodeCell0 + 0);
     b->grid[kNodeCell0 + 1] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[1]>, 64, kNodeCell0 + 1);
     b->grid[kNodeCell0 + 2] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[2]>, 64, kNodeCell0 + 2);
@@ -295,7 +296,8 @@ struct BatchOps {
     if (live(kNodeRecon))
       kb_tracer_recon<F, NT><<<g[kNodeRecon], kReconLanes, 0, st>>>(view<Recon>(b, kNodeRecon));
     if (live(kNodeFace))
-      kb_face_flux<F, NT><<<g[kNodeFace], 64 * kFaceGroup, 0, st>>>(view<FaceFluxArgs<F>>(b, kNodeFace));
+      kb_face_flux<F, NT, kFaceStageDefault><<<g[kNodeFace], 64 * kFaceGroup, 0, st>>>(
+          view<FaceFluxArgs<F>>(b, kNodeFace));
     using CA = CellFluxArgs<F>;
     if (live(kNodeCell0 + 0))
       kb_cell_flux<F, NT, kCellKCap[0]><<<g[kNodeCell0 + 0], 64, 0, st>>>(view<CA>(b, kNodeCell0 + 0));
@@ -543,6 +545,7 @@ int redi_batch_add(redi_batch_t* b, const redi_domain_t* d, void* stream) {
   x.tiles.recon_kt = d->kchunk ? d->kchunk : kReconChunkDefault;
   x.tiles.group = b->group;
   x.tiles.recon_layer_major = 0;
+  x.tiles.face_stage = kFaceStageDefault;
   char* base = b->arena_base + off;
   x.off = off; x.bytes = L.total;
   x.fs.nu = (size_t)(nx + 1) * ny; x.fs.nv = (size_t)nx * (ny + 1);

@@ -697,9 +697,39 @@ struct FaceFluxArgs {
 // LDS and written back as whole rows (kFaceGroup faces x NT tracers).
 // sx: kFaceGroup * 64 * NT fluxes (stored type), stg: kFaceGroup * 64
 // packed targets, both in LDS.
-template <class F, int NT>
+// Wave-wide min / max over the 64 lanes of a wavefront.
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int m = 32; m; m >>= 1) {
+    const int o = __shfl_xor(v, m, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int m = 32; m; m >>= 1) {
+    const int o = __shfl_xor(v, m, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Staged record reads (kStage, per-column reconstruction order only): a
+// chunk's 64 surfaces lie in few layers of each column -- about half as many
+// as surfaces -- so instead of every lane gathering its own two records per
+// tracer, the wavefront loads the layer range [lo, hi] each side touches
+// with consecutive lanes (one coalesced pass per side and tracer) into 64
+// LDS records per side, and each lane reads its records there.  A chunk
+// whose range on either side spans more than 64 layers takes the gather.
+// The records read are the same, so are the bits.
+constexpr int kStageRecs = 128;  // LDS records per wavefront (64 per side)
+constexpr bool kFaceStageDefault = false;  // staged reads unless a caller chooses
+
+template <class F, int NT, bool kStage = false>
 __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
-                                               typename F::R* sx, uint32_t* stg) {
+                                               typename F::R* sx, uint32_t* stg,
+                                               ReconRec* stage_lds = nullptr) {
   using R = typename F::R;
   const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const size_t f0 = (size_t)xcd_band(bid, nb) * kFaceGroup;
@@ -730,11 +760,32 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
     const double PoL = Fs.PoL, PoR = Fs.PoR;
     double eL[NT], eR[NT];
     LayerVals qL[NT], qR[NT];
+    bool staged = false;
+    int lo_l = 0, hi_l = 0, lo_r = 0, hi_r = 0;
+    if constexpr (kStage) {  // wave-uniform layer ranges of this chunk's surfaces
+      lo_l = wave_min(surf ? KoL : 1 << 30);
+      hi_l = wave_max(surf ? KoL : 0);
+      lo_r = wave_min(surf ? KoR : 1 << 30);
+      hi_r = wave_max(surf ? KoR : 0);
+      staged = hi_l >= lo_l && hi_l - lo_l < 64 && hi_r - lo_r < 64;
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       qL[t] = LayerVals{0.0, 0.0, 0.0, 0.0, 0.0};
       qR[t] = qL[t];
-      if (surf) {
+      if (kStage && staged) {
+        ReconRec* sl = stage_lds + wv * kStageRecs;
+        if (lane <= hi_l - lo_l) sl[lane] = recL[(uint32_t)(lo_l - 1 + lane) + (uint32_t)t * ct];
+        if (lane <= hi_r - lo_r) sl[64 + lane] = recR[(uint32_t)(lo_r - 1 + lane) + (uint32_t)t * ct];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (surf) {
+          qL[t] = sl[KoL - lo_l].get();
+          qR[t] = sl[64 + KoR - lo_r].get();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else if (surf) {
         qL[t] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
         qR[t] = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
       }
@@ -787,6 +838,15 @@ __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
   face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
+}
+
+// The same with staged record reads (per-column reconstruction order).
+template <class F, int NT>
+__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux_staged(FaceFluxArgs<F> a) {
+  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ uint32_t stg[kFaceGroup * 64];
+  __shared__ ReconRec stage[kFaceGroup * kStageRecs];
+  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, stage);
 }
 
 // ---------------------------------------------------------------------------

@@ -207,6 +207,7 @@ struct Tiles {
   int recon_layer_major = 0; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
   int recon_kt = kReconChunkDefault;  // layers per tiled-reconstruction workgroup
+  int face_stage = kFaceStageDefault;  // face fluxes read their records through LDS (per-column order)
 };
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
@@ -312,7 +313,10 @@ void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
     const unsigned nb = grid ? nbv : nbu;
     if (!nb) continue;
     slot = tm.begin(REDI_K_FACE_FLUX, st);
-    k_face_flux<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
+    if (tiles.face_stage && !tiles.recon_layer_major)
+      k_face_flux_staged<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
+    else
+      k_face_flux<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
     tm.end(slot, st);
   }
   slot = tm.begin(REDI_K_CELL_FLUX, st);
@@ -730,6 +734,12 @@ int redi_set_recon_chunk(redi_t* r, int kt) {
   if (kt != 8 && kt != 16 && kt != 32)
     return fail(REDI_ERR_ARG, "redi_set_recon_chunk: layer chunk %d is not 8, 16 or 32", kt);
   r->tiles.recon_kt = kt;
+  return REDI_OK;
+}
+
+int redi_set_face_staging(redi_t* r, int on) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_face_staging: null handle");
+  r->tiles.face_stage = on ? 1 : 0;
   return REDI_OK;
 }
 

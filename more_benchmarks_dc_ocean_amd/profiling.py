@@ -27,11 +27,22 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 
 
+BATCHED = "batched_96"  # the batched workload: 96 uneven domains (inputs.case_batch)
+
+
+def _cases(workload: str, steps: int):
+    from more_benchmarks_dc_ocean_amd import inputs
+    if workload == BATCHED:
+        return inputs.case_batch(96, seed=0, nsteps=steps)
+    builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
+               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
+    return [builder(nsteps=steps)]
+
+
 def run(workload: str, steps: int, warmup: int, copy_mib: int, form: str = "fp64") -> None:
     import torch
-    from more_benchmarks_dc_ocean_amd import inputs
     from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
-    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain, make_redi
     dev = "cuda:0"
     n = copy_mib * (1 << 20) // 8
     a = torch.ones(n, dtype=torch.float64, device=dev)
@@ -40,12 +51,26 @@ def run(workload: str, steps: int, warmup: int, copy_mib: int, form: str = "fp64
     b.copy_(a)
     torch.cuda.synchronize()
     del a, b
-    builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
-               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
-    case = builder(nsteps=steps)
-    print(f"[profiling] {workload} {case.nx}x{case.ny}x{case.nz} nt={case.nt}", flush=True)
-    r, dom = make_redi(case, dev)
-    TuneCache(TUNED).apply(r, case.nx, case.ny, case.nz)
+    cases = _cases(workload, steps)
+    cache = TuneCache(TUNED)
+    if workload == BATCHED:
+        from more_benchmarks_dc_ocean_amd.batch import RediBatch, domain_bytes
+        arena = sum(domain_bytes(c.nx, c.ny, c.nz, form, 4) for c in cases)
+        bt = RediBatch(form=form, group=4, max_domains=len(cases), arena_bytes=arena, device=dev)
+        doms = []
+        for c in cases:
+            doms.append(DeviceDomain(c, dev, form))
+            bt.add(doms[-1], cache.batch_shape(c.nx, c.ny, c.nz, form))
+        print(f"[profiling] {workload} {len(cases)} domains, form {form}", flush=True)
+        bt.step(warmup + steps)
+        torch.cuda.synchronize()
+        bt.close()
+        print("[profiling] done", flush=True)
+        return
+    case = cases[0]
+    print(f"[profiling] {workload} {case.nx}x{case.ny}x{case.nz} nt={case.nt} form {form}", flush=True)
+    r, dom = make_redi(case, dev, form)
+    cache.apply(r, case.nx, case.ny, case.nz)
     for _ in range(warmup + steps):
         r.step(dom, 1)
     torch.cuda.synchronize()
@@ -71,23 +96,42 @@ def _short(name: str) -> str:
 
 
 def _base(name: str) -> str:
-    """The byte model's name for a kernel (template arguments and the
-    reconstruction's tiled variant folded in)."""
+    """The byte model's name for a kernel (template arguments, the
+    reconstruction's tiled variant and the batched kernels' kb_ prefix folded
+    in)."""
     b = _short(name).split("<")[0]
+    if b.startswith("kb_"):
+        b = "k_" + b[3:]
     return "k_tracer_recon" if b.startswith("k_tracer_recon") else b
 
 
-def summarize_dir(d: str, workload: str, copy_mib: int = 2048) -> dict:
+def summarize_dir(d: str, workload: str, copy_mib: int = 2048, form: str = "fp64") -> dict:
     """Per-kernel figures from one directory of runs: kt/ (kernel trace
     --stats), fetch/ and write/ (FETCH_SIZE, WRITE_SIZE) and any other
     --pmc passes (SQ_*, TA_*, TD_*, TCC_* counters), all per dispatch."""
-    from . import byte_model, inputs
+    from . import byte_model
     from .autotune import TUNED, TuneCache
-    builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
-               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
-    case = builder(nsteps=1)
-    group = TuneCache(TUNED).lookup(case.nx, case.ny, case.nz).get("group", 4)
-    alg = {k: sum(v) / len(v) for k, v in byte_model.step_launches(case, group).items() if v}
+    cases = _cases(workload, 1)
+    alg: dict = {}
+    cell_by_cap: dict = {}  # batched: the cell-flux bytes per accumulator depth class
+    for case in cases:
+        if workload == BATCHED:
+            cap = next(c for c in (32, 76, 128, 256) if c >= case.nz)
+            cell_by_cap[cap] = cell_by_cap.get(cap, 0.0) + sum(
+                byte_model.step_launches(case, 4, form)["k_cell_flux"])
+        group = 4 if workload == BATCHED else \
+            TuneCache(TUNED).lookup(case.nx, case.ny, case.nz).get("group", 4)
+        for k, v in byte_model.step_launches(case, group, form).items():
+            if not v:
+                continue
+            if workload == BATCHED:  # one launch per kind covers every domain (u and v together)
+                kk = "k_neutral_sweep" if k.startswith("k_neutral_sweep") else k
+                alg[kk] = alg.get(kk, 0.0) + sum(v)
+            else:
+                alg[k] = sum(v) / len(v)
+    if workload == BATCHED:
+        alg["k_neutral_sweep_u"] = alg.pop("k_neutral_sweep", 0.0)
+        alg["k_face_flux"] = alg.get("k_face_flux", 0.0)
     stats = {}
     for row in _rows(os.path.join(d, "kt", "*kernel_stats.csv")):
         stats[_short(row["Name"])] = (int(row["Calls"]), float(row["AverageNs"]) * 1e-6)
@@ -100,14 +144,17 @@ def summarize_dir(d: str, workload: str, copy_mib: int = 2048) -> dict:
     cal = per.get(CALIBRATION_KERNEL, {})
     fscale = copy_mib * 1024.0 / cal["FETCH_SIZE"] if cal.get("FETCH_SIZE") else None
     wscale = copy_mib * 1024.0 / cal["WRITE_SIZE"] if cal.get("WRITE_SIZE") else None
-    out = {"workload": workload, "fetch_scale": fscale, "write_scale": wscale, "kernels": {}}
+    out = {"workload": workload, "form": form, "fetch_scale": fscale, "write_scale": wscale,
+           "kernels": {}}
     for k, (calls, ms) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
-        if not k.startswith("k_"):
+        if not k.startswith(("k_", "kb_")):
             continue
         c = per.get(k, {})
         cyc = ms * 1e-3 * GPU_CLOCK_HZ
         e = {"calls": calls, "avg_ms": ms}
         a = alg.get(_base(k) if _base(k) != "k_neutral_sweep" else "k_neutral_sweep_u")
+        if workload == BATCHED and k.startswith("kb_cell_flux"):
+            a = cell_by_cap.get(int(k.rstrip(">").split(",")[-1]))
         if a is not None:
             e["alg_bytes"] = a
             e["alg_GBps"] = a / (ms * 1e-3) / 1e9
@@ -194,19 +241,27 @@ def main() -> int:
     ap.add_argument("--summarize-dir", metavar="DIR",
                     help="directory with kt/, fetch/, write/ and other --pmc pass outputs")
     ap.add_argument("--json", metavar="FILE", help="with --summarize-dir: also write the figures")
+    ap.add_argument("--markdown", metavar="FILE", help="with --summarize-dir: also write the table")
+    ap.add_argument("--title", default="")
+    ap.add_argument("--commands", default="", help="file with the commands that made the runs")
+    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32_store"])
     args = ap.parse_args()
     if args.summarize_dir:
-        summ = summarize_dir(args.summarize_dir, args.workload, args.copy_mib)
+        summ = summarize_dir(args.summarize_dir, args.workload, args.copy_mib, args.form)
         if args.json:
             with open(args.json, "w") as fh:
                 json.dump(summ, fh, indent=1)
+        if args.markdown:
+            cmds = open(args.commands).read().strip().splitlines() if args.commands else []
+            with open(args.markdown, "w") as fh:
+                fh.write(markdown(summ, args.title or f"{args.workload} {args.form}", cmds))
         print(json.dumps(summ, indent=1))
         return 0
     if args.summarize:
         print(json.dumps(summarize_counters(args.summarize[0], args.summarize[1], args.copy_mib),
                          indent=1))
         return 0
-    run(args.workload, args.steps, args.warmup, args.copy_mib)
+    run(args.workload, args.steps, args.warmup, args.copy_mib, args.form)
     return 0
 
 

@@ -78,6 +78,7 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
             "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
+            "redi_set_face_staging",
             # the batched form (bound in batch.py)
             "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
             "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
@@ -112,6 +113,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_recon_order.argtypes = [_P, _I]
         L.redi_set_recon_chunk.argtypes = [_P, _I]
         L.redi_set_slope_taper.argtypes = [_P, _D]
+        L.redi_set_face_staging.argtypes = [_P, _I]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
@@ -353,6 +355,10 @@ class OceanRedi:
         """Opt-in neutral-slope taper of later calc_coeffs (0 = off, the
         reference's behaviour)."""
         _check(lib().redi_set_slope_taper(self.handle, float(slope_max)), "redi_set_slope_taper")
+
+    def set_face_staging(self, on: bool) -> None:
+        """Face-flux record reads through LDS (schedule only, same bits)."""
+        _check(lib().redi_set_face_staging(self.handle, int(bool(on))), "redi_set_face_staging")
 
     def set_recon_chunk(self, kt: int) -> None:
         """Layers per tiled-reconstruction workgroup (8, 16 or 32)."""

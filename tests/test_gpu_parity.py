@@ -173,3 +173,41 @@ def test_hip_deterministic(gpu_device):
     b = _run_hip(case, 2, gpu_device)
     for k in OUTS:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("name", ["roquet_two_fronts_mini", "wright_vanished_mini", "nz1_mini"])
+def test_hip_face_staging_same_bits(name, gpu_device):
+    """Face-flux records read through LDS: the oracle's bits in the FP64 form,
+    the gathered path's bits in the FP32-store form."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    case, ref = load_golden(name)
+    print(f"[gpu] face staging {name}", flush=True)
+    outs = {}
+    for form in ("fp64", "fp32_store"):
+        for staged in (False, True):
+            r, dom = make_redi(case, gpu_device, form)
+            r.set_face_staging(staged)
+            r.step(dom, case.nsteps)
+            torch.cuda.synchronize()
+            outs[(form, staged)] = dom.tracers_host()
+            r.close()
+    assert np.array_equal(outs[("fp64", True)], ref["hTrN"])
+    assert np.array_equal(outs[("fp64", False)], ref["hTrN"])
+    assert np.array_equal(outs[("fp32_store", True)], outs[("fp32_store", False)])
+
+
+def test_hip_face_staging_front_473x297x75(gpu_device):
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    case = inputs.case_front(nsteps=1)
+    print("[gpu] face staging 473x297x75", flush=True)
+    got = []
+    for staged in (False, True):
+        r, dom = make_redi(case, gpu_device, "fp64")
+        r.set_face_staging(staged)
+        r.step(dom, 1)
+        torch.cuda.synchronize()
+        got.append(dom.tracers_host())
+        r.close()
+    assert np.array_equal(got[0], got[1])
