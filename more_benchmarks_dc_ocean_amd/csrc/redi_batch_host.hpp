@@ -214,7 +214,7 @@ struct BatchOps {
       // Phase A (launch_phase_a)
       ColRec* rec = (ColRec*)d.ws;
       double* P = (double*)(rec + (size_t)N2 * (nz + 1));
-      w[kNodeColumn].add(ColumnArgs<F>{N2, nz, h, tr[d.idx_t - 1], tr[d.idx_s - 1], d.eos, rec, P},
+      w[kNodeColumn].add(ColumnArgs<F>{N2, nz, nx, h, tr[d.idx_t - 1], tr[d.idx_s - 1], d.eos, rec, P},
                          blocks_for(N2, 128));
       const double pa_to_h = 1.0 / (kGravity * d.eos.rho0);
       if (nx > 1) {

@@ -31,9 +31,10 @@
 //
 // Device memory layout (R = the form's storage type):
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
-//   column state (K1 -> K2)     FP64 in both forms, per column contiguous:
-//                               ColRec[col][kt], P[col][kt], kt = top-down
-//                               interface 0..nz
+//   column state (K1 -> K2)     FP64 in both forms, interface-major within
+//                               each row: ColRec / P of column (i, j) at
+//                               interface kt (top-down, 0..nz) at
+//                               (j*(nz+1) + kt)*nx + i
 //   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
 //                               surface-major: (face, s) at s*Nf' + face with
 //                               rows padded to Nf' = a multiple of kFaceGroup
@@ -44,10 +45,11 @@
 //                               tracer's column contiguous by default
 //   sublayer fluxes (K4 -> K5)  surface-major: packed targets [s][face] and
 //                               fluxes [s][face][tracer], rows of Nf' faces
-// The sweep reads each of its two columns once, front to back, while lanes
-// sit at unrelated depths, so the column state is stored per column and
-// each lane streams whole 128-byte lines of its own columns; its lanes are
-// adjacent faces, so its surface-major writes are contiguous.  The face flux
+// The sweep reads each of its two columns once, front to back; its lanes
+// are adjacent faces, which sit at similar depths, so with the column state
+// interface-major per row the lanes read neighbouring records (and the
+// column-state kernel, a lane per column, writes whole rows); its
+// surface-major face-record writes are contiguous.  The face flux
 // kernel gives each face one wavefront with a lane per neutral surface: a
 // face's targets are non-decreasing in s, so the lanes walk down the two
 // columns together and read consecutive per-column records.
@@ -182,9 +184,19 @@ struct TileMap {
 // fields): the sweep's density differences are differences of these
 // derivatives' products, and FP32 derivatives move neutral surfaces into
 // other layers.
+// Column state layout: interface-major within each row of columns, the
+// record of interface kt (top-down) of column (i, j) at (j*(nz+1) + kt)*nx + i.
+// Neighbouring faces sit at similar depths, so the sweep's lanes (adjacent
+// faces) read neighbouring records, and the column-state kernel's lanes
+// (adjacent columns) write whole rows.
+__device__ __forceinline__ size_t colstate_base(size_t c, int nx, int nz) {
+  const size_t j = c / (size_t)nx, i = c % (size_t)nx;
+  return j * (size_t)(nz + 1) * (size_t)nx + i;
+}
+
 template <class F>
 struct ColumnArgs {
-  int N2, nz;
+  int N2, nz, nx;
   const typename F::R* h;
   const typename F::R* thtr;
   const typename F::R* shtr;
@@ -205,8 +217,9 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
   const typename F::R* __restrict__ shtr = a.shtr;
   const EosParams eos = a.eos;
   const double g_rho0 = kGravity * eos.rho0;
-  ColRec* r = a.rec + (size_t)c * (nz + 1);
-  double* pc = a.P + (size_t)c * (nz + 1);
+  const size_t cb = colstate_base((size_t)c, a.nx, nz), cs = (size_t)a.nx;
+  ColRec* r = a.rec + cb;
+  double* pc = a.P + cb;
   double P = 0.0;  // Pint(1) = 0
   auto load = [&](int kt, double& hh, double* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
@@ -219,8 +232,8 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
     if (k > 0) P = P + g_rho0 * (double)h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
     double dRdT, dRdS;
     interface_drho<double>(eos, e[0], e[1], P, dRdT, dRdS);
-    r[k] = ColRec{e[0], e[1], dRdT, dRdS};
-    pc[k] = P;
+    r[(size_t)k * cs] = ColRec{e[0], e[1], dRdT, dRdS};
+    pc[(size_t)k * cs] = P;
   };
   stream_edges<double, 2>(nz, load, emit);
 }
@@ -249,13 +262,14 @@ __device__ __forceinline__ double sec_drho(const ColRec& a, const ColRec& b) {
 struct SweepSide {
   const ColRec* rec;
   const double* P;
+  size_t st;  // elements between consecutive interfaces of the column
   int k, m;
   ColRec w0, w1;
   double p0, p1, p2;
-  __device__ __forceinline__ void init(const ColRec* r, const double* p, int nk) {
-    rec = r; P = p; k = 1; m = 1;
-    w0 = r[0]; w1 = r[1];
-    p0 = p[0]; p1 = p[1]; p2 = p[nk < 2 ? nk : 2];
+  __device__ __forceinline__ void init(const ColRec* r, const double* p, size_t stride, int nk) {
+    rec = r; P = p; st = stride; k = 1; m = 1;
+    w0 = r[0]; w1 = r[stride];
+    p0 = p[0]; p1 = p[stride]; p2 = p[(size_t)(nk < 2 ? nk : 2) * stride];
   }
   __device__ __forceinline__ ColRec cur() const { return k == 1 ? w0 : w1; }  // record at k
   __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }        // P(k)
@@ -263,8 +277,8 @@ struct SweepSide {
   __device__ __forceinline__ void advance(int nk) {  // k -> k+1, k <= nk
     if (k >= 2) {
       m += 1;
-      w0 = w1; w1 = rec[m];
-      p0 = p1; p1 = p2; p2 = P[m + 1 < nk ? m + 1 : nk];
+      w0 = w1; w1 = rec[(size_t)m * st];
+      p0 = p1; p1 = p2; p2 = P[(size_t)(m + 1 < nk ? m + 1 : nk) * st];
     }
     k += 1;
   }
@@ -327,8 +341,9 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
   const size_t cr = (size_t)i + (size_t)a.nx * j;
   SweepSide L, Rt;
-  L.init(a.rec + cl * (nk + 1), a.P + cl * (nk + 1), nk);
-  Rt.init(a.rec + cr * (nk + 1), a.P + cr * (nk + 1), nk);
+  const size_t bl = colstate_base(cl, a.nx, nk), br = colstate_base(cr, a.nx, nk);
+  L.init(a.rec + bl, a.P + bl, (size_t)a.nx, nk);
+  Rt.init(a.rec + br, a.P + br, (size_t)a.nx, nk);
 
   int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
   double lastP_l = 0.0, lastP_r = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;

@@ -236,7 +236,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   ColRec* rec = (ColRec*)ws;
   double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
-  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, h, t, s, eos, rec, P});
+  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, nx, h, t, s, eos, rec, P});
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
