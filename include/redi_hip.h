@@ -174,6 +174,13 @@ int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
  * parity result with the reference is for the default. */
 int redi_set_slope_taper(redi_t *self, double slope_max);
 
+/* Order in which the sweep's face tiles and the cell-flux row segments are
+ * numbered (and so handed to the 8 XCDs in contiguous bands): along the
+ * rows (0, default) or down the rows first (1), which puts vertical
+ * neighbours -- sharing v faces and columns -- on one L2 one after the
+ * other.  Changes only the schedule, never the result. */
+int redi_set_tile_order(redi_t *self, int vertical);
+
 /* Face-flux record reads staged through LDS per chunk of surfaces (1) or
  * gathered lane by lane (0, default).  Applies to the per-column
  * reconstruction order.  Changes only the schedule, never the result. */

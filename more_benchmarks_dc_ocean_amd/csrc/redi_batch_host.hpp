@@ -242,6 +242,7 @@ struct BatchOps {
       FaceFluxArgs<F> fu, fv;
       CellFluxArgs<F> ca;
       phase_b_args<F>(p, d.fs, sc, fu, fv, ca);
+      ca.vertical = d.tiles.vertical;
       TracerSetT<R> ts{};
       for (int t = 0; t < kMaxNT; ++t) {
         // slots past the domain's tracers repeat its last tracer; their

@@ -31,18 +31,52 @@ struct BatchView {
   const Item* __restrict__ items;
 };
 
+// The work tables do not change while a kernel runs (they are copied in
+// before the launch, in stream order), so they are read through the constant
+// address space: every lane needs the same entry, and scalar loads bring it
+// into scalar registers once per virtual block instead of a vector load per
+// field.
+template <class T>
+__device__ __forceinline__ T ld_table(const T* p) {
+  using CT = const __attribute__((address_space(4))) T;
+  return *(CT*)p;  // address-space cast (reinterpret_cast may not change it)
+}
+
+// A whole table item, word by word into scalar registers (readfirstlane
+// marks every word uniform, so the body's uses never reload it).
+template <class Item>
+__device__ __forceinline__ Item ld_item(const Item* p) {
+  static_assert(sizeof(Item) % 4 == 0, "table items are whole words");
+  using CW = const __attribute__((address_space(4))) uint32_t;
+  CW* w = (CW*)p;
+  union {
+    Item it;
+    uint32_t u[sizeof(Item) / 4];
+  } x;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(Item) / 4); ++i) x.u[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  return x.it;
+}
+
 // Runs body(item, local block, item blocks) for every virtual block this
 // workgroup owns.  kBarrier: the body uses LDS across waves, so the next
-// block must wait until every wave is done with it.
-template <bool kBarrier, class Item, class Body>
+// block must wait until every wave is done with it.  kCopy: the item is
+// copied into scalar registers first (bodies whose item is read under
+// divergent branches); otherwise the body reads it in place.
+template <bool kBarrier, bool kCopy = true, class Item, class Body>
 __device__ __forceinline__ void for_each_block(const BatchView<Item>& v, Body&& body) {
-  const uint32_t total = v.count[1];
+  const uint32_t total = ld_table(v.count + 1);
   uint32_t e = 0;
   for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
-    while (v.start[e + 1] <= b) ++e;  // b only grows, so the search only moves forward
-    const uint32_t local = b - v.start[e], nb = v.nblk[e];
+    while (ld_table(v.start + e + 1) <= b) ++e;  // b only grows, so the search only moves forward
+    const uint32_t local = b - ld_table(v.start + e), nb = ld_table(v.nblk + e);
     if (local < nb) {
-      body(v.items[e], (int)local, (int)nb);
+      if constexpr (kCopy) {
+        const Item it = ld_item(v.items + e);
+        body(it, (int)local, (int)nb);
+      } else {
+        body(v.items[e], (int)local, (int)nb);
+      }
       if (kBarrier) __syncthreads();
     }
   }
@@ -93,7 +127,8 @@ __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconIt
 // Every batch item uses the per-column reconstruction order, so the face
 // fluxes may read their records through LDS (kStage).
 template <class F, int NT, bool kStage>
-__global__ void __launch_bounds__(64 * kFaceGroup) kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
+__global__ void __launch_bounds__(64 * kFaceGroup) __attribute__((amdgpu_waves_per_eu(4)))
+    kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
   __shared__ ReconRec stage[kStage ? kFaceGroup * kStageRecs : 1];
@@ -107,7 +142,7 @@ __global__ void __launch_bounds__(64 * kFaceGroup) kb_face_flux(BatchView<FaceFl
 template <class F, int NT, int KCAP>
 __global__ void __launch_bounds__(64) kb_cell_flux(BatchView<CellFluxArgs<F>> v) {
   __shared__ double acc[KCAP * 64];
-  for_each_block<false>(v, [&](const CellFluxArgs<F>& a, int bid, int nb) {
+  for_each_block<false, false>(v, [&](const CellFluxArgs<F>& a, int bid, int nb) {
     cell_flux_body<F, NT, KCAP>(a, bid, nb, acc);
   });
 }

@@ -163,14 +163,23 @@ __device__ __forceinline__ int xcd_band(int bid, int nb) {
   return x * q + (x < r ? x : r) + local;
 }
 
+// Tiles are numbered along the fast axis first, or (vertical) along the
+// slow axis first, so that an XCD's consecutive tiles are vertical
+// neighbours: the v faces and columns two rows share are then read by
+// consecutive tiles of the same L2.
+constexpr int kTileVerticalDefault = 0;
+
 struct TileMap {
   int n0, n1;    // index space extent (fast, slow)
   int tx, ty;    // tile shape (tx * ty == blockDim.x)
   int tiles0;    // tiles along the fast axis
+  int tiles1;    // tiles along the slow axis
+  int vertical;  // tile numbering: slow axis first
   // lanes of block `bid` of `nb` (tiles beyond the index space map outside it)
   __device__ __forceinline__ bool map(int bid, int nb, int& a, int& b) const {
     const int tile = xcd_band(bid, nb);
-    const int t0 = tile % tiles0, t1 = tile / tiles0;
+    const int t0 = vertical ? tile / tiles1 : tile % tiles0;
+    const int t1 = vertical ? tile % tiles1 : tile / tiles0;
     a = t0 * tx + (int)(threadIdx.x % (unsigned)tx);
     b = t1 * ty + (int)(threadIdx.x / (unsigned)tx);
     return a < n0 && b < n1;
@@ -876,6 +885,7 @@ struct CellFluxArgs {
   const uint32_t *tgt_u, *tgt_v;
   const typename F::R *flx_u, *flx_v;
   uint32_t nu, nv;                 // faces per sublayer row of the u / v arrays
+  int vertical;                    // segments numbered down the rows first
   typename F::R* htr[kMaxNT];
   double* tend[kMaxNT];            // optional: the FP64 tendency dTr / areaT per cell
 };
@@ -902,7 +912,8 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
   const int cw = lane / NT, t = lane % NT;
   const int segs = (a.nx + kCells - 1) / kCells;
   const int w = xcd_band(bid, nb);
-  const int cj = w / segs, ci = (w % segs) * kCells + cw;
+  const int cj = a.vertical ? w % a.ny : w / segs;
+  const int ci = (a.vertical ? w / a.ny : w % segs) * kCells + cw;
   const bool cell = cw < kCells && ci < a.nx && cj < a.ny;
   const int nz = a.nz, nsub = 2 * nz + 1;
   for (int K = 0; K < nz; ++K) acc[K * 64 + lane] = 0.0;

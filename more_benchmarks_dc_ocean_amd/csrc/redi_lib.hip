@@ -208,13 +208,16 @@ struct Tiles {
   int group = kTracerGroup;  // tracers per flux launch (1..4)
   int recon_kt = kReconChunkDefault;  // layers per tiled-reconstruction workgroup
   int face_stage = kFaceStageDefault;  // face fluxes read their records through LDS (per-column order)
+  int vertical = kTileVerticalDefault;  // sweep tiles and cell-flux segments numbered down rows first
 };
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
   TileMap m;
   m.n0 = n0; m.n1 = n1; m.tx = t.tx; m.ty = t.ty;
   m.tiles0 = (n0 + t.tx - 1) / t.tx;
-  *nblocks = (unsigned)m.tiles0 * (unsigned)((n1 + t.ty - 1) / t.ty);
+  m.tiles1 = (n1 + t.ty - 1) / t.ty;
+  m.vertical = t.vertical;
+  *nblocks = (unsigned)m.tiles0 * (unsigned)m.tiles1;
   return m;
 }
 
@@ -395,6 +398,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   FaceFluxArgs<F> fu, fv;
   CellFluxArgs<F> ca;
   phase_b_args<F>(p, fs, sc, fu, fv, ca);
+  ca.vertical = tiles.vertical;
 
   const int kcap = cell_kcap(nz);
   const unsigned nbu = nx > 1 ? (unsigned)(fs.nup / kFaceGroup) : 0;
@@ -734,6 +738,12 @@ int redi_set_recon_chunk(redi_t* r, int kt) {
   if (kt != 8 && kt != 16 && kt != 32)
     return fail(REDI_ERR_ARG, "redi_set_recon_chunk: layer chunk %d is not 8, 16 or 32", kt);
   r->tiles.recon_kt = kt;
+  return REDI_OK;
+}
+
+int redi_set_tile_order(redi_t* r, int vertical) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_tile_order: null handle");
+  r->tiles.vertical = vertical ? 1 : 0;
   return REDI_OK;
 }
 

@@ -78,7 +78,7 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
             "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
-            "redi_set_face_staging",
+            "redi_set_face_staging", "redi_set_tile_order",
             # the batched form (bound in batch.py)
             "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
             "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
@@ -114,6 +114,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_recon_chunk.argtypes = [_P, _I]
         L.redi_set_slope_taper.argtypes = [_P, _D]
         L.redi_set_face_staging.argtypes = [_P, _I]
+        L.redi_set_tile_order.argtypes = [_P, _I]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
@@ -355,6 +356,11 @@ class OceanRedi:
         """Opt-in neutral-slope taper of later calc_coeffs (0 = off, the
         reference's behaviour)."""
         _check(lib().redi_set_slope_taper(self.handle, float(slope_max)), "redi_set_slope_taper")
+
+    def set_tile_order(self, vertical: bool) -> None:
+        """Number sweep tiles / cell-flux segments down the rows first
+        (schedule only, same bits)."""
+        _check(lib().redi_set_tile_order(self.handle, int(bool(vertical))), "redi_set_tile_order")
 
     def set_face_staging(self, on: bool) -> None:
         """Face-flux record reads through LDS (schedule only, same bits)."""

@@ -211,3 +211,25 @@ def test_hip_face_staging_front_473x297x75(gpu_device):
         got.append(dom.tracers_host())
         r.close()
     assert np.array_equal(got[0], got[1])
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32_store"])
+def test_hip_tile_order_same_bits(form, gpu_device):
+    """Vertical numbering of sweep tiles and cell-flux segments: same bits
+    (the oracle's in the FP64 form)."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    case, ref = load_golden("roquet_two_fronts_mini")
+    print(f"[gpu] tile order {form}", flush=True)
+    got = []
+    for vertical in (False, True):
+        r, dom = make_redi(case, gpu_device, form)
+        r.set_tile_order(vertical)
+        r.set_tiles(16, 4)
+        r.step(dom, case.nsteps)
+        torch.cuda.synchronize()
+        got.append(dom.tracers_host())
+        r.close()
+    assert np.array_equal(got[0], got[1])
+    if form == "fp64":
+        assert np.array_equal(got[1], ref["hTrN"])
