@@ -12,7 +12,11 @@ and, per field, the max and RMS of the difference, absolute and relative to
 the reference field's max / RMS:
 
   tend[t]     tracer t's step-1 tendency dTr / areaT: C vs B (the form's own
-              error, same inputs) and C vs A (including the input rounding)
+              error, same inputs), C vs A (including the input rounding) and
+              B vs A (the FP64 form's own response to rounding its inputs:
+              the reference's down-gradient guard is a sign test, so a
+              rounding-sized change of T can switch a sublayer's flux on or
+              off)
   T, S        layer temperature / salinity hTr / h after N steps: C vs A
   hTr[t]      every tracer after N steps: C vs A
   PoL..vhEff  the step-1 Phase-A coefficients: C vs B; KoL/KoR as the
@@ -82,6 +86,7 @@ def report(case, nsteps: int, dev: str = "cuda:0", log=print) -> dict:
     for t, n in enumerate(names):
         F[f"tend_{n} (C vs B)"] = _err(C["tend"][t], B["tend"][t])
         F[f"tend_{n} (C vs A)"] = _err(C["tend"][t], A["tend"][t])
+        F[f"tend_{n} (B vs A)"] = _err(B["tend"][t], A["tend"][t])
         F[f"hTr_{n} after {nsteps} (C vs A)"] = _err(C["hTr"][t], A["hTr"][t])
     wet = A["h"] > 0
     for t, n in enumerate(names[:2]):
@@ -102,7 +107,9 @@ def to_markdown(res: dict, cmd: str) -> str:
              f"Command: `{cmd}`", "",
              "A = FP64 form on the FP64 inputs; B = FP64 form on the inputs rounded to FP32; "
              "C = FP32-store form on the rounded inputs.  Relative errors are to the reference "
-             "field's max (max_rel) and RMS (rms_rel).", "",
+             "field's max (max_rel) and RMS (rms_rel).  C vs B is the form's own error; B vs A "
+             "is how much the FP64 step itself moves when its inputs are rounded to FP32 (the "
+             "down-gradient guard is a sign test), which C vs A inherits.", "",
              "| field | max abs | RMS abs | max rel | RMS rel |", "|---|---|---|---|---|"]
     for k, e in res["fields"].items():
         if "mismatch_frac" in e:

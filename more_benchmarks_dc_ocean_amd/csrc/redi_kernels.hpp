@@ -690,6 +690,7 @@ struct FaceFluxArgs {
   int ilo, ihi, jlo, jhi; // faces computed: i in [ilo, ihi), j in [jlo, jhi) (0-based)
   int di, dj;             // left column = (i - di, j - dj), right column = (i, j)
   int nx, ns;             // columns per row, neutral surfaces
+  int nzc;                // layers per column (ns = 2*nzc + 2)
   int wall0, wall1;       // 0-based face coordinate across the faces (i for u,
                           // j for v) of the two wall faces, -1 where open
   uint32_t nf;            // faces in the face array
@@ -721,24 +722,6 @@ struct FaceFluxArgs {
 // LDS and written back as whole rows (kFaceGroup faces x NT tracers).
 // sx: kFaceGroup * 64 * NT fluxes (stored type), stg: kFaceGroup * 64
 // packed targets, both in LDS.
-// Wave-wide min / max over the 64 lanes of a wavefront.
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-  for (int m = 32; m; m >>= 1) {
-    const int o = __shfl_xor(v, m, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int m = 32; m; m >>= 1) {
-    const int o = __shfl_xor(v, m, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
 // Staged record reads (kStage, per-column reconstruction order only): a
 // chunk's 64 surfaces lie in few layers of each column -- about half as many
 // as surfaces -- so instead of every lane gathering its own two records per
@@ -786,12 +769,22 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
     LayerVals qL[NT], qR[NT];
     bool staged = false;
     int lo_l = 0, hi_l = 0, lo_r = 0, hi_r = 0;
-    if constexpr (kStage) {  // wave-uniform layer ranges of this chunk's surfaces
-      lo_l = wave_min(surf ? KoL : 1 << 30);
-      hi_l = wave_max(surf ? KoL : 0);
-      lo_r = wave_min(surf ? KoR : 1 << 30);
-      hi_r = wave_max(surf ? KoR : 0);
-      staged = hi_l >= lo_l && hi_l - lo_l < 64 && hi_r - lo_r < 64;
+    if constexpr (kStage) {
+      // wave-uniform layer ranges of this chunk's surfaces: positions Ko + Po
+      // never decrease along the surfaces (the sweep's backstop), so every
+      // surface's layer lies within one of the first and last surfaces';
+      // a ballot confirms it, else the chunk gathers
+      const int last = (live ? (ns - c0 < 64 ? ns - c0 : 64) : 1) - 1;
+      const int kl0 = __builtin_amdgcn_readlane(KoL, 0), kl1 = __builtin_amdgcn_readlane(KoL, last);
+      const int kr0 = __builtin_amdgcn_readlane(KoR, 0), kr1 = __builtin_amdgcn_readlane(KoR, last);
+      lo_l = kl0 > 1 ? kl0 - 1 : 1;
+      hi_l = kl1 + 1;
+      lo_r = kr0 > 1 ? kr0 - 1 : 1;
+      hi_r = kr1 + 1;
+      const bool out = surf && (KoL < lo_l || KoL > hi_l || KoR < lo_r || KoR > hi_r);
+      staged = live && hi_l - lo_l < 64 && hi_r - lo_r < 64 && __ballot(out) == 0;
+      if (hi_l > a.nzc) hi_l = a.nzc;  // stage no layer past the column
+      if (hi_r > a.nzc) hi_r = a.nzc;
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -801,13 +794,16 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
         ReconRec* sl = stage_lds + wv * kStageRecs;
         if (lane <= hi_l - lo_l) sl[lane] = recL[(uint32_t)(lo_l - 1 + lane) + (uint32_t)t * ct];
         if (lane <= hi_r - lo_r) sl[64 + lane] = recR[(uint32_t)(lo_r - 1 + lane) + (uint32_t)t * ct];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        // a wavefront's LDS accesses execute in issue order, so only the
+        // compiler must keep the reads after the writes (and the next
+        // tracer's writes after these reads)
         __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
         if (surf) {
           qL[t] = sl[KoL - lo_l].get();
           qR[t] = sl[64 + KoR - lo_r].get();
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        asm volatile("" ::: "memory");
         __builtin_amdgcn_wave_barrier();
       } else if (surf) {
         qL[t] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();

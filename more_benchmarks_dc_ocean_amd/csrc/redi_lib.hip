@@ -339,7 +339,7 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   ReconRec* recon = (ReconRec*)sc.recon;
   fu = FaceFluxArgs<F>{};  // u faces i = 2..nx (1-based), every row
   fu.nfa = nx + 1; fu.ilo = 1; fu.ihi = nx; fu.jlo = 0; fu.jhi = ny;
-  fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns;
+  fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns; fu.nzc = nz;
   fu.wall0 = p.ww ? p.nghost : -1;
   fu.wall1 = p.we ? p.nghost + p.nxp : -1;
   fu.nf = (uint32_t)nu; fu.fa = 1; fu.fb = (uint32_t)fs.nup; fu.nfp = (uint32_t)fs.nup;
@@ -348,7 +348,7 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   fu.tgt = sc.tgt_u; fu.flx = (R*)sc.flx_u;
   fv = FaceFluxArgs<F>{};  // v faces j = 2..ny (1-based), every column
   fv.nfa = nx; fv.ilo = 0; fv.ihi = nx; fv.jlo = 1; fv.jhi = ny;
-  fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns;
+  fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns; fv.nzc = nz;
   fv.wall0 = p.ws ? p.nghost : -1;
   fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
   fv.nf = (uint32_t)nv; fv.fa = 1; fv.fb = (uint32_t)fs.nvp; fv.nfp = (uint32_t)fs.nvp;
