@@ -40,6 +40,30 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+RK2_LIB = os.path.join(PKG, "librk2_redi_hip.so")
+
+
+def build_rk2(verbose: bool = False) -> str:
+    """The RK2 benchmark seam (csrc/rk2_redi_hip.cpp, host code over
+    libredi_hip.so) as a self-contained -Bsymbolic library, as the
+    reference's RK2 driver links each kernel."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(rocm, "include"),
+           "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "rk2_redi_hip.cpp"),
+           "-L" + PKG, "-lredi_hip", "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-Bsymbolic", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+           "-o", RK2_LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout + p.stderr)
+        raise RuntimeError("building librk2_redi_hip.so failed")
+    os.replace(RK2_LIB + ".tmp", RK2_LIB)
+    return RK2_LIB
+
+
 FORTRAN = os.path.join(PKG, "fortran")
 FBUILD = os.path.join(FORTRAN, "_build")
 DROPIN_SRC = os.path.join(ROOT, "tests", "fortran", "redi_dropin_driver.F90")
@@ -78,4 +102,5 @@ def build_fortran(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_rk2(verbose=True))
     print(build_fortran(verbose=True))

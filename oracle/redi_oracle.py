@@ -197,10 +197,16 @@ def build_reference() -> bool:
         return have_reference()
     subprocess.run(["make", "-s", "-C", HERE, "ref", "dropin-ref"], check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    # the reference's own driver on both module sets (needs the drop-in
+    # modules built first)
+    subprocess.run(["make", "-s", "-C", HERE, "dc-main"], check=False,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     return have_reference()
 
 
 DROPIN_REF_BIN = os.path.join(HERE, "_ref", "redi_dropin_ref")
+DC_MAIN_REF_BIN = os.path.join(HERE, "_ref", "redi_dc_main_ref")
+DC_MAIN_HIP_BIN = os.path.join(HERE, "_ref", "redi_dc_main_hip")
 
 
 def run_reference(case, nsteps: int | None = None, mode: int = 0,

@@ -66,3 +66,39 @@ def test_driver_against_hip_dropin_modules(tmp_path, eos, use_ext, walls, hoist,
           flush=True)
     c = _case(eos, use_ext, walls)
     _same_as_oracle(_run(HIP_BIN, c, tmp_path, hoist), c)
+
+
+# The reference's own driver, redi/drivers/dc_main.F90 (host lane), compiled
+# where it lies against both module sets by oracle/Makefile (target dc-main):
+# its DC_DUMP file (nx, ny, nz as int32, then the temperature tracer hTr in
+# FP64) must be the same bytes from the reference build and from the
+# HIP-backed drop-in.  Arguments: nx_phys ny_phys nz reps warm-up.
+DC_CASES = [("24", "18", "12", "3", "1"), ("40", "30", "75", "2", "1")]
+
+
+def _dc_main(binary, args, dump, tmp_path):
+    env = dict(os.environ, DC_DUMP=str(dump), OMP_NUM_THREADS="4")
+    p = subprocess.run([binary, *args], capture_output=True, text=True, timeout=300, env=env,
+                       cwd=str(tmp_path))
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "sanity           : OK" in p.stdout, p.stdout
+    return open(dump, "rb").read()
+
+
+def test_reference_dc_main_on_reference_modules(tmp_path):
+    if not os.path.exists(redi_oracle.DC_MAIN_REF_BIN):
+        pytest.skip("the reference build (oracle/_ref) is absent here")
+    raw = _dc_main(redi_oracle.DC_MAIN_REF_BIN, DC_CASES[0], tmp_path / "ref.bin", tmp_path)
+    nx, ny, nz = np.frombuffer(raw[:12], "<i4")
+    assert (nx, ny, nz) == (30, 24, 12) and len(raw) == 12 + 8 * nx * ny * nz
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", DC_CASES)
+def test_reference_dc_main_on_hip_dropin_same_bytes(tmp_path, args, gpu_device):
+    for b in (redi_oracle.DC_MAIN_REF_BIN, redi_oracle.DC_MAIN_HIP_BIN):
+        assert os.path.exists(b), f"{b} is not built (__graft_entry__.build())"
+    print(f"[gpu] reference dc_main {' '.join(args)} on both module sets", flush=True)
+    ref = _dc_main(redi_oracle.DC_MAIN_REF_BIN, args, tmp_path / "ref.bin", tmp_path)
+    hip = _dc_main(redi_oracle.DC_MAIN_HIP_BIN, args, tmp_path / "hip.bin", tmp_path)
+    assert ref == hip, "the HIP drop-in's dump differs from the reference build's"
