@@ -733,13 +733,23 @@ struct FaceFluxArgs {
 constexpr int kStageRecs = 128;  // LDS records per wavefront (64 per side)
 constexpr bool kFaceStageDefault = false;  // staged reads unless a caller chooses
 
+// Face records of a chunk in LDS (FP64 form): face q's 64 surfaces at
+// q*kRecRow, the row padded by one record so that the workgroup's
+// row-by-row stores spread over the banks.
+constexpr int kRecRow = 65;
+template <class F>
+constexpr int face_rec_lds() { return sizeof(FaceRecF<F>) > 16 ? kFaceGroup * kRecRow : 1; }
+
+// Face group `grp` (faces grp*kFaceGroup .. +kFaceGroup-1 of the face array).
 template <class F, int NT, bool kStage = false>
-__device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
-                                               typename F::R* sx, uint32_t* stg,
-                                               ReconRec* stage_lds = nullptr) {
+__device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int grp,
+                                                typename F::R* sx, uint32_t* stg,
+                                                FaceRecF<F>* srec,
+                                                ReconRec* stage_lds = nullptr) {
   using R = typename F::R;
+  using Face = FaceRecF<F>;
   const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const size_t f0 = (size_t)xcd_band(bid, nb) * kFaceGroup;
+  const size_t f0 = (size_t)grp * kFaceGroup;
   const size_t fo = f0 + wv;
   const int i = (int)(fo % (size_t)a.nfa), j = (int)(fo / (size_t)a.nfa);
   const int across = a.di ? i : j;
@@ -752,17 +762,45 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
     cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
     coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
   }
-  const FaceRecF<F>* r = a.rec + (live ? fo * a.fa : 0);
   const ReconRec* recL = a.recon + cl * a.ry.ca;
   const ReconRec* recR = a.recon + cr * a.ry.ca;
   const uint32_t cb = a.ry.cb, ct = a.ry.ct;
   const size_t fb = a.fb;
   const int ns = a.ns, nsub = ns - 1;
+  // 32-byte face records (FP64 form) come through LDS, one chunk ahead:
+  // thread (row rr, face q) loads surface c0 + rr of face f0 + q, so the
+  // kFaceGroup threads of a row read one contiguous run of the
+  // surface-major array, where a wave reading its own face lane by lane
+  // touches a separate line per surface and record half.  The 16-byte
+  // records of the FP32-store form are read lane by lane: measured, their
+  // staging costs more (one more wave-wide live record) than it saves.
+  // Records move as 16-byte words.
+  constexpr bool kRecLds = face_rec_lds<F>() > 1;
+  constexpr int kW = (int)(sizeof(Face) / sizeof(uint4));
+  const int rr_ld = (int)(threadIdx.x / kFaceGroup), q_ld = (int)(threadIdx.x % kFaceGroup);
+  const bool face_ld = kRecLds && f0 + q_ld < a.nf;
+  const uint4* rec_w = reinterpret_cast<const uint4*>(a.rec);
+  const size_t face_w = (f0 + q_ld) * a.fa;
+  uint4* srec_w = reinterpret_cast<uint4*>(srec);
+  uint4 pre[kW];
+#pragma unroll
+  for (int w = 0; w < kW; ++w) pre[w] = make_uint4(0u, 0u, 0u, 0u);
+  if (face_ld && rr_ld < ns) {
+#pragma unroll
+    for (int w = 0; w < kW; ++w) pre[w] = rec_w[(face_w + (size_t)rr_ld * fb) * kW + w];
+  }
+  const Face* r = a.rec + (live ? fo * a.fa : 0);
   for (int c0 = 0; c0 < nsub; c0 += kChunk) {
     const int s = c0 + lane;
     const bool surf = live && s < ns;
+    // every thread is past the previous chunk's reads of srec and sx
+    if constexpr (kRecLds) {
+#pragma unroll
+      for (int w = 0; w < kW; ++w) srec_w[(q_ld * kRecRow + rr_ld) * kW + w] = pre[w];
+    }
+    __syncthreads();
     SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
-    if (surf) Fs = r[(size_t)s * fb].get();
+    if (surf) Fs = kRecLds ? srec[wv * kRecRow + lane].get() : r[(size_t)s * fb].get();
     const int KoL = Fs.KoL, KoR = Fs.KoR;
     const double PoL = Fs.PoL, PoR = Fs.PoR;
     double eL[NT], eR[NT];
@@ -839,6 +877,12 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
 #pragma unroll
     for (int t = 0; t < NT; ++t) sx[(wv * 64 + lane) * NT + t] = x[t];
     stg[wv * 64 + lane] = packed;
+    // the next chunk's records, in flight over the barrier and the row writes
+    const int sn = c0 + kChunk + rr_ld;
+    if (face_ld && sn < ns) {
+#pragma unroll
+      for (int w = 0; w < kW; ++w) pre[w] = rec_w[(face_w + (size_t)sn * fb) * kW + w];
+    }
     __syncthreads();
     // row rr of the chunk, face q of the group
     const int rr = (int)(threadIdx.x / kFaceGroup), q = (int)(threadIdx.x % kFaceGroup);
@@ -849,15 +893,27 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
       for (int t = 0; t < NT; ++t) a.flx[o * NT + t] = sx[(q * 64 + rr) * NT + t];
       a.tgt[o] = stg[q * 64 + rr];
     }
-    __syncthreads();
+    // the next chunk's first barrier orders these reads of sx before its
+    // writes (a caller running several groups per workgroup puts a barrier
+    // between them)
   }
+}
+
+// Block bid of nb over one face grid: consecutive groups per XCD.
+template <class F, int NT, bool kStage = false>
+__device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
+                                               typename F::R* sx, uint32_t* stg,
+                                               FaceRecF<F>* srec,
+                                               ReconRec* stage_lds = nullptr) {
+  face_flux_group<F, NT, kStage>(a, xcd_band(bid, nb), sx, stg, srec, stage_lds);
 }
 
 template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
-  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
+  __shared__ FaceRecF<F> srec[face_rec_lds<F>()];
+  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, srec);
 }
 
 // The same with staged record reads (per-column reconstruction order).
@@ -865,8 +921,9 @@ template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux_staged(FaceFluxArgs<F> a) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
+  __shared__ FaceRecF<F> srec[face_rec_lds<F>()];
   __shared__ ReconRec stage[kFaceGroup * kStageRecs];
-  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, stage);
+  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, srec, stage);
 }
 
 // ---------------------------------------------------------------------------

@@ -312,11 +312,12 @@ void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
   int slot = tm.begin(REDI_K_TRACER_RECON, st);
   launch_recon<F, NT>(p, tiles, h, ts, recon, ry, st);
   tm.end(slot, st);
+  const bool staged = tiles.face_stage && !tiles.recon_layer_major;
   for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
     const unsigned nb = grid ? nbv : nbu;
     if (!nb) continue;
     slot = tm.begin(REDI_K_FACE_FLUX, st);
-    if (tiles.face_stage && !tiles.recon_layer_major)
+    if (staged)
       k_face_flux_staged<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
     else
       k_face_flux<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
