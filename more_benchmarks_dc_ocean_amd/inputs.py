@@ -14,7 +14,12 @@ The generators build the domains of the benchmark workloads:
 * ``case_front``  473x297x75, Roquet SpV EOS, 4 tracers, fractal bathymetry,
   one tanh front;
 * ``case_two_fronts``  1440x1080x75, as case_front with two fronts and mesoscale noise;
-* ``case_domain`` one batched/served domain of arbitrary size.
+* ``case_domain`` one batched/served domain of arbitrary size;
+* ``case_analytic``  the reference's own benchmark state: all wet, uniform
+  layers, analytic T/S, linear EOS, 2 tracers (473x297x30 by default);
+* ``case_gabight``  case_front's T/S, front and EOS over the real 0.1 degree
+  Great Australian Bight bathymetry the reference ships for its ePBL kernel,
+  resampled to any size.
 
 All randomness comes from ``numpy.random.default_rng`` seeded with
 ``(config, domain, field)`` so a case is reproducible bit-for-bit on any host
@@ -355,6 +360,110 @@ def case_two_fronts(nx_phys: int = 1440, ny_phys: int = 1080, nz: int = 75, nt: 
     return case_fractal(2, nx_phys, ny_phys, nz, nt, nsteps, seed, land=0.30,
                         fronts=[("y", 0.33, 1.0, 100_000.0), ("x", 0.67, 0.75, 100_000.0)],
                         noise_sigma=5.0, noise_T=0.3, noise_S=0.03, **kw)
+
+
+def case_analytic(nx_phys: int = 473, ny_phys: int = 297, nz: int = 30, nsteps: int = 30,
+                  nghost: int = 3) -> RediCase:
+    """The state the reference's own Redi benchmarks time
+    (legacy_testing/redi_benchmark/redi_bench.F90:60-130, 322-359; the same
+    state as ideal_benchmark/wrap/rk2_redi.F90:77-137, 204-235): every face
+    wet, walls on all four edges, uniform 8 km x 11.1 km metrics, uniform
+    layers over 4000 m, layer-centre T = 2 + 16 exp(-z/800) + 3 fx + 3 fy and
+    S = 34.5 + 0.7 (1 - exp(-z/1500)) - 0.2 fx with fx, fy the array
+    position in [0, 1] (ghosts included), the linear EOS at its defaults,
+    khtr = 100 m^2/s through the face arrays, dt = 900 s."""
+    nx, ny = nx_phys + 2 * nghost, ny_phys + 2 * nghost
+    hh = 4000.0 / nz
+    z = (np.arange(nz, dtype=np.float64) + 0.5) * hh          # native k = 0 at the bed:
+    z = z[::-1].copy()                                        # depth (nz - 1 - k + 0.5) hh
+    fx = np.arange(nx, dtype=np.float64) / (nx - 1)
+    fy = np.arange(ny, dtype=np.float64) / (ny - 1)
+    zz = z[:, None, None]
+    T = 2.0 + 16.0 * np.exp(-zz / 800.0) + 3.0 * fx[None, None, :] + 3.0 * fy[None, :, None]
+    S = (34.5 + 0.7 * (1.0 - np.exp(-zz / 1500.0))) - 0.2 * fx[None, None, :]
+    S = np.broadcast_to(S, (nz, ny, nx))
+    h = np.full((nz, ny, nx), hh)
+    hTr = np.stack([T * hh, S * hh])
+    ocean = np.ones((ny, nx), dtype=bool)
+    m = _metrics(nx, ny, 8000.0, 11100.0, ocean, 100.0)
+    m["wet_u"][:] = 1.0  # the reference sets every face wet, the array's edge faces too
+    m["wet_v"][:] = 1.0
+    return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost, h=h, hTr=hTr,
+                    eos_variant=EOS_LINEAR, khtr=100.0, nsteps=nsteps,
+                    name=f"analytic_{nx_phys}x{ny_phys}x{nz}", land_fraction=0.0, **m)
+
+
+GABIGHT_NX, GABIGHT_NY = 473, 297
+GABIGHT_SHA256 = "2f809c4e5c3c3b9b7326106128f2b66bb01014ed115272c7d3fd9728f6b94e14"
+GABIGHT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                            "gabight_bathy_0p1_473x297.f64.xz")
+
+
+def gabight_bathymetry() -> np.ndarray:
+    """The reference's real bathymetry (epbl/gabight_bathy_0p1_473x297.f64,
+    read at epbl/drivers/dc_main.F90:450-466): 473 x 297 little-endian FP64
+    depths in m (0 on land), Fortran order b(i, j), returned as [j, i].
+    Stored xz-compressed; the decompressed bytes are checked against the
+    original file's SHA-256."""
+    import hashlib
+    import lzma
+    with open(GABIGHT_PATH, "rb") as f:
+        raw = lzma.decompress(f.read())
+    if hashlib.sha256(raw).hexdigest() != GABIGHT_SHA256:
+        raise ValueError(f"{GABIGHT_PATH}: content does not match the reference bathymetry")
+    return np.frombuffer(raw, dtype="<f8").reshape(GABIGHT_NY, GABIGHT_NX).copy()
+
+
+def _nint(x: np.ndarray) -> np.ndarray:
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return (np.sign(x) * np.floor(np.abs(x) + 0.5)).astype(np.int64)
+
+
+def resample_bathymetry(b: np.ndarray, nx_phys: int, ny_phys: int, nghost: int) -> np.ndarray:
+    """Nearest-neighbour resample of b[j, i] onto an (ny_phys + 2 nghost) x
+    (nx_phys + 2 nghost) array, ghosts included, by the reference's rule
+    (epbl/drivers/dc_main.F90:114-124): array cell i (1-based) takes source
+    column clamp(nint(((i - nghost) - 0.5) / nx_phys * NX + 0.5), 1, NX)."""
+    BNY, BNX = b.shape
+    nx, ny = nx_phys + 2 * nghost, ny_phys + 2 * nghost
+    i = np.arange(1, nx + 1, dtype=np.float64)
+    j = np.arange(1, ny + 1, dtype=np.float64)
+    bi = np.clip(_nint(((i - nghost) - 0.5) / nx_phys * BNX + 0.5), 1, BNX) - 1
+    bj = np.clip(_nint(((j - nghost) - 0.5) / ny_phys * BNY + 0.5), 1, BNY) - 1
+    return b[np.ix_(bj, bi)]
+
+
+def case_gabight(nx_phys: int = 473, ny_phys: int = 297, nz: int = 75, nt: int = 4,
+                 nsteps: int = 100, seed: int = 0, nghost: int = 3,
+                 eos_variant: int = EOS_ROQUET_SPV, vanished: bool = False) -> RediCase:
+    """case_front's state over the real Great Australian Bight bathymetry.
+
+    Depth: the reference's 0.1 degree gabight file, resampled as the
+    reference's ePBL driver does; a column is wet where its depth exceeds
+    1 m (epbl/drivers/dc_main.F90:130), else land.  Layers: the nz-layer
+    10 m-top stretched grid over the deepest column, z*-scaled per column
+    (or z-coordinate with vanished layers).  T/S, the meridional front and
+    the passive tracers as case_front; 8 km x 11.1 km metrics."""
+    cfg = 5
+    nx, ny = nx_phys + 2 * nghost, ny_phys + 2 * nghost
+    dx, dy = 8000.0, 11100.0
+    raw = resample_bathymetry(gabight_bathymetry(), nx_phys, ny_phys, nghost)
+    ocean = raw > 1.0
+    depth = np.where(ocean, raw, 0.0)
+    dmax = float(depth.max())
+    h_ref = stretched_thickness(nz, dmax)
+    h, zc = _columns(depth, h_ref, dmax, vanished=vanished)
+    yy = np.mgrid[0:ny, 0:nx][0].astype(np.float64)
+    jit = _rng(cfg, seed, 2).uniform(-0.05, 0.05)
+    front = 1.0 * np.tanh((yy * dy - (0.5 + jit) * ny * dy) / 100_000.0)
+    hTr = _layer_tracers(cfg, seed, h, zc, 3.0, 0.5, 0.05, [front], nt)
+    m = _metrics(nx, ny, dx, dy, ocean, 1000.0)
+    return RediCase(nx_phys=nx_phys, ny_phys=ny_phys, nz=nz, nghost=nghost, h=h,
+                    hTr=hTr, eos_variant=eos_variant,
+                    alpha_T=2.0e-4, beta_S=7.6e-4, khtr=1000.0, nsteps=nsteps,
+                    name=f"gabight_{nx_phys}x{ny_phys}x{nz}_s{seed}",
+                    land_fraction=float(1.0 - ocean[nghost:-nghost, nghost:-nghost].mean()),
+                    **m)
 
 
 def case_domain(cfg: int, domain: int, side_x: int, side_y: int, nz: int,

@@ -1,0 +1,79 @@
+"""Workloads beyond the synthetic configurations (MI355X).
+
+* The step over the reference's real gabight bathymetry: the oracle's bits
+  in the FP64 form, the FP32-store form within its tolerance, and the full
+  473x297x75 domain's invariants.
+* The reference's own benchmark state (inputs.case_analytic): the oracle's
+  bits (tests/test_rk2_seam.py checks it is the state the RK2 seam builds)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import redi_oracle
+from more_benchmarks_dc_ocean_amd import inputs
+from test_gpu_fp32 import check as check_fp32
+from test_gpu_parity import _compare, _run_hip
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nxp,nyp,nz,vanished", [(120, 80, 30, False), (96, 64, 25, True)])
+def test_gabight_fp64_matches_oracle(nxp, nyp, nz, vanished, gpu_device):
+    case = inputs.case_gabight(nxp, nyp, nz, nt=4, nsteps=2, vanished=vanished)
+    got = _run_hip(case, 2, gpu_device)
+    orc = redi_oracle.run_oracle(case)
+    _compare(got, orc, case.name)
+    for k in ("hTr1", "hTrN"):
+        assert np.array_equal(got[k], orc[k]), k
+
+
+def test_gabight_full_size_fp64_matches_oracle(gpu_device):
+    """473x297x75 at the bathymetry's native resolution, one step."""
+    case = inputs.case_gabight(nsteps=1)
+    got = _run_hip(case, 1, gpu_device)
+    orc = redi_oracle.run_oracle(case, nsteps=1)
+    _compare(got, orc, case.name)
+    assert np.array_equal(got["hTr1"], orc["hTr1"])
+
+
+def test_gabight_fp32_store_within_tolerance(gpu_device):
+    check_fp32(inputs.case_gabight(120, 80, 30, nt=4, nsteps=1), gpu_device)
+
+
+def test_gabight_full_size_invariants(gpu_device):
+    """473x297x75 at the bathymetry's native resolution, two steps: finite,
+    each tracer's interior total conserved behind the walls, a uniform
+    passive tracer left bit-identical, and the same bits on a second run."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    case = inputs.case_gabight(nsteps=2)
+    case.hTr[3] = 0.5 * case.h
+    g = case.nghost
+    area = case.areaT[None]
+    tot0 = [(case.hTr[t] * area)[:, g:-g, g:-g].sum() for t in range(case.nt)]
+    print(f"[gpu] gabight {case.nx}x{case.ny}x{case.nz} land={case.land_fraction:.3f}", flush=True)
+    r, dom = make_redi(case, gpu_device)
+    r.step(dom, 2)
+    torch.cuda.synchronize()
+    a = dom.tracers_host()
+    assert np.isfinite(a).all()
+    for t in range(case.nt):
+        tot = (a[t] * area)[:, g:-g, g:-g].sum()
+        assert abs(tot - tot0[t]) <= 1e-12 * abs(tot0[t]), (t, tot, tot0[t])
+    assert np.array_equal(a[3], case.hTr[3])
+    assert not np.array_equal(a[0], case.hTr[0])
+    dom.set_tracers(case.hTr)
+    r.step(dom, 2)
+    torch.cuda.synchronize()
+    assert np.array_equal(dom.tracers_host(), a)
+    r.close()
+
+
+def test_analytic_reference_state_matches_oracle(gpu_device):
+    case = inputs.case_analytic(40, 30, 30, nsteps=3)
+    got = _run_hip(case, 3, gpu_device)
+    orc = redi_oracle.run_oracle(case)
+    _compare(got, orc, case.name)
+    assert np.array_equal(got["hTrN"], orc["hTrN"])
+

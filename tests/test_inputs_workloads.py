@@ -1,0 +1,91 @@
+"""Workloads beyond the synthetic configurations (CPU): the real-bathymetry
+workload (the reference's gabight depths and its nearest-neighbour
+resampling) and the reference's own benchmark state."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from more_benchmarks_dc_ocean_amd import inputs
+
+
+def test_bathymetry_is_the_reference_file():
+    b = inputs.gabight_bathymetry()  # raises unless the bytes hash to the reference file's
+    assert b.shape == (inputs.GABIGHT_NY, inputs.GABIGHT_NX)
+    assert b.min() == 0.0 and 6000.0 < b.max() < 7000.0
+    assert 0.85 < float((b > 1.0).mean()) < 0.93
+
+
+def test_resample_at_native_size_is_the_identity_with_edge_ghosts():
+    b = inputs.gabight_bathymetry()
+    g = 3
+    r = inputs.resample_bathymetry(b, inputs.GABIGHT_NX, inputs.GABIGHT_NY, g)
+    assert r.shape == (inputs.GABIGHT_NY + 2 * g, inputs.GABIGHT_NX + 2 * g)
+    assert np.array_equal(r[g:-g, g:-g], b)
+    # ghost cells clamp to the nearest edge row / column of the source
+    assert np.array_equal(r[:g, g:-g], np.repeat(b[:1], g, axis=0))
+    assert np.array_equal(r[g:-g, -g:], np.repeat(b[:, -1:], g, axis=1))
+
+
+@pytest.mark.parametrize("nxp,nyp", [(236, 148), (120, 80), (946, 594)])
+def test_resample_picks_source_cells_in_order(nxp, nyp):
+    """Every target cell takes one source cell; source indices never
+    decrease, the first and last interior cells take source cells within one
+    stride of the source's ends, and, at twice the native size, every source
+    cell twice."""
+    src = np.arange(inputs.GABIGHT_NY * inputs.GABIGHT_NX, dtype=np.float64).reshape(
+        inputs.GABIGHT_NY, inputs.GABIGHT_NX)
+    r = inputs.resample_bathymetry(src, nxp, nyp, 3)[3:-3, 3:-3]
+    ci = (r[0] % inputs.GABIGHT_NX).astype(int)
+    cj = (r[:, 0] // inputs.GABIGHT_NX).astype(int)
+    assert np.all(np.diff(ci) >= 0) and np.all(np.diff(cj) >= 0)
+    sx = -(-inputs.GABIGHT_NX // nxp)  # source cells per target cell, rounded up
+    sy = -(-inputs.GABIGHT_NY // nyp)
+    assert ci[0] <= sx and cj[0] <= sy
+    assert ci[-1] >= inputs.GABIGHT_NX - 1 - sx and cj[-1] >= inputs.GABIGHT_NY - 1 - sy
+    # each row is one source row, each column one source column
+    assert np.all(r == (cj[:, None] * inputs.GABIGHT_NX + ci[None, :]))
+    if nxp == 2 * inputs.GABIGHT_NX:
+        assert np.array_equal(np.bincount(ci), np.full(inputs.GABIGHT_NX, 2))
+        assert np.array_equal(np.bincount(cj), np.full(inputs.GABIGHT_NY, 2))
+
+
+def test_gabight_case_land_and_columns():
+    c = inputs.case_gabight(120, 80, 30, nt=3, nsteps=1)
+    g = c.nghost
+    depth = c.h.sum(axis=0)
+    raw = inputs.resample_bathymetry(inputs.gabight_bathymetry(), 120, 80, g)
+    ocean = raw > 1.0
+    assert np.array_equal(depth > 0, ocean)
+    assert np.allclose(depth[ocean], raw[ocean], rtol=1e-12)
+    assert np.all(c.hTr[:, :, ~ocean] == 0.0)
+    assert c.land_fraction == pytest.approx(1.0 - ocean[g:-g, g:-g].mean())
+    # a wet face needs ocean on both sides
+    assert np.array_equal(c.wet_u[:, 1:-1] > 0, ocean[:, :-1] & ocean[:, 1:])
+
+
+def test_analytic_state_follows_the_reference_loops():
+    """inputs.case_analytic against a cell-by-cell walk of the reference's
+    build_state loops (Fortran (i, j, k) with k = 1 at the bed,
+    legacy_testing/redi_benchmark/redi_bench.F90:322-359), and its setup."""
+    nxp, nyp, nz = 5, 4, 3
+    c = inputs.case_analytic(nxp, nyp, nz)
+    nx, ny = nxp + 6, nyp + 6
+    hh = 4000.0 / nz
+    for kk in range(1, nz + 1):
+        zz = (nz - kk + 0.5) * hh
+        for jj in range(1, ny + 1):
+            fy = (jj - 1) / (ny - 1)
+            for ii in range(1, nx + 1):
+                fx = (ii - 1) / (nx - 1)
+                tt = 2.0 + 16.0 * math.exp(-zz / 800.0) + 3.0 * fx + 3.0 * fy
+                ss = 34.5 + 0.7 * (1.0 - math.exp(-zz / 1500.0)) - 0.2 * fx
+                assert c.h[kk - 1, jj - 1, ii - 1] == hh
+                assert c.hTr[0, kk - 1, jj - 1, ii - 1] == pytest.approx(tt * hh, rel=4e-16)
+                assert c.hTr[1, kk - 1, jj - 1, ii - 1] == pytest.approx(ss * hh, rel=4e-16)
+    assert np.all(c.wet_u == 1.0) and np.all(c.wet_v == 1.0)
+    assert np.all(c.dy_cu == 11100.0) and np.all(c.idxCu == 1.0 / 8000.0)
+    assert np.all(c.khtr_u_ext == 100.0) and c.khtr == 100.0 and c.use_ext == 1
+    assert c.walls == (1, 1, 1, 1) and c.eos_variant == inputs.EOS_LINEAR and c.dt == 900.0

@@ -20,6 +20,7 @@ import pytest
 from conftest import ROOT
 
 import redi_oracle
+from more_benchmarks_dc_ocean_amd import inputs
 from more_benchmarks_dc_ocean_amd.inputs import RediCase
 
 HEADER = os.path.join(ROOT, "include", "rk2_redi_hip.h")
@@ -57,6 +58,12 @@ def test_seam_stages_match_oracle(gpu_device):
     L.rk2_redi_init(nxp, nyp, nz)
     h, t, s = (np.zeros((nz, ny, nx)) for _ in range(3))
     assert L.rk2_redi_export(h.ctypes.data, t.ctypes.data, s.ctypes.data) == 0
+    # the seam's C++ state and inputs.case_analytic restate the same formulas
+    # (the exp of numpy and of the C library may differ in the last bit)
+    ana = inputs.case_analytic(nxp, nyp, nz)
+    assert np.array_equal(h, ana.h)
+    for got, ref in ((t, ana.hTr[0]), (s, ana.hTr[1])):
+        assert np.max(np.abs(got - ref) / np.abs(ref)) <= 4e-16
     case = RediCase(nx_phys=nxp, ny_phys=nyp, nz=nz, h=h.copy(), hTr=np.stack([t, s]),
                     dy_cu=np.full((ny, nx + 1), 11100.0), dx_cv=np.full((ny + 1, nx), 8000.0),
                     idxCu=np.full((ny, nx + 1), 1.0 / 8000.0),
