@@ -997,19 +997,19 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
     // so every addition still lands in reference order
     int curK = 0;
     double cur = 0.0;
-    for (int s0 = 0; s0 < nsub; s0 += kBatch) {
-      uint32_t pk[kBatch];
-      R x[kBatch];
+    // every (sublayer, face) slot of a computed face holds a flux (0 where
+    // nothing contributes), so both loads issue without waiting; rows past
+    // the last sublayer re-read the last one and are skipped, which keeps
+    // the loads free of branches
+    auto load = [&](int s0, uint32_t (&pk)[kBatch], R (&x)[kBatch]) {
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        // every (sublayer, face) slot of a computed face holds a flux (0 where
-        // nothing contributes), so both loads issue without waiting; rows past
-        // the last sublayer re-read the last one and are skipped below, which
-        // keeps the loads free of branches
         const int s = s0 + u < nsub ? s0 + u : nsub - 1;
         pk[u] = tg[(size_t)s * nf];
         x[u] = fx[(size_t)s * nf * NT];
       }
+    };
+    auto add = [&](int s0, const uint32_t (&pk)[kBatch], const R (&x)[kBatch]) {
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         if (s0 + u >= nsub || pk[u] == 0) continue;
@@ -1021,6 +1021,19 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
         }
         cur = left ? cur + (double)x[u] : cur - (double)x[u];
       }
+    };
+    // two batches of rows in registers: the next batch's loads are in
+    // flight while the current one is added (one wave per SIMD, so the
+    // registers are free and nothing else hides the latency)
+    uint32_t pa[kBatch], pb[kBatch];
+    R xa[kBatch], xb[kBatch];
+    load(0, pa, xa);
+    for (int s0 = 0; s0 < nsub; s0 += 2 * kBatch) {
+      const bool second = s0 + kBatch < nsub;
+      if (second) load(s0 + kBatch, pb, xb);
+      add(s0, pa, xa);
+      if (s0 + 2 * kBatch < nsub) load(s0 + 2 * kBatch, pa, xa);
+      if (second) add(s0 + kBatch, pb, xb);
     }
     if (curK) acc[(curK - 1) * 64 + lane] = cur;
   }
