@@ -24,7 +24,8 @@ The generators build the domains of the benchmark workloads:
 All randomness comes from ``numpy.random.default_rng`` seeded with
 ``(config, domain, field)`` so a case is reproducible bit-for-bit on any host
 with the same numpy.  ``write_input`` / ``read_output`` implement the "REDI" /
-"REDO" v1 files read and written by ``oracle/ref_driver.F90``.
+"REDO" v1 files read and written by ``oracle/ref_driver.F90``;
+``write_state_dump`` / ``read_state_dump`` the reference drivers' raw dumps.
 """
 from __future__ import annotations
 
@@ -520,6 +521,34 @@ def read_output(path: str) -> dict:
     if off != len(buf):
         raise ValueError(f"{path}: {len(buf) - off} trailing bytes")
     return out
+
+
+def write_state_dump(path: str, *fields: np.ndarray) -> None:
+    """The reference drivers' raw state dump: int32 nx, ny, nz, then each
+    (nz, ny, nx) FP64 field in Fortran order (x fastest), little-endian,
+    no record markers.  One field (the temperature tracer hTr) is
+    redi/drivers/dc_main.F90:156-163's DC_DUMP file; h, hTr_T, hTr_S is
+    legacy_testing/redi_benchmark/redi_bench.F90:379-394's."""
+    nz, ny, nx = fields[0].shape
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, nz], dtype="<i4").tobytes())
+        for a in fields:
+            if a.shape != (nz, ny, nx):
+                raise ValueError(f"field shape {a.shape} != {(nz, ny, nx)}")
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+
+
+def read_state_dump(path: str) -> List[np.ndarray]:
+    """Fields of a write_state_dump / reference dump file, each (nz, ny, nx)."""
+    raw = open(path, "rb").read()
+    if len(raw) < 12:
+        raise ValueError(f"{path}: no header")
+    nx, ny, nz = (int(v) for v in np.frombuffer(raw[:12], "<i4"))
+    n = nx * ny * nz * 8
+    if nx <= 0 or ny <= 0 or nz <= 0 or (len(raw) - 12) % n:
+        raise ValueError(f"{path}: {len(raw) - 12} data bytes for a {nx}x{ny}x{nz} grid")
+    return [np.frombuffer(raw[12 + q * n:12 + (q + 1) * n], "<f8").reshape(nz, ny, nx).copy()
+            for q in range((len(raw) - 12) // n)]
 
 
 def crop(c: RediCase, i0: int, j0: int, nxp: int, nyp: int) -> RediCase:

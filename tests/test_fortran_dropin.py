@@ -102,3 +102,42 @@ def test_reference_dc_main_on_hip_dropin_same_bytes(tmp_path, args, gpu_device):
     ref = _dc_main(redi_oracle.DC_MAIN_REF_BIN, args, tmp_path / "ref.bin", tmp_path)
     hip = _dc_main(redi_oracle.DC_MAIN_HIP_BIN, args, tmp_path / "hip.bin", tmp_path)
     assert ref == hip, "the HIP drop-in's dump differs from the reference build's"
+
+
+# The Python path against the same driver: inputs.case_analytic is the state
+# dc_main.F90 builds (:172-206), so N steps of it must give the bytes of the
+# driver's dump after its warm-up + timed calls.
+def test_analytic_state_stepped_by_the_oracle_is_dc_mains_dump(tmp_path):
+    if not os.path.exists(redi_oracle.DC_MAIN_REF_BIN):
+        pytest.skip("the reference build (oracle/_ref) is absent here")
+    args = DC_CASES[0]
+    raw = _dc_main(redi_oracle.DC_MAIN_REF_BIN, args, tmp_path / "ref.bin", tmp_path)
+    nxp, nyp, nz, reps, warm = (int(a) for a in args)
+    case = inputs.case_analytic(nxp, nyp, nz, nsteps=reps + warm)
+    out = redi_oracle.run_oracle(case)
+    inputs.write_state_dump(str(tmp_path / "oracle.bin"), out["hTrN"][0])
+    assert open(tmp_path / "oracle.bin", "rb").read() == raw
+    (t,) = inputs.read_state_dump(str(tmp_path / "ref.bin"))
+    assert np.array_equal(t, out["hTrN"][0])
+
+
+@pytest.mark.gpu
+def test_hip_context_api_gives_dc_mains_dump_473x297x30(tmp_path, gpu_device):
+    """The reference driver's own default domain (473x297x30, 2 timed calls
+    after 1 warm-up) through redi_calc_coeffs / redi_apply_flux on the GPU,
+    written in the driver's dump format: the reference build's bytes."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    assert os.path.exists(redi_oracle.DC_MAIN_REF_BIN), "oracle/_ref is not built"
+    args = ("473", "297", "30", "2", "1")
+    print(f"[gpu] dc_main dump {' '.join(args)} from the context API", flush=True)
+    raw = _dc_main(redi_oracle.DC_MAIN_REF_BIN, args, tmp_path / "ref.bin", tmp_path)
+    case = inputs.case_analytic(473, 297, 30, nsteps=3)
+    r, dom = make_redi(case, gpu_device)
+    for _ in range(3):
+        assert r.calc_coeffs(dom) == 0
+        assert r.apply_flux(dom) == 0
+    torch.cuda.synchronize()
+    inputs.write_state_dump(str(tmp_path / "hip.bin"), dom.tracers_host()[0])
+    r.close()
+    assert open(tmp_path / "hip.bin", "rb").read() == raw

@@ -89,3 +89,22 @@ def test_analytic_state_follows_the_reference_loops():
     assert np.all(c.dy_cu == 11100.0) and np.all(c.idxCu == 1.0 / 8000.0)
     assert np.all(c.khtr_u_ext == 100.0) and c.khtr == 100.0 and c.use_ext == 1
     assert c.walls == (1, 1, 1, 1) and c.eos_variant == inputs.EOS_LINEAR and c.dt == 900.0
+
+
+def test_state_dump_round_trip_and_layout(tmp_path):
+    """Three fields (the redi_bench.F90 dump: h, hTr_T, hTr_S) round-trip;
+    the header is int32 nx, ny, nz and x runs fastest in the data."""
+    c = inputs.case_analytic(5, 4, 3)
+    p = str(tmp_path / "s.bin")
+    inputs.write_state_dump(p, c.h, c.hTr[0], c.hTr[1])
+    raw = open(p, "rb").read()
+    assert np.array_equal(np.frombuffer(raw[:12], "<i4"), [c.nx, c.ny, c.nz])
+    assert len(raw) == 12 + 3 * 8 * c.nx * c.ny * c.nz
+    # element (i=1, j=0, k=0) (0-based) follows element (0, 0, 0)
+    assert np.frombuffer(raw[20:28], "<f8")[0] == c.h[0, 0, 1]
+    h, t, s = inputs.read_state_dump(p)
+    assert np.array_equal(h, c.h) and np.array_equal(t, c.hTr[0]) and np.array_equal(s, c.hTr[1])
+    with open(p, "ab") as f:
+        f.write(b"x")
+    with pytest.raises(ValueError):
+        inputs.read_state_dump(p)
