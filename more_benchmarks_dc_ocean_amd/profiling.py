@@ -7,9 +7,9 @@ Runs, in one process on cuda:0:
   2. ``--warmup`` + ``--steps`` Redi steps of the chosen workload.
 
 Typical use (each counter in its own run):
-    rocprofv3 --kernel-trace --stats -d gpurun_out/prof -o run -- \
+    rocprofv3 --kernel-trace --stats -d prof_out/kt -o run -- \
         python -m more_benchmarks_dc_ocean_amd.profiling --workload two_fronts_1440x1080x75 --steps 3
-    rocprofv3 --pmc FETCH_SIZE -d gpurun_out/pmc_fetch -o run -- \
+    rocprofv3 --pmc FETCH_SIZE -d prof_out/fetch -o run -- \
         python -m more_benchmarks_dc_ocean_amd.profiling --workload two_fronts_1440x1080x75 --steps 1
 ``summarize`` turns the CSV outputs into the per-kernel figures kept under
 profiles/.

@@ -1,7 +1,7 @@
 /*
  * redi_oracle.c -- CPU restatement of the reference Redi neutral-diffusion
- * step, FP64.  TEST INFRASTRUCTURE ONLY: it is the checker for the HIP
- * product and the CPU baseline, never part of the product path.
+ * step, FP64.  TEST INFRASTRUCTURE ONLY: it checks the results of the HIP
+ * product and serves as the CPU baseline; it is never part of the product path.
  *
  * Each function restates one reference routine, cited as file:line under
  * redi/ in the reference tree, in its own code but with the reference's

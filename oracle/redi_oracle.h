@@ -3,7 +3,7 @@
  * INFRASTRUCTURE ONLY).
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
- * this library, and only as the checker / the CPU baseline.  The product
+ * this library, and only to check results or as the CPU baseline.  The product
  * (the HIP library under more_benchmarks_dc_ocean_amd/) never links or calls
  * it.
  *

@@ -1,7 +1,7 @@
 """Python access to the oracle (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
-import this module, and only as the checker or the timed CPU baseline; the
+import this module, and only to check results or as the timed CPU baseline; the
 product path never touches it.
 
 * ``OracleRedi`` mirrors ``ocean_redi_t`` + ``redi_calc_coeffs`` +

@@ -2,7 +2,7 @@
 
 Tests marked ``gpu`` need an MI355X (HIP device) and call the product
 through the C ABI of libredi_hip.so; everything else runs on the CPU.
-The oracle (``oracle/``) is used here only as the checker.
+The oracle (``oracle/``) is used here only to check results.
 """
 from __future__ import annotations
 

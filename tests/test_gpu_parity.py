@@ -2,7 +2,7 @@
 
 Every call goes through the C ABI of libredi_hip.so.  Bar for the plain FP64
 form: the reference's own outputs reproduced to <= 1e-13 per-cell relative
-error (tolerance of the north-star target; the reference's cross-checks use
+error (the stated tolerance for the plain form; the reference's cross-checks use
 1e-12, redi/drivers/dc_main.F90:237-241).  In practice the kernels follow the
 reference's operation order with FP contraction off, so the comparisons below
 also record whether the match is exact.
