@@ -27,6 +27,9 @@ second), plus
                 a bounded sample of the same workload, or the oracle's C
                 restatement when that binary is absent.
 
+  real_bathymetry
+                both forms on the reference's real 0.1-degree gabight
+                bathymetry (473x297x75, 4 tracers).
   reference_benchmark
                 the FP64 form on the reference's own benchmark state
                 (473x297x30, all wet, linear EOS, 2 tracers) beside the
@@ -199,6 +202,20 @@ def time_reference_config(steps: int, warmup: int = 10) -> dict:
             "speedup_vs_fastest_published": round(best / r["ms_per_step"], 2)}
 
 
+def time_gabight(steps: int, warmup: int) -> dict:
+    """Both forms on the real Great Australian Bight bathymetry at its native
+    473x297 resolution, 75 layers, 4 tracers (inputs.case_gabight)."""
+    from more_benchmarks_dc_ocean_amd import inputs
+    case = inputs.case_gabight(nsteps=steps)
+    out = {"workload": case.name, "nx_phys": case.nx_phys, "ny_phys": case.ny_phys, "nz": case.nz,
+           "ntracers": case.nt, "eos": "roquet_spv", "land_fraction": round(case.land_fraction, 4),
+           "steps": steps, "unit": "Mcell-steps/s"}
+    for form in ("fp64", "fp32_store"):
+        r = time_form(case, form, steps, warmup, case.name)
+        out[form] = {"value": r["value"], "ms_per_step": r["ms_per_step"]}
+    return out
+
+
 def time_batched(cases, form: str, steps: int, warmup: int, persist: int = 1,
                  tuned: bool = True) -> dict:
     """Batched workload: every domain of ``cases`` in one RediBatch, timed
@@ -328,6 +345,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="timed steps of the reference's own benchmark configuration (0 = skip it)")
+    ap.add_argument("--gabight-steps", type=int, default=10,
+                    help="timed steps of the real-bathymetry workload (0 = skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; the Redi step of one "
@@ -371,6 +390,8 @@ def main() -> int:
         line["forms"] = {"fp32_store": f32}
     if args.reference_steps > 0:
         line["reference_benchmark"] = time_reference_config(args.reference_steps)
+    if args.gabight_steps > 0:
+        line["real_bathymetry"] = time_gabight(args.gabight_steps, args.warmup)
     if args.batched > 0:
         from more_benchmarks_dc_ocean_amd import inputs as _inp
         t0 = time.time()

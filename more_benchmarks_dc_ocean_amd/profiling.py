@@ -35,7 +35,9 @@ def _cases(workload: str, steps: int):
     if workload == BATCHED:
         return inputs.case_batch(96, seed=0, nsteps=steps)
     builder = {"seamount_64x64x25": inputs.case_seamount, "front_473x297x75": inputs.case_front,
-               "two_fronts_1440x1080x75": inputs.case_two_fronts}[workload]
+               "two_fronts_1440x1080x75": inputs.case_two_fronts,
+               "gabight_473x297x75": inputs.case_gabight,
+               "analytic_473x297x30": inputs.case_analytic}[workload]
     return [builder(nsteps=steps)]
 
 
