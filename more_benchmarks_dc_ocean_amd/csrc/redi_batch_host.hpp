@@ -218,7 +218,8 @@ struct BatchOps {
                          blocks_for(N2, 128));
       const double pa_to_h = 1.0 / (kGravity * d.eos.rho0);
       if (nx > 1) {
-        SweepItem<F> it{SweepArgs<F>{nx, nz, nx + 1, 1, (uint32_t)d.fs.nup, 1, 0, 1, 0, d.wet_u, rec, P,
+        SweepItem<F> it{SweepArgs<F>{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0,
+                                     d.wet_u, rec, P,
                                      (FaceRecF<F>*)d.fs.u, d.fs.mono_u, pa_to_h, d.p.idxCu, d.slope_max},
                         TileMap{}};
         unsigned nb;
@@ -226,7 +227,8 @@ struct BatchOps {
         w[kNodeSweep].add(it, nb);
       }
       if (ny > 1) {
-        SweepItem<F> it{SweepArgs<F>{nx, nz, nx, 1, (uint32_t)d.fs.nvp, 0, 1, 0, 1, d.wet_v, rec, P,
+        SweepItem<F> it{SweepArgs<F>{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1,
+                                     d.wet_v, rec, P,
                                      (FaceRecF<F>*)d.fs.v, d.fs.mono_v, pa_to_h, d.p.idyCv, d.slope_max},
                         TileMap{}};
         unsigned nb;

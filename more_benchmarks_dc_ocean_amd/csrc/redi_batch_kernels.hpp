@@ -103,8 +103,9 @@ __global__ void __launch_bounds__(128) kb_column_state(BatchView<ColumnArgs<F>> 
 
 template <class F>
 __global__ void __launch_bounds__(kBatchSweepLanes) kb_neutral_sweep(BatchView<SweepItem<F>> v) {
-  for_each_block<false>(v, [](const SweepItem<F>& it, int bid, int nb) {
-    neutral_sweep_body<F>(it.a, it.tm, bid, nb);
+  __shared__ uint4 stage[kSweepStageWords];
+  for_each_block<false>(v, [&](const SweepItem<F>& it, int bid, int nb) {
+    neutral_sweep_body<F>(it.a, it.tm, bid, nb, stage);
   });
 }
 
@@ -131,10 +132,9 @@ __global__ void __launch_bounds__(64 * kFaceGroup) __attribute__((amdgpu_waves_p
     kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
-  __shared__ FaceRecF<F> srec[face_rec_lds<F>()];
   __shared__ ReconRec stage[kStage ? kFaceGroup * kStageRecs : 1];
   for_each_block<true>(v, [&](const FaceFluxArgs<F>& a, int bid, int nb) {
-    face_flux_body<F, NT, kStage>(a, bid, nb, sx, stg, srec, stage);
+    face_flux_body<F, NT, kStage>(a, bid, nb, sx, stg, stage);
   });
 }
 

@@ -149,6 +149,13 @@ struct Fp32 {
 
 template <class F> using FaceRecF = typename F::Face;
 
+// Face record (face, s) lives at face*fa + s*fb: face-major, fa = ns,
+// fb = 1.  A face kernel's wave reads its face's surfaces as one contiguous
+// run; the sweep, whose lanes are adjacent faces, transposes its records
+// through LDS to store whole runs.
+inline uint32_t face_fa(int ns) { return (uint32_t)ns; }
+inline uint32_t face_fb(int) { return 1u; }
+
 // Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
 // non-decreasing in s on the left (bit 0) / right (bit 1) column.
 constexpr uint8_t kMonoL = 1, kMonoR = 2;
@@ -331,34 +338,68 @@ __device__ __forceinline__ double slope_taper(double hm, double topL, double bot
 // Density differences, positions, pressures and thicknesses are carried in
 // FP64 in both forms; positions and thicknesses are stored in the form's
 // type.
+// The sweep's records reach the face-major face array through LDS: each
+// lane puts its surface's record in its wave's stage, and every run of
+// kSweepRunWords / (record words) surfaces the wave writes the stage out
+// transposed -- 8 lanes per face, 8 faces per store instruction -- so every
+// store fills whole 128-byte runs of records.
+constexpr int kSweepRunWords = 8;                              // 16-byte words per face per run
+constexpr int kSweepStageWords = 64 * kSweepRunWords;          // per wave
+constexpr size_t kSweepStageBytes = kSweepStageWords * sizeof(uint4);
+
 template <class F>
 __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const TileMap& tm,
-                                                   int bid, int nb) {
+                                                   int bid, int nb, uint4* stage) {
   using Face = FaceRecF<F>;
+  constexpr int kW = (int)(sizeof(Face) / sizeof(uint4));
+  constexpr int kRun = kSweepRunWords / kW;  // surfaces per run
+  const int lane = (int)(threadIdx.x & 63);
+  uint4* stw = stage + (threadIdx.x >> 6) * kSweepStageWords;
   int fi, fj;
-  if (!tm.map(bid, nb, fi, fj)) return;
+  const bool inr = tm.map(bid, nb, fi, fj);
   const int i = a.i0 + fi, j = a.j0 + fj;
-  const size_t fo = (size_t)i + (size_t)a.nfa * j;
+  const uint32_t fo = inr ? (uint32_t)i + (uint32_t)a.nfa * (uint32_t)j : 0u;
   const int nk = a.nz, ns = 2 * nk + 2;
-  FaceRecF<F>* o = a.out + fo * a.fa;
-  const size_t fst = a.fb;
-  if (!(a.wet[fo] > 0.0)) {  // dry face: the inert values of :722-730
-    for (int s = 0; s < ns; ++s) o[s * fst] = Face::put(SurfVals{0.0, 0.0, 0.0, 1, 1});
-    a.mono[fo] = kMonoL | kMonoR;
-    return;
-  }
+  const bool wet = inr && a.wet[fo] > 0.0;  // dry face: the inert values of :722-730
+  const SurfVals inert{0.0, 0.0, 0.0, 1, 1};
+  auto put = [&](int s, const SurfVals& v) {
+    const Face r = Face::put(v);
+    uint4 w[kW];
+    __builtin_memcpy(w, &r, sizeof(Face));
+#pragma unroll
+    for (int q = 0; q < kW; ++q) stw[lane * kSweepRunWords + (s % kRun) * kW + q] = w[q];
+  };
+  // surfaces s0 .. s0+cnt-1 of every face of the wave
+  auto flush = [&](int s0, int cnt) {
+    __syncthreads();
+    const int wd = lane % kSweepRunWords, sq = wd / kW;
+#pragma unroll
+    for (int it = 0; it < 64 / (64 / kSweepRunWords); ++it) {
+      const int q = it * (64 / kSweepRunWords) + lane / kSweepRunWords;
+      const uint32_t foq = (uint32_t)__shfl((int)fo, q, 64);
+      const int okq = __shfl(inr ? 1 : 0, q, 64);
+      if (okq && sq < cnt) {
+        uint4* dst = reinterpret_cast<uint4*>(a.out + (size_t)foq * a.fa + (size_t)(s0 + sq) * a.fb);
+        dst[wd % kW] = stw[q * kSweepRunWords + wd];
+      }
+    }
+    __syncthreads();
+  };
   const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
   const size_t cr = (size_t)i + (size_t)a.nx * j;
-  SweepSide L, Rt;
-  const size_t bl = colstate_base(cl, a.nx, nk), br = colstate_base(cr, a.nx, nk);
-  L.init(a.rec + bl, a.P + bl, (size_t)a.nx, nk);
-  Rt.init(a.rec + br, a.P + br, (size_t)a.nx, nk);
+  SweepSide L{}, Rt{};
+  if (wet) {
+    const size_t bl = colstate_base(cl, a.nx, nk), br = colstate_base(cr, a.nx, nk);
+    L.init(a.rec + bl, a.P + bl, (size_t)a.nx, nk);
+    Rt.init(a.rec + br, a.P + br, (size_t)a.nx, nk);
+  }
 
   int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
   double lastP_l = 0.0, lastP_r = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;
   bool reached_bottom = false, search_l = false, mono_l = true, mono_r = true;
   SurfVals pend{};
   for (int ks = 1; ks <= ns; ++ks) {
+   if (wet) {
     if (!reached_bottom) {
       const double dRho = sec_drho(Rt.cur(), L.cur());
       if (dRho < 0.0)
@@ -457,19 +498,26 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         contK_l = pend.KoL;
         contK_r = pend.KoR;
       }
-      o[(size_t)(ks - 2) * fst] = Face::put(pend);
+      put(ks - 2, pend);
     }
     pend = SurfVals{PoL, PoR, 0.0, KoL, KoR};
     lastPos_l = posL;
     lastPos_r = posR;
+   } else if (ks > 1) {
+    put(ks - 2, inert);
+   }
+    if (ks > 1 && (ks - 2) % kRun == kRun - 1) flush(ks - 2 - (kRun - 1), kRun);
   }
-  o[(size_t)(ns - 1) * fst] = Face::put(pend);
-  a.mono[fo] = (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0));
+  put(ns - 1, wet ? pend : inert);
+  flush(ns - 1 - (ns - 1) % kRun, (ns - 1) % kRun + 1);
+  if (inr) a.mono[fo] = wet ? (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0)) : (uint8_t)(kMonoL | kMonoR);
 }
 
+// Dynamic LDS: kSweepStageBytes per wave of the block.
 template <class F>
 __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap tm) {
-  neutral_sweep_body<F>(a, tm, (int)blockIdx.x, (int)gridDim.x);
+  extern __shared__ uint4 sweep_stage[];
+  neutral_sweep_body<F>(a, tm, (int)blockIdx.x, (int)gridDim.x, sweep_stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -733,18 +781,10 @@ struct FaceFluxArgs {
 constexpr int kStageRecs = 128;  // LDS records per wavefront (64 per side)
 constexpr bool kFaceStageDefault = false;  // staged reads unless a caller chooses
 
-// Face records of a chunk in LDS (FP64 form): face q's 64 surfaces at
-// q*kRecRow, the row padded by one record so that the workgroup's
-// row-by-row stores spread over the banks.
-constexpr int kRecRow = 65;
-template <class F>
-constexpr int face_rec_lds() { return sizeof(FaceRecF<F>) > 16 ? kFaceGroup * kRecRow : 1; }
-
 // Face group `grp` (faces grp*kFaceGroup .. +kFaceGroup-1 of the face array).
 template <class F, int NT, bool kStage = false>
 __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int grp,
                                                 typename F::R* sx, uint32_t* stg,
-                                                FaceRecF<F>* srec,
                                                 ReconRec* stage_lds = nullptr) {
   using R = typename F::R;
   using Face = FaceRecF<F>;
@@ -767,40 +807,12 @@ __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int gr
   const uint32_t cb = a.ry.cb, ct = a.ry.ct;
   const size_t fb = a.fb;
   const int ns = a.ns, nsub = ns - 1;
-  // 32-byte face records (FP64 form) come through LDS, one chunk ahead:
-  // thread (row rr, face q) loads surface c0 + rr of face f0 + q, so the
-  // kFaceGroup threads of a row read one contiguous run of the
-  // surface-major array, where a wave reading its own face lane by lane
-  // touches a separate line per surface and record half.  The 16-byte
-  // records of the FP32-store form are read lane by lane: measured, their
-  // staging costs more (one more wave-wide live record) than it saves.
-  // Records move as 16-byte words.
-  constexpr bool kRecLds = face_rec_lds<F>() > 1;
-  constexpr int kW = (int)(sizeof(Face) / sizeof(uint4));
-  const int rr_ld = (int)(threadIdx.x / kFaceGroup), q_ld = (int)(threadIdx.x % kFaceGroup);
-  const bool face_ld = kRecLds && f0 + q_ld < a.nf;
-  const uint4* rec_w = reinterpret_cast<const uint4*>(a.rec);
-  const size_t face_w = (f0 + q_ld) * a.fa;
-  uint4* srec_w = reinterpret_cast<uint4*>(srec);
-  uint4 pre[kW];
-#pragma unroll
-  for (int w = 0; w < kW; ++w) pre[w] = make_uint4(0u, 0u, 0u, 0u);
-  if (face_ld && rr_ld < ns) {
-#pragma unroll
-    for (int w = 0; w < kW; ++w) pre[w] = rec_w[(face_w + (size_t)rr_ld * fb) * kW + w];
-  }
   const Face* r = a.rec + (live ? fo * a.fa : 0);
   for (int c0 = 0; c0 < nsub; c0 += kChunk) {
     const int s = c0 + lane;
     const bool surf = live && s < ns;
-    // every thread is past the previous chunk's reads of srec and sx
-    if constexpr (kRecLds) {
-#pragma unroll
-      for (int w = 0; w < kW; ++w) srec_w[(q_ld * kRecRow + rr_ld) * kW + w] = pre[w];
-    }
-    __syncthreads();
     SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
-    if (surf) Fs = kRecLds ? srec[wv * kRecRow + lane].get() : r[(size_t)s * fb].get();
+    if (surf) Fs = r[(size_t)s * fb].get();
     const int KoL = Fs.KoL, KoR = Fs.KoR;
     const double PoL = Fs.PoL, PoR = Fs.PoR;
     double eL[NT], eR[NT];
@@ -877,12 +889,6 @@ __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int gr
 #pragma unroll
     for (int t = 0; t < NT; ++t) sx[(wv * 64 + lane) * NT + t] = x[t];
     stg[wv * 64 + lane] = packed;
-    // the next chunk's records, in flight over the barrier and the row writes
-    const int sn = c0 + kChunk + rr_ld;
-    if (face_ld && sn < ns) {
-#pragma unroll
-      for (int w = 0; w < kW; ++w) pre[w] = rec_w[(face_w + (size_t)sn * fb) * kW + w];
-    }
     __syncthreads();
     // row rr of the chunk, face q of the group
     const int rr = (int)(threadIdx.x / kFaceGroup), q = (int)(threadIdx.x % kFaceGroup);
@@ -893,9 +899,7 @@ __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int gr
       for (int t = 0; t < NT; ++t) a.flx[o * NT + t] = sx[(q * 64 + rr) * NT + t];
       a.tgt[o] = stg[q * 64 + rr];
     }
-    // the next chunk's first barrier orders these reads of sx before its
-    // writes (a caller running several groups per workgroup puts a barrier
-    // between them)
+    __syncthreads();
   }
 }
 
@@ -903,17 +907,15 @@ __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int gr
 template <class F, int NT, bool kStage = false>
 __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
                                                typename F::R* sx, uint32_t* stg,
-                                               FaceRecF<F>* srec,
                                                ReconRec* stage_lds = nullptr) {
-  face_flux_group<F, NT, kStage>(a, xcd_band(bid, nb), sx, stg, srec, stage_lds);
+  face_flux_group<F, NT, kStage>(a, xcd_band(bid, nb), sx, stg, stage_lds);
 }
 
 template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
-  __shared__ FaceRecF<F> srec[face_rec_lds<F>()];
-  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, srec);
+  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
 }
 
 // The same with staged record reads (per-column reconstruction order).
@@ -921,9 +923,8 @@ template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux_staged(FaceFluxArgs<F> a) {
   __shared__ typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ uint32_t stg[kFaceGroup * 64];
-  __shared__ FaceRecF<F> srec[face_rec_lds<F>()];
   __shared__ ReconRec stage[kFaceGroup * kStageRecs];
-  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, srec, stage);
+  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, stage);
 }
 
 // ---------------------------------------------------------------------------

@@ -244,21 +244,21 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
-    SweepArgs<F> a{nx, nz, nx + 1, 1, (uint32_t)fs.nup, 1, 0, 1, 0, wet_u, rec, P,
+    SweepArgs<F> a{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec, P,
                    (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
-    k_neutral_sweep<F><<<nb, bs, 0, st>>>(a, m);
+    k_neutral_sweep<F><<<nb, bs, (bs / 64) * kSweepStageBytes, st>>>(a, m);
     tm.end(slot, st);
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
-    SweepArgs<F> a{nx, nz, nx, 1, (uint32_t)fs.nvp, 0, 1, 0, 1, wet_v, rec, P,
+    SweepArgs<F> a{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec, P,
                    (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
-    k_neutral_sweep<F><<<nb, bs, 0, st>>>(a, m);
+    k_neutral_sweep<F><<<nb, bs, (bs / 64) * kSweepStageBytes, st>>>(a, m);
     tm.end(slot, st);
   }
   HIP_TRY(hipGetLastError());
@@ -343,7 +343,7 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns; fu.nzc = nz;
   fu.wall0 = p.ww ? p.nghost : -1;
   fu.wall1 = p.we ? p.nghost + p.nxp : -1;
-  fu.nf = (uint32_t)nu; fu.fa = 1; fu.fb = (uint32_t)fs.nup; fu.nfp = (uint32_t)fs.nup;
+  fu.nf = (uint32_t)nu; fu.fa = face_fa(ns); fu.fb = face_fb(ns); fu.nfp = (uint32_t)fs.nup;
   fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu;
   fu.rec = (const FaceRecF<F>*)fs.u; fu.recon = recon;
   fu.tgt = sc.tgt_u; fu.flx = (R*)sc.flx_u;
@@ -352,7 +352,7 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns; fv.nzc = nz;
   fv.wall0 = p.ws ? p.nghost : -1;
   fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
-  fv.nf = (uint32_t)nv; fv.fa = 1; fv.fb = (uint32_t)fs.nvp; fv.nfp = (uint32_t)fs.nvp;
+  fv.nf = (uint32_t)nv; fv.fa = face_fa(ns); fv.fb = face_fb(ns); fv.nfp = (uint32_t)fs.nvp;
   fv.dt = p.dt; fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv;
   fv.rec = (const FaceRecF<F>*)fs.v; fv.recon = recon;
   fv.tgt = sc.tgt_v; fv.flx = (R*)sc.flx_v;
@@ -446,7 +446,7 @@ int export_coeffs(const FaceState& fs, int nx, int ny, int nz, const redi_coeffs
                   bool all_faces, bool with_khtr, hipStream_t st) {
   const int ns = 2 * nz + 2;
   const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
-  const uint32_t ua = 1, ub = (uint32_t)fs.nup, va = 1, vb = (uint32_t)fs.nvp;
+  const uint32_t ua = face_fa(ns), ub = face_fb(ns), va = face_fa(ns), vb = face_fb(ns);
   const FaceRecF<F>* ru = (const FaceRecF<F>*)fs.u;
   const FaceRecF<F>* rv = (const FaceRecF<F>*)fs.v;
   CoeffArrays cu{d.uPoL, d.uPoR, d.uhEff, d.uKoL, d.uKoR};
