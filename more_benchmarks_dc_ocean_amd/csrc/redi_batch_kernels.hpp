@@ -130,8 +130,8 @@ __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconIt
 template <class F, int NT, bool kStage>
 __global__ void __launch_bounds__(64 * kFaceGroup) __attribute__((amdgpu_waves_per_eu(4)))
     kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
-  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ uint32_t stg[kFaceGroup * 64];
+  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
   __shared__ ReconRec stage[kStage ? kFaceGroup * kStageRecs : 1];
   for_each_block<true>(v, [&](const FaceFluxArgs<F>& a, int bid, int nb) {
     face_flux_body<F, NT, kStage>(a, bid, nb, sx, stg, stage);

@@ -886,19 +886,27 @@ __device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int gr
         x[t] = (R)((d * hEff) * coef);
       }
     }
+    // the chunk's rows in LDS in their global order: row rr holds the
+    // group's faces q, each with its NT fluxes
 #pragma unroll
-    for (int t = 0; t < NT; ++t) sx[(wv * 64 + lane) * NT + t] = x[t];
-    stg[wv * 64 + lane] = packed;
+    for (int t = 0; t < NT; ++t) sx[(lane * kFaceGroup + wv) * NT + t] = x[t];
+    stg[lane * kFaceGroup + wv] = packed;
     __syncthreads();
-    // row rr of the chunk, face q of the group
-    const int rr = (int)(threadIdx.x / kFaceGroup), q = (int)(threadIdx.x % kFaceGroup);
-    const int so = c0 + rr;
-    if (rr < kChunk && so < nsub) {
-      const size_t o = (size_t)so * a.nfp + f0 + q;
-#pragma unroll
-      for (int t = 0; t < NT; ++t) a.flx[o * NT + t] = sx[(q * 64 + rr) * NT + t];
-      a.tgt[o] = stg[q * 64 + rr];
+    // each row (kFaceGroup faces x NT fluxes, a multiple of 16 bytes) is one
+    // contiguous stretch of the surface-major flux array: the workgroup
+    // writes the rows as 16-byte words, consecutive threads taking
+    // consecutive words, and each row's packed targets as one word
+    const int rows = nsub - c0 < kChunk ? nsub - c0 : kChunk;
+    constexpr int kRowWords = (int)(kFaceGroup * NT * sizeof(R) / sizeof(uint4));
+    const uint4* sxw = reinterpret_cast<const uint4*>(sx);
+    for (int k = (int)threadIdx.x; k < rows * kRowWords; k += 64 * kFaceGroup) {
+      const int row = k / kRowWords, w = k % kRowWords;
+      uint4* dst = reinterpret_cast<uint4*>(a.flx + ((size_t)(c0 + row) * a.nfp + f0) * NT);
+      dst[w] = sxw[k];
     }
+    if ((int)threadIdx.x < rows)
+      *reinterpret_cast<uint4*>(a.tgt + (size_t)(c0 + threadIdx.x) * a.nfp + f0) =
+          reinterpret_cast<const uint4*>(stg)[threadIdx.x];
     __syncthreads();
   }
 }
@@ -913,16 +921,16 @@ __device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid
 
 template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux(FaceFluxArgs<F> a) {
-  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ uint32_t stg[kFaceGroup * 64];
+  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
   face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
 }
 
 // The same with staged record reads (per-column reconstruction order).
 template <class F, int NT>
 __global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux_staged(FaceFluxArgs<F> a) {
-  __shared__ typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ uint32_t stg[kFaceGroup * 64];
+  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
+  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
   __shared__ ReconRec stage[kFaceGroup * kStageRecs];
   face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, stage);
 }
