@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconIt
 // Every batch item uses the per-column reconstruction order, so the face
 // fluxes may read their records through LDS (kStage).
 template <class F, int NT, bool kStage>
-__global__ void __launch_bounds__(64 * kFaceGroup) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(64 * kFaceGroup, kFaceWaves)
     kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
   __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
   __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
