@@ -732,10 +732,13 @@ constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
 constexpr int kFaceGroup = 4;
 constexpr int kChunk = 63;  // sublayers per pass of a wavefront (64 surfaces)
 // A face wave gathers the reconstruction records of kFaceSub tracers at a
-// time, which fits the kernel in 96 VGPRs and kFaceWaves waves per SIMD
-// (measured on MI355X: the FP32-store kernel 8% faster than holding all four
-// tracers' records at four waves per SIMD, the FP64 kernel the same).
-constexpr int kFaceSub = 2, kFaceWaves = 5;
+// time, which fits the kernel in 80 VGPRs and kFaceWaves waves per SIMD.
+// The kernel's cost is mostly the latency of its dependent loads, so more
+// waves beat more loads in flight per wave.  Measured on MI355X at four
+// tracers: holding all four tracers' records, 126 VGPRs, four waves; two at
+// a time, 96, five waves; one at a time, six waves, 17% (FP64) and 12%
+// (FP32-store) faster than four; seven waves spill.
+constexpr int kFaceSub = 1, kFaceWaves = 6;
 
 template <class F>
 struct FaceFluxArgs {
