@@ -973,6 +973,18 @@ struct CellFluxArgs {
   double* tend[kMaxNT];            // optional: the FP64 tendency dTr / areaT per cell
 };
 
+// A pointer the compiler cannot trace to a kernel argument (the batched
+// kernels read theirs from a work table) is dereferenced with flat
+// instructions, which also count against the LDS wait counter: every LDS
+// wait then waits for all loads in flight.  Every device array here is in
+// global memory; pointers of this type keep the accesses global.
+template <class T>
+using Global = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ Global<T>* global_ptr(T* p) {
+  return (Global<T>*)p;
+}
+
 // Layers an accumulator column of k_cell_flux holds.
 constexpr int kCellKCap[4] = {32, 76, 128, 256};
 
@@ -990,7 +1002,9 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
                                                double* acc) {
   using R = typename F::R;
   constexpr int kCells = 64 / NT;  // cells per wavefront
-  constexpr int kBatch = 32;       // sublayer rows in flight per face
+  // sublayer rows per batch: columns of at most 32 layers (65 sublayers)
+  // take a face in two batches of 32, deeper ones stream batches of 16
+  constexpr int kBatch = KCAP <= 32 ? 32 : 16;
   const int lane = (int)threadIdx.x;
   const int cw = lane / NT, t = lane % NT;
   const int segs = (a.nx + kCells - 1) / kCells;
@@ -1020,8 +1034,8 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
     const bool left = (f & 1) != 0;
     const int shift = left ? kTargetShiftL : kTargetShiftR;
     const size_t nf = f < 2 ? a.nu : a.nv;
-    const uint32_t* tg = (f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
-    const R* fx = (f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
+    const Global<const uint32_t>* tg = global_ptr(f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
+    const Global<const R>* fx = global_ptr(f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
     // dTr(K) of the run of consecutive contributions to one layer K stays
     // in a register; it is read from / written back to LDS when K changes,
     // so every addition still lands in reference order
@@ -1053,28 +1067,42 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
       }
     };
     // two batches of rows in registers: the next batch's loads are in
-    // flight while the current one is added (one wave per SIMD, so the
-    // registers are free and nothing else hides the latency)
+    // flight while the current one is added (one wave per SIMD, so nothing
+    // else hides the latency).  Two batches of 16 are 64 loads, about the
+    // 63 a wave's load counter tracks.  In the steady loop the loads are issued
+    // unconditionally (no branch around them), so that the wait for one
+    // batch never has to wait for the batch behind it (measured on MI355X:
+    // 10.8 against 11.8 ms with conditional loads of 32-row batches, FP64);
+    // the last batches, which may be past the last sublayer, come after it
     uint32_t pa[kBatch], pb[kBatch];
     R xa[kBatch], xb[kBatch];
     load(0, pa, xa);
-    for (int s0 = 0; s0 < nsub; s0 += 2 * kBatch) {
-      const bool second = s0 + kBatch < nsub;
-      if (second) load(s0 + kBatch, pb, xb);
+    int s0 = 0;
+    for (; s0 + 2 * kBatch < nsub; s0 += 2 * kBatch) {
+      load(s0 + kBatch, pb, xb);
       add(s0, pa, xa);
-      if (s0 + 2 * kBatch < nsub) load(s0 + 2 * kBatch, pa, xa);
-      if (second) add(s0 + kBatch, pb, xb);
+      load(s0 + 2 * kBatch, pa, xa);
+      add(s0 + kBatch, pb, xb);
+    }
+    if (s0 + kBatch < nsub) {  // the last one or two batches
+      load(s0 + kBatch, pb, xb);
+      add(s0, pa, xa);
+      add(s0 + kBatch, pb, xb);
+    } else {
+      add(s0, pa, xa);
     }
     if (curK) acc[(curK - 1) * 64 + lane] = cur;
   }
   const size_t n2 = (size_t)nx * ny;
   const size_t c = (size_t)ci + (size_t)nx * cj;
-  const double iaij = 1.0 / a.areaT[c];
-  R* hp = a.htr[0];
-  double* tp = a.tend[0];
+  const double iaij = 1.0 / global_ptr(a.areaT)[c];
+  R* hq = a.htr[0];
+  double* tq = a.tend[0];
 #pragma unroll
   for (int q = 1; q < NT; ++q)
-    if (t == q) { hp = a.htr[q]; tp = a.tend[q]; }
+    if (t == q) { hq = a.htr[q]; tq = a.tend[q]; }
+  Global<R>* hp = global_ptr(hq);
+  Global<double>* tp = global_ptr(tq);
   for (int K = 1; K <= nz; ++K) {
     const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
     const double d = acc[(K - 1) * 64 + lane] * iaij;
