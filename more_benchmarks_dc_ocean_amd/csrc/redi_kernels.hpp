@@ -1003,7 +1003,8 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
   using R = typename F::R;
   constexpr int kCells = 64 / NT;  // cells per wavefront
   // sublayer rows per batch: columns of at most 32 layers (65 sublayers)
-  // take a face in two batches of 32, deeper ones stream batches of 16
+  // take a face in two or three batches of 32, deeper ones stream batches
+  // of 16
   constexpr int kBatch = KCAP <= 32 ? 32 : 16;
   const int lane = (int)threadIdx.x;
   const int cw = lane / NT, t = lane % NT;
@@ -1077,19 +1078,29 @@ __device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid
     uint32_t pa[kBatch], pb[kBatch];
     R xa[kBatch], xb[kBatch];
     load(0, pa, xa);
-    int s0 = 0;
-    for (; s0 + 2 * kBatch < nsub; s0 += 2 * kBatch) {
-      load(s0 + kBatch, pb, xb);
-      add(s0, pa, xa);
-      load(s0 + 2 * kBatch, pa, xa);
-      add(s0 + kBatch, pb, xb);
-    }
-    if (s0 + kBatch < nsub) {  // the last one or two batches
-      load(s0 + kBatch, pb, xb);
-      add(s0, pa, xa);
-      add(s0 + kBatch, pb, xb);
+    if constexpr (KCAP <= 32) {  // at most three batches: no steady loop
+      for (int s0 = 0; s0 < nsub; s0 += 2 * kBatch) {
+        const bool second = s0 + kBatch < nsub;
+        if (second) load(s0 + kBatch, pb, xb);
+        add(s0, pa, xa);
+        if (s0 + 2 * kBatch < nsub) load(s0 + 2 * kBatch, pa, xa);
+        if (second) add(s0 + kBatch, pb, xb);
+      }
     } else {
-      add(s0, pa, xa);
+      int s0 = 0;
+      for (; s0 + 2 * kBatch < nsub; s0 += 2 * kBatch) {
+        load(s0 + kBatch, pb, xb);
+        add(s0, pa, xa);
+        load(s0 + 2 * kBatch, pa, xa);
+        add(s0 + kBatch, pb, xb);
+      }
+      if (s0 + kBatch < nsub) {  // the last one or two batches
+        load(s0 + kBatch, pb, xb);
+        add(s0, pa, xa);
+        add(s0 + kBatch, pb, xb);
+      } else {
+        add(s0, pa, xa);
+      }
     }
     if (curK) acc[(curK - 1) * 64 + lane] = cur;
   }
