@@ -174,17 +174,20 @@ int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
  * parity result with the reference is for the default. */
 int redi_set_slope_taper(redi_t *self, double slope_max);
 
-/* Order in which the sweep's face tiles and the cell-flux row segments are
- * numbered (and so handed to the 8 XCDs in contiguous bands): along the
- * rows (0, default) or down the rows first (1), which puts vertical
- * neighbours -- sharing v faces and columns -- on one L2 one after the
- * other.  Changes only the schedule, never the result. */
+/* Order in which the sweep's face tiles are numbered (and so handed to
+ * the 8 XCDs in contiguous bands): along the rows (0, default) or down the
+ * rows first (1), which puts vertical neighbours -- sharing columns -- on
+ * one L2 one after the other.  Changes only the schedule, never the
+ * result. */
 int redi_set_tile_order(redi_t *self, int vertical);
 
-/* Face-flux record reads staged through LDS per chunk of surfaces (1) or
- * gathered lane by lane (0, default).  Applies to the per-column
- * reconstruction order.  Changes only the schedule, never the result. */
-int redi_set_face_staging(redi_t *self, int on);
+/* Cells per run of the Phase-B flux kernels (1..4096, default 32): one
+ * wavefront walks a run of cells along a row (u faces) or up a column
+ * (v faces), face by face, and adds each face's sublayer fluxes into its
+ * two cells in the reference's order.  A run's two end faces are also
+ * computed by the neighbouring runs.  Changes only the schedule, never
+ * the result. */
+int redi_set_flux_segment(redi_t *self, int cells);
 
 /* Layers per workgroup of the per-column tiled reconstruction (8, 16 or 32,
  * default 16; the workgroup takes 256 / kt columns).  Changes only the
@@ -251,14 +254,15 @@ long long redi_state_bytes_form(int nx_total, int ny_total, int nz, int form);
  * by HIP events recorded on the launch's own stream.  redi_timing_collect
  * waits for those events, adds their elapsed times per kernel kind into
  * `out` and forgets them. */
-#define REDI_NKERNELS 7
+#define REDI_NKERNELS 8
 #define REDI_K_COLUMN_STATE 0
 #define REDI_K_SWEEP_U 1
 #define REDI_K_SWEEP_V 2
 #define REDI_K_FACE_KHTR 3
 #define REDI_K_TRACER_RECON 4
-#define REDI_K_CELL_FLUX 5   /* flux divergence per cell */
-#define REDI_K_FACE_FLUX 6   /* sublayer fluxes per face */
+#define REDI_K_FLUX_X 5         /* u-face fluxes, added as W and E contributions */
+#define REDI_K_FLUX_Y 6         /* v-face fluxes, added as S and N contributions */
+#define REDI_K_TRACER_UPDATE 7  /* hTr += dTr / areaT */
 typedef struct {
     double ms[REDI_NKERNELS];           /* summed kernel time */
     long long launches[REDI_NKERNELS];  /* launches timed */

@@ -21,9 +21,9 @@ Each term is charged to the kernel that moves it:
     k_neutral_sweep_u/v coefficient write of its face grid, wet of that grid
     k_face_khtr         khtr_u / khtr_v
     k_tracer_recon      the group's tracer reads (one launch per tracer group)
-    k_face_flux         coefficient read-back and dy_cu/idxCu (dx_cv/idyCv)
+    k_flux_x / k_flux_y coefficient read-back and dy_cu/idxCu (dx_cv/idyCv)
                         of its face grid, shared over the tracer groups
-    k_cell_flux         the group's tracer writes, areaT shared over groups
+    k_tracer_update     the group's tracer writes, areaT shared over groups
 
 so the launches of one step add up to the step total exactly.
 """
@@ -77,18 +77,17 @@ def step_launches(case, group: int, form: str = "fp64") -> Dict[str, list]:
         out["k_neutral_sweep_u"] = [coeff_bytes(d["Nu"], d["ns"], form) + 8.0 * d["Nu"]]
     if d["ny"] > 1:
         out["k_neutral_sweep_v"] = [coeff_bytes(d["Nv"], d["ns"], form) + 8.0 * d["Nv"]]
-    ff, cf, rc = [], [], []
+    fx, fy, cf, rc = [], [], [], []
     for gn in groups:
         share = gn / nt
         rc.append(s * d["N3"] * gn)
-        if d["nx"] > 1:
-            ff.append(share * (coeff_bytes(d["Nu"], d["ns"], form) + 16.0 * d["Nu"]))
-        if d["ny"] > 1:
-            ff.append(share * (coeff_bytes(d["Nv"], d["ns"], form) + 16.0 * d["Nv"]))
+        fx.append(share * (coeff_bytes(d["Nu"], d["ns"], form) + 16.0 * d["Nu"]) if d["nx"] > 1 else 0.0)
+        fy.append(share * (coeff_bytes(d["Nv"], d["ns"], form) + 16.0 * d["Nv"]) if d["ny"] > 1 else 0.0)
         cf.append(s * d["N3"] * gn + share * 8.0 * d["N2"])
     out["k_tracer_recon"] = rc
-    out["k_face_flux"] = ff
-    out["k_cell_flux"] = cf
+    out["k_flux_x"] = fx
+    out["k_flux_y"] = fy
+    out["k_tracer_update"] = cf
     return out
 
 

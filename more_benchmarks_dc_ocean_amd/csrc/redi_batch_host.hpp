@@ -13,7 +13,8 @@
 //            built in pinned host memory and copied to the device in stream
 //            order before the first step after a change
 //   graph    the step's launches -- column state, sweep, face diffusivities,
-//            reconstruction, face fluxes, cell fluxes per accumulator depth --
+//            reconstruction, x and y fluxes per layer-capacity class, tracer
+//            update --
 //            captured once into a hipGraph: the persistent grids do not depend
 //            on the batch, so table changes never invalidate it
 #pragma once
@@ -103,7 +104,10 @@ DomainLayout domain_layout(int nx, int ny, int nz, int form, int group) {
   return L;
 }
 
-enum BatchNode { kNodeColumn, kNodeSweep, kNodeKhtr, kNodeRecon, kNodeFace, kNodeCell0, kNodes = kNodeCell0 + 4 };
+enum BatchNode {
+  kNodeColumn, kNodeSweep, kNodeKhtr, kNodeRecon,
+  kNodeFluxX0, kNodeFluxY0 = kNodeFluxX0 + 4, kNodeUpdate = kNodeFluxY0 + 4, kNodes
+};
 
 struct NodeTable {
   size_t count = 0, start = 0, nblk = 0, items = 0;  // byte offsets in the table buffer
@@ -163,8 +167,8 @@ struct BatchOps {
       case kNodeSweep: return sizeof(SweepItem<F>);
       case kNodeKhtr: return sizeof(KhtrArgs);
       case kNodeRecon: return sizeof(Recon);
-      case kNodeFace: return sizeof(FaceFluxArgs<F>);
-      default: return sizeof(CellFluxArgs<F>);
+      case kNodeUpdate: return sizeof(UpdateArgs<F>);
+      default: return sizeof(FluxArgs<F>);
     }
   }
 
@@ -241,25 +245,27 @@ struct BatchOps {
       w[kNodeKhtr].add(KhtrArgs{nu, nv, p.khtr_u_ext, p.khtr_v_ext, p.khtr, d.fs.khtr_u, d.fs.khtr_v},
                        blocks_for((size_t)nu + nv, 256));
       const PhaseBScratch sc = carve_phase_b(d.ws, nx, ny, nz, NT, form_sizes<F>());
-      FaceFluxArgs<F> fu, fv;
-      CellFluxArgs<F> ca;
-      phase_b_args<F>(p, d.fs, sc, fu, fv, ca);
-      ca.vertical = d.tiles.vertical;
+      FluxArgs<F> fx, fy;
+      UpdateArgs<F> ua;
+      unsigned nbx = 0, nby = 0;
+      phase_b_args<F>(p, d.fs, sc, d.tiles.seg, fx, fy, ua, &nbx, &nby);
       TracerSetT<R> ts{};
       for (int t = 0; t < kMaxNT; ++t) {
         // slots past the domain's tracers repeat its last tracer; their
         // results land in the discard field
         ts.q[t] = tr[std::min(t, d.nt - 1)];
-        ca.htr[t] = t < d.nt ? tr[t] : (R*)sc.discard;
-        ca.tend[t] = nullptr;
+        ua.htr[t] = t < d.nt ? tr[t] : (R*)sc.discard;
+        ua.tend[t] = nullptr;
       }
+      ua.nt = NT;
       const ReconLayout ry = recon_layout(0, N2, nz, NT);
-      fu.ry = fv.ry = ry;
+      fx.ry = fy.ry = ry;
       w[kNodeRecon].add(Recon{ReconArgs<F, NT>{N2, nz, h, ts, (ReconRec*)sc.recon, ry}, d.tiles.recon_kt, 0},
                         recon_tiled_blocks(N2, nz, d.tiles.recon_kt));
-      if (nx > 1) w[kNodeFace].add(fu, (uint32_t)(d.fs.nup / kFaceGroup));
-      if (ny > 1) w[kNodeFace].add(fv, (uint32_t)(d.fs.nvp / kFaceGroup));
-      w[kNodeCell0 + cell_kcap(nz)].add(ca, cell_flux_blocks<NT>(nx, ny));
+      const int fc = flux_class(nz);
+      w[kNodeFluxX0 + fc].add(fx, nbx);
+      w[kNodeFluxY0 + fc].add(fy, nby);
+      w[kNodeUpdate].add(ua, update_blocks(N2, nz));
     }
     for (int k = 0; k < kNodes; ++k) w[k].finish();
   }
@@ -278,12 +284,15 @@ struct BatchOps {
     b->grid[kNodeSweep] = persistent_grid(b, kb_neutral_sweep<F>, kBatchSweepLanes, kNodeSweep);
     b->grid[kNodeKhtr] = persistent_grid(b, kb_face_khtr, 256, kNodeKhtr);
     b->grid[kNodeRecon] = persistent_grid(b, kb_tracer_recon<F, NT>, kReconLanes, kNodeRecon);
-    b->grid[kNodeFace] = persistent_grid(b, kb_face_flux<F, NT, kFaceStageDefault>, 64 * kFaceGroup,
-                                         kNodeFace);
-    b->grid[kNodeCell0 + 0] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[0]>, 64, kNodeCell0 + 0);
-    b->grid[kNodeCell0 + 1] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[1]>, 64, kNodeCell0 + 1);
-    b->grid[kNodeCell0 + 2] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[2]>, 64, kNodeCell0 + 2);
-    b->grid[kNodeCell0 + 3] = persistent_grid(b, kb_cell_flux<F, NT, kCellKCap[3]>, 64, kNodeCell0 + 3);
+    b->grid[kNodeFluxX0 + 0] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[0], false>, 64, kNodeFluxX0 + 0);
+    b->grid[kNodeFluxX0 + 1] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[1], false>, 64, kNodeFluxX0 + 1);
+    b->grid[kNodeFluxX0 + 2] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[2], false>, 64, kNodeFluxX0 + 2);
+    b->grid[kNodeFluxX0 + 3] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[3], false>, 64, kNodeFluxX0 + 3);
+    b->grid[kNodeFluxY0 + 0] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[0], true>, 64, kNodeFluxY0 + 0);
+    b->grid[kNodeFluxY0 + 1] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[1], true>, 64, kNodeFluxY0 + 1);
+    b->grid[kNodeFluxY0 + 2] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[2], true>, 64, kNodeFluxY0 + 2);
+    b->grid[kNodeFluxY0 + 3] = persistent_grid(b, kb_flux<F, NT, kFluxNzc[3], true>, 64, kNodeFluxY0 + 3);
+    b->grid[kNodeUpdate] = persistent_grid(b, kb_tracer_update<F>, kUpdLanes, kNodeUpdate);
   }
 
   // The step's launches on `st` (captured into the graph).  A kernel kind
@@ -298,18 +307,27 @@ struct BatchOps {
     if (live(kNodeKhtr)) kb_face_khtr<<<g[kNodeKhtr], 256, 0, st>>>(view<KhtrArgs>(b, kNodeKhtr));
     if (live(kNodeRecon))
       kb_tracer_recon<F, NT><<<g[kNodeRecon], kReconLanes, 0, st>>>(view<Recon>(b, kNodeRecon));
-    if (live(kNodeFace))
-      kb_face_flux<F, NT, kFaceStageDefault><<<g[kNodeFace], 64 * kFaceGroup, 0, st>>>(
-          view<FaceFluxArgs<F>>(b, kNodeFace));
-    using CA = CellFluxArgs<F>;
-    if (live(kNodeCell0 + 0))
-      kb_cell_flux<F, NT, kCellKCap[0]><<<g[kNodeCell0 + 0], 64, 0, st>>>(view<CA>(b, kNodeCell0 + 0));
-    if (live(kNodeCell0 + 1))
-      kb_cell_flux<F, NT, kCellKCap[1]><<<g[kNodeCell0 + 1], 64, 0, st>>>(view<CA>(b, kNodeCell0 + 1));
-    if (live(kNodeCell0 + 2))
-      kb_cell_flux<F, NT, kCellKCap[2]><<<g[kNodeCell0 + 2], 64, 0, st>>>(view<CA>(b, kNodeCell0 + 2));
-    if (live(kNodeCell0 + 3))
-      kb_cell_flux<F, NT, kCellKCap[3]><<<g[kNodeCell0 + 3], 64, 0, st>>>(view<CA>(b, kNodeCell0 + 3));
+    using FA = FluxArgs<F>;
+    // every x class before any y class: the y kernels read the dTr partials
+    // the x kernels wrote (stream order)
+    if (live(kNodeFluxX0 + 0))
+      kb_flux<F, NT, kFluxNzc[0], false><<<g[kNodeFluxX0 + 0], 64, 0, st>>>(view<FA>(b, kNodeFluxX0 + 0));
+    if (live(kNodeFluxX0 + 1))
+      kb_flux<F, NT, kFluxNzc[1], false><<<g[kNodeFluxX0 + 1], 64, 0, st>>>(view<FA>(b, kNodeFluxX0 + 1));
+    if (live(kNodeFluxX0 + 2))
+      kb_flux<F, NT, kFluxNzc[2], false><<<g[kNodeFluxX0 + 2], 64, 0, st>>>(view<FA>(b, kNodeFluxX0 + 2));
+    if (live(kNodeFluxX0 + 3))
+      kb_flux<F, NT, kFluxNzc[3], false><<<g[kNodeFluxX0 + 3], 64, 0, st>>>(view<FA>(b, kNodeFluxX0 + 3));
+    if (live(kNodeFluxY0 + 0))
+      kb_flux<F, NT, kFluxNzc[0], true><<<g[kNodeFluxY0 + 0], 64, 0, st>>>(view<FA>(b, kNodeFluxY0 + 0));
+    if (live(kNodeFluxY0 + 1))
+      kb_flux<F, NT, kFluxNzc[1], true><<<g[kNodeFluxY0 + 1], 64, 0, st>>>(view<FA>(b, kNodeFluxY0 + 1));
+    if (live(kNodeFluxY0 + 2))
+      kb_flux<F, NT, kFluxNzc[2], true><<<g[kNodeFluxY0 + 2], 64, 0, st>>>(view<FA>(b, kNodeFluxY0 + 2));
+    if (live(kNodeFluxY0 + 3))
+      kb_flux<F, NT, kFluxNzc[3], true><<<g[kNodeFluxY0 + 3], 64, 0, st>>>(view<FA>(b, kNodeFluxY0 + 3));
+    if (live(kNodeUpdate))
+      kb_tracer_update<F><<<g[kNodeUpdate], kUpdLanes, 0, st>>>(view<UpdateArgs<F>>(b, kNodeUpdate));
   }
 
   static int capture(redi_batch_s* b) {
@@ -476,7 +494,7 @@ int redi_batch_create(redi_batch_t** out, int form, int group, int max_domains, 
   size_t off = 0;
   for (int k = 0; k < kNodes; ++k) {
     NodeTable& t = b->tab[k];
-    t.cap = (k == kNodeSweep || k == kNodeFace) ? 2 * max_domains : max_domains;
+    t.cap = k == kNodeSweep ? 2 * max_domains : max_domains;
     t.item_size = (size_t)with_batch_ops(form, group, [&](auto ops) {
       return (int)decltype(ops)::item_size(k);
     });
@@ -548,7 +566,6 @@ int redi_batch_add(redi_batch_t* b, const redi_domain_t* d, void* stream) {
   x.tiles.recon_kt = d->kchunk ? d->kchunk : kReconChunkDefault;
   x.tiles.group = b->group;
   x.tiles.recon_layer_major = 0;
-  x.tiles.face_stage = kFaceStageDefault;
   char* base = b->arena_base + off;
   x.off = off; x.bytes = L.total;
   x.fs.nu = (size_t)(nx + 1) * ny; x.fs.nv = (size_t)nx * (ny + 1);

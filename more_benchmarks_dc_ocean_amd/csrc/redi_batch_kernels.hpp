@@ -125,26 +125,22 @@ __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconIt
   });
 }
 
-// Every batch item uses the per-column reconstruction order, so the face
-// fluxes may read their records through LDS (kStage).
-template <class F, int NT, bool kStage>
-__global__ void __launch_bounds__(64 * kFaceGroup, kFaceWaves)
-    kb_face_flux(BatchView<FaceFluxArgs<F>> v) {
-  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
-  __shared__ ReconRec stage[kStage ? kFaceGroup * kStageRecs : 1];
-  for_each_block<true>(v, [&](const FaceFluxArgs<F>& a, int bid, int nb) {
-    face_flux_body<F, NT, kStage>(a, bid, nb, sx, stg, stage);
+// One wavefront per workgroup (the LDS lists are the wavefront's own), so
+// no barrier between blocks.
+template <class F, int NT, int NZC, bool kY>
+__global__ void __launch_bounds__(64) kb_flux(BatchView<FluxArgs<F>> v) {
+  __shared__ FluxLds<NT, NZC> lds;
+  for_each_block<false>(v, [&](const FluxArgs<F>& a, int bid, int nb) {
+    flux_body<F, NT, NZC, kY>(a, bid, nb, lds);
   });
 }
 
-// One wavefront per workgroup and per-lane accumulator columns: no barrier
-// between blocks.
-template <class F, int NT, int KCAP>
-__global__ void __launch_bounds__(64) kb_cell_flux(BatchView<CellFluxArgs<F>> v) {
-  __shared__ double acc[KCAP * 64];
-  for_each_block<false, false>(v, [&](const CellFluxArgs<F>& a, int bid, int nb) {
-    cell_flux_body<F, NT, KCAP>(a, bid, nb, acc);
+template <class F>
+__global__ void __launch_bounds__(kUpdLanes) kb_tracer_update(BatchView<UpdateArgs<F>> v) {
+  __shared__ double tile[kUpdTileDoubles];
+  __shared__ double inv[kUpdCells];
+  for_each_block<true>(v, [&](const UpdateArgs<F>& a, int bid, int nb) {
+    tracer_update_body<F>(a, bid, nb, tile, inv);
   });
 }
 

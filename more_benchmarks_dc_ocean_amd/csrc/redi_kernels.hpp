@@ -5,27 +5,22 @@
 //   k_neutral_sweep       Phase A, once per wet face: the continuous neutral
 //                         surface sweep over the two adjacent columns.
 //   k_tracer_recon_tiled  Phase B, per tracer group: layer averages, PPM edges
-//                         and limited edge values of every column.
-//   k_face_flux           Phase B, once per face for up to NT tracers: the flux
-//                         of every contributing sublayer and its target layers.
-//   k_cell_flux           Phase B, once per cell: the four bounding faces'
-//                         sublayer fluxes added into the cell's layers in the
-//                         reference's order, then hTr += dTr / areaT.
+//                         and the PPM limiter branch of every column.
+//   k_flux (x, y)         Phase B, per tracer group and face grid: the flux of
+//                         every contributing sublayer of every face, added
+//                         into the bounding cells' dTr in the reference's
+//                         order (W, E in x; S, N in y).
+//   k_tracer_update       Phase B: hTr += dTr / areaT.
 //   k_export_coeffs       face records -> the reference's Po/Ko/hEff arrays.
 //
 // Every kernel is a template on a precision form F:
 //   Fp64  the plain form: fields, records and arithmetic in FP64, in the
 //         reference's rounding order, so results match the reference bits;
-//   Fp32  the FP32-store form: the fields h / hTr, the face records, the
-//         per-step reconstruction and the sublayer fluxes are stored in
-//         32-bit words (FP32 values, int16 layer indices, positions as
-//         32-bit fractions; the reconstruction as an FP64 layer mean plus
-//         FP32 offsets of its edge values, so differences between columns
-//         keep FP64 accuracy).  The arithmetic stays FP64: every difference
-//         of nearly equal values -- the T/S edges, EOS derivatives and
-//         density differences that place the neutral surfaces, the tracer
-//         differences across a face, the tendency accumulation -- is
-//         formed in FP64 from the stored values.
+//   Fp32  the FP32-store form: the fields h / hTr and the face records are
+//         stored in 32-bit words (FP32 values, int16 layer indices, positions
+//         as 31-bit fractions) and each sublayer flux is rounded to FP32; the
+//         column state, the reconstruction, the density differences and the
+//         tendency accumulation stay FP64.
 // Both forms come from one source: records convert through get() / put(),
 // which are identities for Fp64.
 //
@@ -36,23 +31,23 @@
 //                               interface kt (top-down, 0..nz) at
 //                               (j*(nz+1) + kt)*nx + i
 //   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
-//                               surface-major: (face, s) at s*Nf' + face with
-//                               rows padded to Nf' = a multiple of kFaceGroup
-//                               faces; one monotonicity flag per face
+//                               face-major: (face, s) at face*ns + s; one flag
+//                               byte per face
 //   tracer recon (K3 -> K4)     ReconRec (layer mean, interface values at the
-//                               layer's top and bottom; FP64 in both forms)
-//                               per layer, column and tracer of a group; each
-//                               tracer's column contiguous by default
-//   sublayer fluxes (K4 -> K5)  surface-major: packed targets [s][face] and
-//                               fluxes [s][face][tracer], rows of Nf' faces
+//                               layer's top and bottom, limiter branch; FP64
+//                               in both forms) per layer, column and tracer
+//                               of a group; each tracer's column contiguous
+//                               by default
+//   dTr (K4 -> K5)              FP64, one block of nz*NT values per cell,
+//                               (K, t) at (K-1)*NT + t
 // The sweep reads each of its two columns once, front to back; its lanes
 // are adjacent faces, which sit at similar depths, so with the column state
 // interface-major per row the lanes read neighbouring records (and the
-// column-state kernel, a lane per column, writes whole rows); its
-// surface-major face-record writes are contiguous.  The face flux
-// kernel gives each face one wavefront with a lane per neutral surface: a
-// face's targets are non-decreasing in s, so the lanes walk down the two
-// columns together and read consecutive per-column records.
+// column-state kernel, a lane per column, writes whole rows); it stages its
+// face records in LDS and stores them as whole runs.  The flux kernels give
+// a face one wavefront with a lane per neutral surface: a face's targets are
+// non-decreasing in s, so the lanes walk down the two columns together and
+// read consecutive per-column records.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -156,9 +151,11 @@ template <class F> using FaceRecF = typename F::Face;
 inline uint32_t face_fa(int ns) { return (uint32_t)ns; }
 inline uint32_t face_fb(int) { return 1u; }
 
-// Flag bits per face: Ko of the contributing sublayers (hEff != 0) is
-// non-decreasing in s on the left (bit 0) / right (bit 1) column.
-constexpr uint8_t kMonoL = 1, kMonoR = 2;
+// Flag bits per face (written by the sweep): Ko of the contributing
+// sublayers (hEff != 0) is non-decreasing in s on the left (bit 0) / right
+// (bit 1) column; bit 2: the face has a contributing sublayer, so the flux
+// kernels skip faces without one (dry faces, whose records are inert).
+constexpr uint8_t kMonoL = 1, kMonoR = 2, kHasFlux = 4;
 
 // Thread -> cell/face mapping over TX x TY tiles of a 2-D index space, with
 // tiles handed to the 8 XCDs in contiguous bands: workgroups are dispatched
@@ -396,7 +393,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
 
   int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
   double lastP_l = 0.0, lastP_r = 0.0, lastPos_l = 0.0, lastPos_r = 0.0;
-  bool reached_bottom = false, search_l = false, mono_l = true, mono_r = true;
+  bool reached_bottom = false, search_l = false, mono_l = true, mono_r = true, has_flux = false;
   SurfVals pend{};
   for (int ks = 1; ks <= ns; ++ks) {
    if (wet) {
@@ -493,6 +490,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         hm = slope_taper(hm, lastPos_l, posL, lastPos_r, posR, a.pa_to_h, a.inv_dx[fo], a.slope_max);
       pend.hEff = Face::stored_heff(hm);
       if (pend.hEff != 0.0) {
+        has_flux = true;
         mono_l = mono_l && pend.KoL >= contK_l;
         mono_r = mono_r && pend.KoR >= contK_r;
         contK_l = pend.KoL;
@@ -510,7 +508,9 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   }
   put(ns - 1, wet ? pend : inert);
   flush(ns - 1 - (ns - 1) % kRun, (ns - 1) % kRun + 1);
-  if (inr) a.mono[fo] = wet ? (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0)) : (uint8_t)(kMonoL | kMonoR);
+  if (inr)
+    a.mono[fo] = wet ? (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0) | (has_flux ? kHasFlux : 0))
+                     : (uint8_t)(kMonoL | kMonoR);
 }
 
 // Dynamic LDS: kSweepStageBytes per wave of the block.
@@ -694,8 +694,9 @@ inline unsigned recon_tiled_blocks(int N2, int nz, int kt) {
 }
 
 // ---------------------------------------------------------------------------
-// K4: sublayer fluxes, once per face (redi_face_flux + redi_sublayer_dT,
-// :893-925, :1037-1092)
+// K4: sublayer fluxes of one face grid and their divergence into the cells
+// (redi_face_flux + redi_sublayer_dT, :893-925, :1037-1092, accumulated as
+// redi_apply_flux_impl does, :1094-1181)
 // ---------------------------------------------------------------------------
 
 // Pieces of redi_sublayer_dT (:893-925).  The interface value at position
@@ -721,258 +722,6 @@ __device__ __forceinline__ R guarded_dT(R tlt, R tlb, R trt, R trb, R tlay, R tr
 
 constexpr int kMaxNT = 4;
 
-// A face's sublayer ks contributes when hEff(ks) != 0, to layer KoL(ks) of
-// its left column and KoR(ks) of its right column (1-based, top-down, at
-// most 256).  Its targets are stored packed as KoL | KoR << 16; 0 marks a
-// sublayer that contributes nothing.
-constexpr int kTargetShiftL = 0, kTargetShiftR = 16;
-
-// The fluxes of one face grid (u or v).  One wavefront per face, a
-// workgroup takes kFaceGroup consecutive faces.
-constexpr int kFaceGroup = 4;
-constexpr int kChunk = 63;  // sublayers per pass of a wavefront (64 surfaces)
-// A face wave gathers the reconstruction records of kFaceSub tracers at a
-// time, which fits the kernel in 80 VGPRs and kFaceWaves waves per SIMD.
-// The kernel's cost is mostly the latency of its dependent loads, so more
-// waves beat more loads in flight per wave.  Measured on MI355X at four
-// tracers: holding all four tracers' records, 126 VGPRs, four waves; two at
-// a time, 96, five waves; one at a time, six waves, 17% (FP64) and 12%
-// (FP32-store) faster than four; seven waves spill.
-constexpr int kFaceSub = 1, kFaceWaves = 6;
-
-template <class F>
-struct FaceFluxArgs {
-  int nfa;                // faces per row of the face array
-  int ilo, ihi, jlo, jhi; // faces computed: i in [ilo, ihi), j in [jlo, jhi) (0-based)
-  int di, dj;             // left column = (i - di, j - dj), right column = (i, j)
-  int nx, ns;             // columns per row, neutral surfaces
-  int nzc;                // layers per column (ns = 2*nzc + 2)
-  int wall0, wall1;       // 0-based face coordinate across the faces (i for u,
-                          // j for v) of the two wall faces, -1 where open
-  uint32_t nf;            // faces in the face array
-  uint32_t fa, fb;        // face record (face, s) at face*fa + s*fb
-  uint32_t nfp;           // faces per sublayer row of tgt / flx (nf rounded up to kFaceGroup)
-  double dt;
-  const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
-  const FaceRecF<F>* rec;
-  ReconLayout ry;
-  const ReconRec* recon;      // tracer t's records from recon + t*ry.ct
-  uint32_t* tgt;              // packed targets, (s, face) at s*nfp + face
-  typename F::R* flx;         // fluxes, (s, face, t) at (s*nfp + face)*NT + t
-};
-
-// flx = dT * hEff * Coef with Coef = dt*khtr*dy_cu*idxCu (v: dt*khtr*dx_cv*idyCv)
-// for every contributing sublayer, in the reference's rounding order
-// (:1060-1090), computed in FP64 in both forms and stored in the form's type.  The reference evaluates this twice per face, once from
-// each bounding cell; both evaluations are the same expression on the same
-// operands, so one evaluation serves both cells.
-// Lane l of a face's wavefront takes neutral surface s = c0 + l: it loads
-// the surface's record and, on each side, the record of the layer the
-// surface lies in, and forms the interface value there.  Sublayer s lies
-// between surfaces s and s+1, so lane l (< 63) takes the bottom-surface
-// values from lane l+1 by a wavefront shuffle; they are the same
-// expressions the reference evaluates for the sublayer's bottom.  A face's
-// targets are non-decreasing in s, so the lanes read consecutive records of
-// each column.  The kFaceGroup waves of a workgroup read one line of each
-// surface-major face-record row between them; their fluxes are staged in
-// LDS and written back as whole rows (kFaceGroup faces x NT tracers).
-// sx: kFaceGroup * 64 * NT fluxes (stored type), stg: kFaceGroup * 64
-// packed targets, both in LDS.
-// Staged record reads (kStage, per-column reconstruction order only): a
-// chunk's 64 surfaces lie in few layers of each column -- about half as many
-// as surfaces -- so instead of every lane gathering its own two records per
-// tracer, the wavefront loads the layer range [lo, hi] each side touches
-// with consecutive lanes (one coalesced pass per side and tracer) into 64
-// LDS records per side, and each lane reads its records there.  A chunk
-// whose range on either side spans more than 64 layers takes the gather.
-// The records read are the same, so are the bits.
-constexpr int kStageRecs = 128;  // LDS records per wavefront (64 per side)
-constexpr bool kFaceStageDefault = false;  // staged reads unless a caller chooses
-
-// Face group `grp` (faces grp*kFaceGroup .. +kFaceGroup-1 of the face array).
-template <class F, int NT, bool kStage = false>
-__device__ __forceinline__ void face_flux_group(const FaceFluxArgs<F>& a, int grp,
-                                                typename F::R* sx, uint32_t* stg,
-                                                ReconRec* stage_lds = nullptr) {
-  using R = typename F::R;
-  using Face = FaceRecF<F>;
-  const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const size_t f0 = (size_t)grp * kFaceGroup;
-  const size_t fo = f0 + wv;
-  const int i = (int)(fo % (size_t)a.nfa), j = (int)(fo / (size_t)a.nfa);
-  const int across = a.di ? i : j;
-  const bool live = fo < a.nf && i >= a.ilo && i < a.ihi && j >= a.jlo && j < a.jhi &&
-                    across != a.wall0 && across != a.wall1;
-  uint32_t cl = 0, cr = 0;
-  double coef = 0.0;
-  if (live) {
-    cr = (uint32_t)i + (uint32_t)a.nx * (uint32_t)j;
-    cl = cr - (uint32_t)a.di - (uint32_t)a.nx * (uint32_t)a.dj;
-    coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
-  }
-  const ReconRec* recL = a.recon + cl * a.ry.ca;
-  const ReconRec* recR = a.recon + cr * a.ry.ca;
-  const uint32_t cb = a.ry.cb, ct = a.ry.ct;
-  const size_t fb = a.fb;
-  const int ns = a.ns, nsub = ns - 1;
-  const Face* r = a.rec + (live ? fo * a.fa : 0);
-  for (int c0 = 0; c0 < nsub; c0 += kChunk) {
-    const int s = c0 + lane;
-    const bool surf = live && s < ns;
-    SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
-    if (surf) Fs = r[(size_t)s * fb].get();
-    const int KoL = Fs.KoL, KoR = Fs.KoR;
-    const double PoL = Fs.PoL, PoR = Fs.PoR;
-    bool staged = false;
-    int lo_l = 0, hi_l = 0, lo_r = 0, hi_r = 0;
-    if constexpr (kStage) {
-      // wave-uniform layer ranges of this chunk's surfaces: positions Ko + Po
-      // never decrease along the surfaces (the sweep's backstop), so every
-      // surface's layer lies within one of the first and last surfaces';
-      // a ballot confirms it, else the chunk gathers
-      const int last = (live ? (ns - c0 < 64 ? ns - c0 : 64) : 1) - 1;
-      const int kl0 = __builtin_amdgcn_readlane(KoL, 0), kl1 = __builtin_amdgcn_readlane(KoL, last);
-      const int kr0 = __builtin_amdgcn_readlane(KoR, 0), kr1 = __builtin_amdgcn_readlane(KoR, last);
-      lo_l = kl0 > 1 ? kl0 - 1 : 1;
-      hi_l = kl1 + 1;
-      lo_r = kr0 > 1 ? kr0 - 1 : 1;
-      hi_r = kr1 + 1;
-      const bool out = surf && (KoL < lo_l || KoL > hi_l || KoR < lo_r || KoR > hi_r);
-      staged = live && hi_l - lo_l < 64 && hi_r - lo_r < 64 && __ballot(out) == 0;
-      if (hi_l > a.nzc) hi_l = a.nzc;  // stage no layer past the column
-      if (hi_r > a.nzc) hi_r = a.nzc;
-    }
-    // surface s+1, the bottom of sublayer s
-    const double PoLb = __shfl_down(PoL, 1, 64), PoRb = __shfl_down(PoR, 1, 64);
-    const int KoLb = __shfl_down(KoL, 1, 64), KoRb = __shfl_down(KoR, 1, 64);
-    const bool contrib = surf && lane < kChunk && s < nsub && Fs.hEff != 0.0;
-    const uint32_t packed =
-        contrib ? (((uint32_t)KoL << kTargetShiftL) | ((uint32_t)KoR << kTargetShiftR)) : 0u;
-    const double hEff = Fs.hEff;
-    R x[NT];
-    // tracers in sub-groups of kFaceSub: a sub-group's records are gathered
-    // and its fluxes formed before the next sub-group's records are loaded,
-    // so a wave holds the records of kFaceSub tracers at a time
-#pragma unroll
-    for (int g0 = 0; g0 < NT; g0 += kFaceSub) {
-      LayerVals qL[kFaceSub], qR[kFaceSub];
-      double eL[kFaceSub], eR[kFaceSub];
-#pragma unroll
-      for (int u = 0; u < kFaceSub; ++u) {
-        const int t = g0 + u;
-        if (t >= NT) break;
-        qL[u] = LayerVals{0.0, 0.0, 0.0, 0.0, 0.0};
-        qR[u] = qL[u];
-        if (kStage && staged) {
-          ReconRec* sl = stage_lds + wv * kStageRecs;
-          if (lane <= hi_l - lo_l) sl[lane] = recL[(uint32_t)(lo_l - 1 + lane) + (uint32_t)t * ct];
-          if (lane <= hi_r - lo_r) sl[64 + lane] = recR[(uint32_t)(lo_r - 1 + lane) + (uint32_t)t * ct];
-          // a wavefront's LDS accesses execute in issue order, so only the
-          // compiler must keep the reads after the writes (and the next
-          // tracer's writes after these reads)
-          __builtin_amdgcn_wave_barrier();
-          asm volatile("" ::: "memory");
-          if (surf) {
-            qL[u] = sl[KoL - lo_l].get();
-            qR[u] = sl[64 + KoR - lo_r].get();
-          }
-          asm volatile("" ::: "memory");
-          __builtin_amdgcn_wave_barrier();
-        } else if (surf) {
-          qL[u] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
-          qR[u] = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
-        }
-        eL[u] = edge_value(qL[u], PoL);
-        eR[u] = edge_value(qR[u], PoR);
-      }
-      double eLb[kFaceSub], eRb[kFaceSub];
-#pragma unroll
-      for (int u = 0; u < kFaceSub; ++u) {
-        if (g0 + u >= NT) break;
-        eLb[u] = __shfl_down(eL[u], 1, 64);
-        eRb[u] = __shfl_down(eR[u], 1, 64);
-      }
-#pragma unroll
-      for (int u = 0; u < kFaceSub; ++u) {
-        const int t = g0 + u;
-        if (t >= NT) break;
-        x[t] = R(0);
-        if (contrib) {
-          const double tlay = ppm_ave<double>(PoL, PoLb + (double)(KoLb - KoL), qL[u].aL, qL[u].aR, qL[u].Tl);
-          const double trlay = ppm_ave<double>(PoR, PoRb + (double)(KoRb - KoR), qR[u].aL, qR[u].aR, qR[u].Tl);
-          const double d = guarded_dT<double>(eL[u], eLb[u], eR[u], eRb[u], tlay, trlay);
-          x[t] = (R)((d * hEff) * coef);
-        }
-      }
-      // the next sub-group's loads stay after this one's fluxes
-      if (g0 + kFaceSub < NT) asm volatile("" ::: "memory");
-    }
-    // the chunk's rows in LDS in their global order: row rr holds the
-    // group's faces q, each with its NT fluxes
-#pragma unroll
-    for (int t = 0; t < NT; ++t) sx[(lane * kFaceGroup + wv) * NT + t] = x[t];
-    stg[lane * kFaceGroup + wv] = packed;
-    __syncthreads();
-    // each row (kFaceGroup faces x NT fluxes, a multiple of 16 bytes) is one
-    // contiguous stretch of the surface-major flux array: the workgroup
-    // writes the rows as 16-byte words, consecutive threads taking
-    // consecutive words, and each row's packed targets as one word
-    const int rows = nsub - c0 < kChunk ? nsub - c0 : kChunk;
-    constexpr int kRowWords = (int)(kFaceGroup * NT * sizeof(R) / sizeof(uint4));
-    const uint4* sxw = reinterpret_cast<const uint4*>(sx);
-    for (int k = (int)threadIdx.x; k < rows * kRowWords; k += 64 * kFaceGroup) {
-      const int row = k / kRowWords, w = k % kRowWords;
-      uint4* dst = reinterpret_cast<uint4*>(a.flx + ((size_t)(c0 + row) * a.nfp + f0) * NT);
-      dst[w] = sxw[k];
-    }
-    if ((int)threadIdx.x < rows)
-      *reinterpret_cast<uint4*>(a.tgt + (size_t)(c0 + threadIdx.x) * a.nfp + f0) =
-          reinterpret_cast<const uint4*>(stg)[threadIdx.x];
-    __syncthreads();
-  }
-}
-
-// Block bid of nb over one face grid: consecutive groups per XCD.
-template <class F, int NT, bool kStage = false>
-__device__ __forceinline__ void face_flux_body(const FaceFluxArgs<F>& a, int bid, int nb,
-                                               typename F::R* sx, uint32_t* stg,
-                                               ReconRec* stage_lds = nullptr) {
-  face_flux_group<F, NT, kStage>(a, xcd_band(bid, nb), sx, stg, stage_lds);
-}
-
-template <class F, int NT>
-__global__ void __launch_bounds__(64 * kFaceGroup, kFaceWaves) k_face_flux(FaceFluxArgs<F> a) {
-  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
-  face_flux_body<F, NT>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg);
-}
-
-// The same with staged record reads (per-column reconstruction order).
-template <class F, int NT>
-__global__ void __launch_bounds__(64 * kFaceGroup) k_face_flux_staged(FaceFluxArgs<F> a) {
-  __shared__ alignas(16) typename F::R sx[kFaceGroup * 64 * NT];
-  __shared__ alignas(16) uint32_t stg[kFaceGroup * 64];
-  __shared__ ReconRec stage[kFaceGroup * kStageRecs];
-  face_flux_body<F, NT, true>(a, (int)blockIdx.x, (int)gridDim.x, sx, stg, stage);
-}
-
-// ---------------------------------------------------------------------------
-// K5: flux divergence per cell (redi_apply_flux_impl, :1094-1181)
-// ---------------------------------------------------------------------------
-template <class F>
-struct CellFluxArgs {
-  int nx, ny, nz;
-  int wuf_w, wuf_e, wvf_s, wvf_n;  // 1-based wall faces (:1139-1142)
-  int wall_w, wall_e, wall_s, wall_n;
-  const double* areaT;
-  const uint32_t *tgt_u, *tgt_v;
-  const typename F::R *flx_u, *flx_v;
-  uint32_t nu, nv;                 // faces per sublayer row of the u / v arrays
-  int vertical;                    // segments numbered down the rows first
-  typename F::R* htr[kMaxNT];
-  double* tend[kMaxNT];            // optional: the FP64 tendency dTr / areaT per cell
-};
-
 // A pointer the compiler cannot trace to a kernel argument (the batched
 // kernels read theirs from a work table) is dereferenced with flat
 // instructions, which also count against the LDS wait counter: every LDS
@@ -985,156 +734,325 @@ __device__ __forceinline__ Global<T>* global_ptr(T* p) {
   return (Global<T>*)p;
 }
 
-// Layers an accumulator column of k_cell_flux holds.
-constexpr int kCellKCap[4] = {32, 76, 128, 256};
+// The reference adds, for every cell, the fluxes of its four faces into
+// dTr(1:nz) in a fixed order -- W, E, S, N, each face's sublayers
+// ascending (:1136-1175) -- and FP64 addition is not associative, so that
+// order is part of the result.  Here the order is kept by walking the
+// faces in it:
+//
+//   k_flux_x  one wavefront per run of `seg` cells along a row.  It takes
+//             the row's u faces left to right; face i's right-side fluxes
+//             are cell i's W contributions (its first, added to 0) and its
+//             left-side fluxes are cell i-1's E contributions (its second),
+//             after which cell i-1's partial dTr is stored.
+//   k_flux_y  one wavefront per run of cells up a column, the column's v
+//             faces south to north: face j's right-side fluxes are cell
+//             j's S contributions (added to its stored W+E partial), its
+//             left-side fluxes cell j-1's N contributions (the last), after
+//             which cell j-1's dTr is complete and stored.
+//   k_tracer_update  hTr += dTr * (1/areaT), per cell and layer.
+//
+// A run's first face (whose left cell belongs to the previous run) and last
+// face (whose right cell belongs to the next) are computed by both runs, the
+// same expression on the same operands, so both get the same bits.
+//
+// Within a face, one lane per neutral surface computes the sublayer fluxes
+// (every sublayer's flux once, where the reference evaluates it from each
+// bounding cell: the same expression on the same operands); the
+// contributing ones (hEff != 0) are compacted into LDS in s order with
+// their two target layers.  Then one lane per (layer K, tracer) of the
+// cell adds its layer's contributions in s order.  Along a face the
+// targets of each side normally never decrease, so layer K's contributions
+// are one run of the compacted list, found from a per-layer start table;
+// a side whose targets decrease somewhere is scanned whole by every lane
+// instead, which is the same additions in the same order.
+//
+// dTr per cell lives in HBM between the two kernels (FP64, `acc`, the
+// (cell, K, t) entry at cell*nz*NT + (K-1)*NT + t), one contiguous block per
+// cell, so each lane's loads and stores are coalesced.
+constexpr int kChunk = 63;      // sublayers per pass of a wavefront (64 surfaces)
+constexpr int kFluxSegDefault = 32;  // cells per run
 
-// One wavefront per workgroup; lane = (cell, tracer), 64 / NT consecutive
-// cells of one row.  dTr(1:nz) of each lane lives in LDS in FP64 (both
-// forms), one column per lane (acc[K][lane]), so a lane can add into any
-// layer: every open face of the cell (W, E, S, N) is streamed sublayer by
-// sublayer (s ascending) and each contributing sublayer's flux is added
-// into dTr(nz+1-Ko(s)), exactly the order of the reference's loops,
-// whatever the pattern of Ko.  The lanes of a wavefront read one contiguous
-// stretch of each surface-major sublayer row.  Then hTr += dTr * (1/areaT).
-// acc: KCAP * 64 doubles of LDS.
-template <class F, int NT, int KCAP>
-__device__ __forceinline__ void cell_flux_body(const CellFluxArgs<F>& a, int bid, int nb,
-                                               double* acc) {
+// Layer-capacity classes: a domain of nz layers runs the kernels of the
+// smallest class with nzc >= nz (the LDS lists and the register
+// accumulators are sized by the class).
+constexpr int kFluxNzc[4] = {32, 76, 128, 256};
+
+template <class F>
+struct FluxArgs {
+  int nx, ny, nz;         // cells per row, rows, layers
+  int wall0, wall1;       // 0-based face coordinate across the faces (i for u,
+                          // j for v) of the two wall faces, -1 where open
+  int seg;                // cells per run
+  int nseg;               // runs per line of cells (row for x, column for y)
+  uint32_t fa, fb;        // face record (face, s) at face*fa + s*fb
+  double dt;
+  const double *khtr, *len, *inv;  // face diffusivity, face length, 1 / spacing
+  const FaceRecF<F>* rec;
+  const uint8_t* flags;   // face flags (kHasFlux)
+  ReconLayout ry;
+  const ReconRec* recon;  // tracer t's records from recon + t*ry.ct
+  double* acc;            // dTr, (cell, K, t) at cell*nz*NT + (K-1)*NT + t
+};
+
+// LDS of one flux wavefront: the compacted contributions of one face.
+template <int NT, int NZC>
+struct FluxLds {
+  double x[(2 * NZC + 1) * NT];      // fluxes, entry e's tracer t at e*NT + t
+  uint16_t tgt[2][2 * NZC + 1];      // entry e's target layer, left / right column
+  uint16_t start[2][NZC + 2];        // first entry whose target >= K (K = 1..nz+1)
+};
+
+// The contributing sublayers of face `fo` between columns cl (left) and cr
+// (right), compacted into L in s order; returns their count (wave-uniform).
+template <class F, int NT, int NZC>
+__device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t fo, uint32_t cl,
+                                                  uint32_t cr, FluxLds<NT, NZC>& L) {
   using R = typename F::R;
-  constexpr int kCells = 64 / NT;  // cells per wavefront
-  // sublayer rows per batch: columns of at most 32 layers (65 sublayers)
-  // take a face in two or three batches of 32, deeper ones stream batches
-  // of 16
-  constexpr int kBatch = KCAP <= 32 ? 32 : 16;
-  const int lane = (int)threadIdx.x;
-  const int cw = lane / NT, t = lane % NT;
-  const int segs = (a.nx + kCells - 1) / kCells;
-  const int w = xcd_band(bid, nb);
-  const int cj = a.vertical ? w % a.ny : w / segs;
-  const int ci = (a.vertical ? w / a.ny : w % segs) * kCells + cw;
-  const bool cell = cw < kCells && ci < a.nx && cj < a.ny;
-  const int nz = a.nz, nsub = 2 * nz + 1;
-  for (int K = 0; K < nz; ++K) acc[K * 64 + lane] = 0.0;
-  if (!cell) return;  // lanes only ever touch their own accumulator column
-  const int nx = a.nx, ny = a.ny;
-  const int i = ci + 1, j = cj + 1;  // 1-based
-  // the four faces: W (u face i, this cell on the right), E (u face i+1,
-  // left), S (v face j, right), N (v face j+1, left) (:1136-1175)
-  const bool act[4] = {
-      i >= 2 && !((a.wall_w && i == a.wuf_w) || (a.wall_e && i == a.wuf_e)),
-      i <= nx - 1 && !((a.wall_w && i + 1 == a.wuf_w) || (a.wall_e && i + 1 == a.wuf_e)),
-      j >= 2 && !((a.wall_s && j == a.wvf_s) || (a.wall_n && j == a.wvf_n)),
-      j <= ny - 1 && !((a.wall_s && j + 1 == a.wvf_s) || (a.wall_n && j + 1 == a.wvf_n))};
-  const size_t fo[4] = {(size_t)(i - 1) + (size_t)(nx + 1) * (j - 1),
-                        (size_t)i + (size_t)(nx + 1) * (j - 1),
-                        (size_t)(i - 1) + (size_t)nx * (j - 1),
-                        (size_t)(i - 1) + (size_t)nx * j};
-#pragma unroll 1
-  for (int f = 0; f < 4; ++f) {
-    if (!act[f]) continue;
-    const bool left = (f & 1) != 0;
-    const int shift = left ? kTargetShiftL : kTargetShiftR;
-    const size_t nf = f < 2 ? a.nu : a.nv;
-    const Global<const uint32_t>* tg = global_ptr(f < 2 ? a.tgt_u : a.tgt_v) + fo[f];
-    const Global<const R>* fx = global_ptr(f < 2 ? a.flx_u : a.flx_v) + fo[f] * NT + t;
-    // dTr(K) of the run of consecutive contributions to one layer K stays
-    // in a register; it is read from / written back to LDS when K changes,
-    // so every addition still lands in reference order
-    int curK = 0;
-    double cur = 0.0;
-    // every (sublayer, face) slot of a computed face holds a flux (0 where
-    // nothing contributes), so both loads issue without waiting; rows past
-    // the last sublayer re-read the last one and are skipped, which keeps
-    // the loads free of branches
-    auto load = [&](int s0, uint32_t (&pk)[kBatch], R (&x)[kBatch]) {
+  using Face = FaceRecF<F>;
+  const int lane = (int)(threadIdx.x & 63);
+  const double coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
+  const ReconRec* recL = a.recon + cl * a.ry.ca;
+  const ReconRec* recR = a.recon + cr * a.ry.ca;
+  const uint32_t cb = a.ry.cb, ct = a.ry.ct;
+  const size_t fb = a.fb;
+  const int ns = 2 * a.nz + 2, nsub = ns - 1;
+  const Face* r = a.rec + fo * a.fa;
+  int nc = 0;
+  for (int c0 = 0; c0 < nsub; c0 += kChunk) {
+    const int s = c0 + lane;
+    const bool surf = s < ns;
+    SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
+    if (surf) Fs = r[(size_t)s * fb].get();
+    const int KoL = Fs.KoL, KoR = Fs.KoR;
+    const double PoL = Fs.PoL, PoR = Fs.PoR;
+    // surface s+1, the bottom of sublayer s (lane 63's value is never used)
+    const double PoLb = __shfl_down(PoL, 1, 64), PoRb = __shfl_down(PoR, 1, 64);
+    const int KoLb = __shfl_down(KoL, 1, 64), KoRb = __shfl_down(KoR, 1, 64);
+    const bool contrib = surf && lane < kChunk && s < nsub && Fs.hEff != 0.0;
+    const double hEff = Fs.hEff;
+    double x[NT];
+    // one tracer's records at a time: a wave holds two gathered records
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int s = s0 + u < nsub ? s0 + u : nsub - 1;
-        pk[u] = tg[(size_t)s * nf];
-        x[u] = fx[(size_t)s * nf * NT];
+    for (int t = 0; t < NT; ++t) {
+      LayerVals qL{0.0, 0.0, 0.0, 0.0, 0.0}, qR = qL;
+      if (surf) {
+        qL = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
+        qR = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
       }
-    };
-    auto add = [&](int s0, const uint32_t (&pk)[kBatch], const R (&x)[kBatch]) {
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        if (s0 + u >= nsub || pk[u] == 0) continue;
-        const int K = (int)((pk[u] >> shift) & 0xffffu);
-        if (K != curK) {
-          if (curK) acc[(curK - 1) * 64 + lane] = cur;
-          cur = acc[(K - 1) * 64 + lane];
-          curK = K;
-        }
-        cur = left ? cur + (double)x[u] : cur - (double)x[u];
-      }
-    };
-    // two batches of rows in registers: the next batch's loads are in
-    // flight while the current one is added (one wave per SIMD, so nothing
-    // else hides the latency).  Two batches of 16 are 64 loads, about the
-    // 63 a wave's load counter tracks.  In the steady loop the loads are issued
-    // unconditionally (no branch around them), so that the wait for one
-    // batch never has to wait for the batch behind it (measured on MI355X:
-    // 10.8 against 11.8 ms with conditional loads of 32-row batches, FP64);
-    // the last batches, which may be past the last sublayer, come after it
-    uint32_t pa[kBatch], pb[kBatch];
-    R xa[kBatch], xb[kBatch];
-    load(0, pa, xa);
-    if constexpr (KCAP <= 32) {  // at most three batches: no steady loop
-      for (int s0 = 0; s0 < nsub; s0 += 2 * kBatch) {
-        const bool second = s0 + kBatch < nsub;
-        if (second) load(s0 + kBatch, pb, xb);
-        add(s0, pa, xa);
-        if (s0 + 2 * kBatch < nsub) load(s0 + 2 * kBatch, pa, xa);
-        if (second) add(s0 + kBatch, pb, xb);
-      }
-    } else {
-      int s0 = 0;
-      for (; s0 + 2 * kBatch < nsub; s0 += 2 * kBatch) {
-        load(s0 + kBatch, pb, xb);
-        add(s0, pa, xa);
-        load(s0 + 2 * kBatch, pa, xa);
-        add(s0 + kBatch, pb, xb);
-      }
-      if (s0 + kBatch < nsub) {  // the last one or two batches
-        load(s0 + kBatch, pb, xb);
-        add(s0, pa, xa);
-        add(s0 + kBatch, pb, xb);
-      } else {
-        add(s0, pa, xa);
+      const double eL = edge_value(qL, PoL), eR = edge_value(qR, PoR);
+      const double eLb = __shfl_down(eL, 1, 64), eRb = __shfl_down(eR, 1, 64);
+      x[t] = 0.0;
+      if (contrib) {
+        const double tlay = ppm_ave<double>(PoL, PoLb + (double)(KoLb - KoL), qL.aL, qL.aR, qL.Tl);
+        const double trlay = ppm_ave<double>(PoR, PoRb + (double)(KoRb - KoR), qR.aL, qR.aR, qR.Tl);
+        const double d = guarded_dT<double>(eL, eLb, eR, eRb, tlay, trlay);
+        x[t] = (double)(R)((d * hEff) * coef);  // the form's stored flux
       }
     }
-    if (curK) acc[(curK - 1) * 64 + lane] = cur;
-  }
-  const size_t n2 = (size_t)nx * ny;
-  const size_t c = (size_t)ci + (size_t)nx * cj;
-  const double iaij = 1.0 / global_ptr(a.areaT)[c];
-  R* hq = a.htr[0];
-  double* tq = a.tend[0];
+    const uint64_t m = __ballot(contrib);
+    const int pos = nc + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (contrib) {
 #pragma unroll
-  for (int q = 1; q < NT; ++q)
-    if (t == q) { hq = a.htr[q]; tq = a.tend[q]; }
-  Global<R>* hp = global_ptr(hq);
-  Global<double>* tp = global_ptr(tq);
-  for (int K = 1; K <= nz; ++K) {
-    const size_t o = (size_t)(nz - K) * n2 + c;  // native layer nz+1-K (1-based)
-    const double d = acc[(K - 1) * 64 + lane] * iaij;
-    hp[o] = (R)((double)hp[o] + d);
-    if (tp) tp[o] = d;
+      for (int t = 0; t < NT; ++t) L.x[pos * NT + t] = x[t];
+      L.tgt[0][pos] = (uint16_t)KoL;
+      L.tgt[1][pos] = (uint16_t)KoR;
+    }
+    nc += __popcll(m);
+  }
+  return nc;
+}
+
+// Lane `lane` of pass p takes entry idx = p*64 + lane of a cell's dTr block,
+// layer K = idx / NT + 1 and tracer t = idx % NT.
+//
+// v + (sign) the contributions of one side (0 left, 1 right) of the face in
+// L to layer K, tracer t, in s order.
+template <int NT, int NZC>
+__device__ __forceinline__ double fold_side(double v, const FluxLds<NT, NZC>& L, int side, bool mono,
+                                            int nc, int K, int t, bool plus) {
+  if (mono) {
+    const int lo = L.start[side][K], hi = L.start[side][K + 1];
+    for (int e = lo; e < hi; ++e) v = plus ? v + L.x[e * NT + t] : v - L.x[e * NT + t];
+  } else {
+    for (int e = 0; e < nc; ++e)
+      if ((int)L.tgt[side][e] == K) v = plus ? v + L.x[e * NT + t] : v - L.x[e * NT + t];
+  }
+  return v;
+}
+
+// Run `w` of the x (kY = false) or y (kY = true) face grid.
+template <class F, int NT, int NZC, bool kY>
+__device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
+                                          FluxLds<NT, NZC>& L) {
+  constexpr int P = (NZC * NT + 63) / 64;  // passes over a cell's dTr block
+  const int lane = (int)(threadIdx.x & 63);
+  const int w = xcd_band(bid, nb);
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nk = nz * NT;
+  // x: runs along row `line`; y: runs up column `line`, the runs of one
+  // height side by side (adjacent columns share their records' cache lines)
+  const int line = kY ? w % nx : w / a.nseg;
+  const int sg = kY ? w / nx : w % a.nseg;
+  const int n_along = kY ? ny : nx;
+  const int c_lo = sg * a.seg;
+  if (line >= (kY ? nx : ny) || c_lo >= n_along) return;
+  const int c_hi = c_lo + a.seg < n_along ? c_lo + a.seg : n_along;
+  Global<double>* acc = global_ptr(a.acc);
+  // dTr partials of the run's current cells: prev = the left cell of the
+  // face at hand, cur = its right cell
+  double prev[P], cur[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) prev[p] = cur[p] = 0.0;
+  for (int f = c_lo; f <= c_hi; ++f) {
+    const bool doL = f > c_lo;  // the face's left cell is this run's (E / N)
+    const bool doR = f < c_hi;  // its right cell is this run's (W / S)
+    // right column (i, j) of the face, its left column (i - 1, j) or (i, j - 1)
+    const int i = kY ? line : f, j = kY ? f : line;
+    const uint32_t cr = (uint32_t)i + (uint32_t)nx * (uint32_t)j;
+    const uint32_t cl = kY ? cr - (uint32_t)nx : cr - 1u;
+    if (kY && doR) {  // the cell's W + E partial, loaded early
+      const Global<double>* src = acc + (size_t)cr * nk;
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int idx = p * 64 + lane;
+        cur[p] = idx < nk ? src[idx] : 0.0;
+      }
+    }
+    // faces computed: i = 2..nx (u), j = 2..ny (v), 1-based, and not a wall
+    const int across = kY ? j : i;
+    const size_t fo = kY ? (size_t)i + (size_t)nx * j : (size_t)i + (size_t)(nx + 1) * j;
+    const bool live = across >= 1 && across <= n_along - 1 && across != a.wall0 &&
+                      across != a.wall1 && (a.flags[fo] & kHasFlux);
+    int nc = 0;
+    bool monoL = true, monoR = true;
+    if (live) {
+      nc = face_contributions<F, NT, NZC>(a, fo, cl, cr, L);
+      __syncthreads();
+      // a side is monotone when no entry's target is below the previous one's
+      bool dL = false, dR = false;
+      for (int e = lane + 1; e < nc; e += 64) {
+        dL = dL || L.tgt[0][e] < L.tgt[0][e - 1];
+        dR = dR || L.tgt[1][e] < L.tgt[1][e - 1];
+      }
+      monoL = __ballot(dL) == 0;
+      monoR = __ballot(dR) == 0;
+      for (int k = lane; k < nz + 2; k += 64) {
+        L.start[0][k] = (uint16_t)nc;
+        L.start[1][k] = (uint16_t)nc;
+      }
+      __syncthreads();
+      // start[K] = first entry with target >= K: entry e starts layers
+      // target(e-1)+1 .. target(e)
+      for (int e = lane; e < nc; e += 64) {
+        if (monoL) {
+          const int hi = L.tgt[0][e], lo = e ? L.tgt[0][e - 1] + 1 : 1;
+          for (int k = lo; k <= hi; ++k) L.start[0][k] = (uint16_t)e;
+        }
+        if (monoR) {
+          const int hi = L.tgt[1][e], lo = e ? L.tgt[1][e - 1] + 1 : 1;
+          for (int k = lo; k <= hi; ++k) L.start[1][k] = (uint16_t)e;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int idx = p * 64 + lane;
+      const int K = idx / NT + 1, t = idx % NT;
+      const bool ok = idx < nk;
+      if (doL) {  // E (x) / N (y): the left cell's second / last face
+        if (ok && nc) prev[p] = fold_side<NT, NZC>(prev[p], L, 0, monoL, nc, K, t, true);
+        if (ok) acc[(size_t)cl * nk + idx] = prev[p];
+      }
+      if (doR) {  // W (x, added to 0) / S (y, added to the W + E partial)
+        double v = kY ? cur[p] : 0.0;
+        if (ok && nc) v = fold_side<NT, NZC>(v, L, 1, monoR, nc, K, t, false);
+        prev[p] = v;
+      }
+    }
+    if (live) __syncthreads();  // the next face overwrites L
   }
 }
 
-template <class F, int NT, int KCAP>
-__global__ void __launch_bounds__(64) k_cell_flux(CellFluxArgs<F> a) {
-  __shared__ double acc[KCAP * 64];
-  cell_flux_body<F, NT, KCAP>(a, (int)blockIdx.x, (int)gridDim.x, acc);
+template <class F, int NT, int NZC, bool kY>
+__global__ void __launch_bounds__(64) k_flux(FluxArgs<F> a) {
+  __shared__ FluxLds<NT, NZC> lds;
+  flux_body<F, NT, NZC, kY>(a, (int)blockIdx.x, (int)gridDim.x, lds);
 }
 
-// Workgroups of k_cell_flux for an nx x ny domain.
-template <int NT>
-inline unsigned cell_flux_blocks(int nx, int ny) {
-  constexpr int kCells = 64 / NT;
-  return (unsigned)ny * (unsigned)((nx + kCells - 1) / kCells);
+// Runs (workgroups) of one face grid.
+inline unsigned flux_blocks(int nx, int ny, int seg, bool y, int* nseg) {
+  const int n = y ? ny : nx;
+  *nseg = (n + seg - 1) / seg;
+  return (unsigned)*nseg * (unsigned)(y ? nx : ny);
 }
 
+// ---------------------------------------------------------------------------
+// K5: hTr += dTr * (1/areaT) (redi_apply_flux_impl, :1177-1179)
+// ---------------------------------------------------------------------------
+template <class F>
+struct UpdateArgs {
+  int N2, nz, nt;                   // columns, layers, tracers of the group
+  const double* areaT;
+  const double* acc;                // dTr, the flux kernels' layout
+  typename F::R* htr[kMaxNT];
+  double* tend[kMaxNT];             // optional: the FP64 tendency dTr / areaT per cell
+};
+
+// A workgroup takes kUpdCells consecutive columns x kUpdK layers: it reads
+// their dTr blocks along each column's run of (K, t) entries, transposes
+// them through LDS, and updates each tracer layer as one contiguous run of
+// columns.
+constexpr int kUpdCells = 64, kUpdK = 8, kUpdLanes = 256;
+
+template <class F>
+__device__ __forceinline__ void tracer_update_body(const UpdateArgs<F>& a, int bid, int nb,
+                                                   double* tile, double* inv) {
+  using R = typename F::R;
+  const int nz = a.nz, NT = a.nt, N2 = a.N2, nk = nz * NT;
+  const int nkb = (nz + kUpdK - 1) / kUpdK;
+  const int b = xcd_band(bid, nb);
+  const int c0 = (b / nkb) * kUpdCells, k0 = (b % nkb) * kUpdK;  // k0: 0-based top-down layer
+  const int rows = (nz - k0 < kUpdK ? nz - k0 : kUpdK) * NT;      // (K, t) rows of the tile
+  const int tid = (int)threadIdx.x;
+  constexpr int kRowMax = kUpdK * kMaxNT;
+  const Global<const double>* acc = global_ptr(a.acc);
+  for (int q = tid; q < kUpdCells * kRowMax; q += kUpdLanes) {
+    const int cc = q / kRowMax, r = q % kRowMax, c = c0 + cc;
+    if (c < N2 && r < rows) tile[r * (kUpdCells + 1) + cc] = acc[(size_t)c * nk + (size_t)k0 * NT + r];
+  }
+  if (tid < kUpdCells && c0 + tid < N2) inv[tid] = 1.0 / global_ptr(a.areaT)[c0 + tid];
+  __syncthreads();
+  for (int q = tid; q < kRowMax * kUpdCells; q += kUpdLanes) {
+    const int r = q / kUpdCells, cc = q % kUpdCells, c = c0 + cc;
+    if (c >= N2 || r >= rows) continue;
+    const int k = k0 + r / NT, t = r % NT;
+    R* hq = a.htr[0];
+    double* tq = a.tend[0];
+#pragma unroll
+    for (int u = 1; u < kMaxNT; ++u)
+      if (t == u) { hq = a.htr[u]; tq = a.tend[u]; }
+    const size_t o = (size_t)(nz - 1 - k) * (size_t)N2 + c;  // top-down layer k+1 -> native
+    const double d = tile[r * (kUpdCells + 1) + cc] * inv[cc];
+    Global<R>* hp = global_ptr(hq);
+    hp[o] = (R)((double)hp[o] + d);
+    if (tq) global_ptr(tq)[o] = d;
+  }
+}
+
+constexpr int kUpdTileDoubles = kUpdK * kMaxNT * (kUpdCells + 1);
+
+template <class F>
+__global__ void __launch_bounds__(kUpdLanes) k_tracer_update(UpdateArgs<F> a) {
+  __shared__ double tile[kUpdTileDoubles];
+  __shared__ double inv[kUpdCells];
+  tracer_update_body<F>(a, (int)blockIdx.x, (int)gridDim.x, tile, inv);
+}
+
+inline unsigned update_blocks(int N2, int nz) {
+  return (unsigned)((N2 + kUpdCells - 1) / kUpdCells) * (unsigned)((nz + kUpdK - 1) / kUpdK);
+}
 // ---------------------------------------------------------------------------
 // Face records <-> reference arrays, khtr faces, initial fill
 // ---------------------------------------------------------------------------

@@ -80,9 +80,10 @@ constexpr int kTracerGroup = kMaxNT;  // tracers per flux launch
 // tracer runs through the two-tracer kernels, so at least two).
 int recon_slots(int nt) { return std::min(std::max(nt, 2), kTracerGroup); }
 
-// Faces per sublayer row of the surface-major face arrays: the face count
-// rounded up to whole face-flux workgroups.
-size_t padded_faces(size_t nf) { return (nf + kFaceGroup - 1) / kFaceGroup * kFaceGroup; }
+// Faces of a face-record array: the face count rounded up to a multiple of
+// 4, so every face grid's records start 256-byte aligned.
+constexpr size_t kFacePad = 4;
+size_t padded_faces(size_t nf) { return (nf + kFacePad - 1) / kFacePad * kFacePad; }
 
 // Bytes of the per-form records and values (the column state is FP64 in
 // both forms).
@@ -100,24 +101,19 @@ FormSizes sizes_of(int form) {
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
 //   Phase A: ColRec + P per column interface.
-//   Phase B, per tracer slot: a ReconRec per layer of every column and a
-//   flux per sublayer of every (padded) u and v face; once: the packed
-//   targets (4 B) per sublayer and face, and a discard layer field that
-//   takes the second output of a two-tracer launch run for a lone tracer.
+//   Phase B, per tracer slot: a ReconRec and an FP64 dTr value per layer of
+//   every column; once: a discard layer field that takes the second output
+//   of a two-tracer launch run for a lone tracer.
 // Every array starts on a 256-byte boundary.
 struct PhaseBScratch {
   void* recon;       // N2 * nz * slots records (ReconLayout)
   void* discard;     // N2 * nz values
-  void* flx_u;       // (ns - 1) * Nu' * slots values, Nu' = padded u faces
-  void* flx_v;       // (ns - 1) * Nv' * slots values
-  uint32_t* tgt_u;   // (ns - 1) * Nu'
-  uint32_t* tgt_v;   // (ns - 1) * Nv'
+  double* acc;       // N2 * nz * slots dTr values
   size_t bytes;
 };
 
 PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const FormSizes& z) {
-  const size_t N2 = (size_t)nx * ny, nsub = 2 * (size_t)nz + 1;
-  const size_t nu = padded_faces((size_t)(nx + 1) * ny), nv = padded_faces((size_t)nx * (ny + 1));
+  const size_t N2 = (size_t)nx * ny;
   char* b = (char*)base;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -128,10 +124,7 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const
   PhaseBScratch s;
   s.recon = take(z.recon * N2 * nz * slots);
   s.discard = take(z.val * N2 * nz);
-  s.flx_u = take(z.val * nsub * nu * slots);
-  s.flx_v = take(z.val * nsub * nv * slots);
-  s.tgt_u = (uint32_t*)take(4 * nsub * nu);
-  s.tgt_v = (uint32_t*)take(4 * nsub * nv);
+  s.acc = (double*)take(sizeof(double) * N2 * nz * slots);
   s.bytes = off;
   return s;
 }
@@ -142,8 +135,8 @@ size_t workspace_bytes(int nx, int ny, int nz, int nt, int form) {
   return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt), z).bytes);
 }
 
-// The per-face Phase-A state: FaceRec (face, s) at s*nup + face
-// (surface-major, rows padded) for u and v faces, plus the monotonicity
+// The per-face Phase-A state: FaceRec (face, s) at face*ns + s
+// (face-major, the face count padded) for u and v faces, plus the face
 // flags and the resolved face diffusivities.
 struct FaceState {
   void* u = nullptr;
@@ -207,8 +200,8 @@ struct Tiles {
   int recon_layer_major = 0; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
   int recon_kt = kReconChunkDefault;  // layers per tiled-reconstruction workgroup
-  int face_stage = kFaceStageDefault;  // face fluxes read their records through LDS (per-column order)
-  int vertical = kTileVerticalDefault;  // sweep tiles and cell-flux segments numbered down rows first
+  int seg = kFluxSegDefault;  // cells per run of the flux kernels
+  int vertical = kTileVerticalDefault;  // sweep tiles numbered down rows first
 };
 
 TileMap make_tiles(int n0, int n1, const Tiles& t, unsigned* nblocks) {
@@ -292,87 +285,71 @@ void launch_recon(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
   }
 }
 
-template <class F, int NT>
-void launch_cell(int kcap, unsigned nbc, const CellFluxArgs<F>& ca, hipStream_t st) {
-  switch (kcap) {
-    case 0: k_cell_flux<F, NT, kCellKCap[0]><<<nbc, 64, 0, st>>>(ca); break;
-    case 1: k_cell_flux<F, NT, kCellKCap[1]><<<nbc, 64, 0, st>>>(ca); break;
-    case 2: k_cell_flux<F, NT, kCellKCap[2]><<<nbc, 64, 0, st>>>(ca); break;
-    default: k_cell_flux<F, NT, kCellKCap[3]><<<nbc, 64, 0, st>>>(ca); break;
-  }
+template <class F, int NT, int NZC>
+void launch_flux_pair(const FluxArgs<F>& fx, const FluxArgs<F>& fy, unsigned nbx, unsigned nby,
+                      hipStream_t st, Timer& tm) {
+  int slot = tm.begin(REDI_K_FLUX_X, st);
+  k_flux<F, NT, NZC, false><<<nbx, 64, 0, st>>>(fx);
+  tm.end(slot, st);
+  slot = tm.begin(REDI_K_FLUX_Y, st);
+  k_flux<F, NT, NZC, true><<<nby, 64, 0, st>>>(fy);
+  tm.end(slot, st);
 }
 
-// One tracer group of Phase B: reconstruction, the face fluxes of both
-// grids, the divergence.
+// Smallest layer-capacity class of the flux kernels that holds nz layers.
+int flux_class(int nz) {
+  int c = 0;
+  while (kFluxNzc[c] < nz) ++c;
+  return c;
+}
+
+// One tracer group of Phase B: reconstruction, the fluxes of both face
+// grids with their divergence, the tracer update.
 template <class F, int NT>
 void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
                   const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
-                  FaceFluxArgs<F>& fu, FaceFluxArgs<F>& fv, unsigned nbu, unsigned nbv,
-                  const CellFluxArgs<F>& ca, int kcap, unsigned nbc, hipStream_t st, Timer& tm) {
+                  const FluxArgs<F>& fx, const FluxArgs<F>& fy, unsigned nbx, unsigned nby,
+                  const UpdateArgs<F>& ua, hipStream_t st, Timer& tm) {
   int slot = tm.begin(REDI_K_TRACER_RECON, st);
   launch_recon<F, NT>(p, tiles, h, ts, recon, ry, st);
   tm.end(slot, st);
-  const bool staged = tiles.face_stage && !tiles.recon_layer_major;
-  for (int grid = 0; grid < 2; ++grid) {  // u faces, then v faces
-    const unsigned nb = grid ? nbv : nbu;
-    if (!nb) continue;
-    slot = tm.begin(REDI_K_FACE_FLUX, st);
-    if (staged)
-      k_face_flux_staged<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
-    else
-      k_face_flux<F, NT><<<nb, 64 * kFaceGroup, 0, st>>>(grid ? fv : fu);
-    tm.end(slot, st);
+  switch (flux_class(p.nz)) {
+    case 0: launch_flux_pair<F, NT, kFluxNzc[0]>(fx, fy, nbx, nby, st, tm); break;
+    case 1: launch_flux_pair<F, NT, kFluxNzc[1]>(fx, fy, nbx, nby, st, tm); break;
+    case 2: launch_flux_pair<F, NT, kFluxNzc[2]>(fx, fy, nbx, nby, st, tm); break;
+    default: launch_flux_pair<F, NT, kFluxNzc[3]>(fx, fy, nbx, nby, st, tm); break;
   }
-  slot = tm.begin(REDI_K_CELL_FLUX, st);
-  launch_cell<F, NT>(kcap, nbc, ca, st);
+  slot = tm.begin(REDI_K_TRACER_UPDATE, st);
+  k_tracer_update<F><<<update_blocks(p.nx * p.ny, p.nz), kUpdLanes, 0, st>>>(ua);
   tm.end(slot, st);
 }
 
-// Face-flux arguments of both face grids and the cell-flux arguments of one
-// domain's Phase B over the scratch `sc` (tracer pointers and the
-// reconstruction layout are set per tracer group).
+// Flux arguments of both face grids and the update arguments of one
+// domain's Phase B over the scratch `sc` (the reconstruction layout, the
+// tracer pointers and the group size are set per tracer group).
 template <class F>
-void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc, FaceFluxArgs<F>& fu,
-                  FaceFluxArgs<F>& fv, CellFluxArgs<F>& ca) {
-  using R = typename F::R;
+void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc, int seg,
+                  FluxArgs<F>& fx, FluxArgs<F>& fy, UpdateArgs<F>& ua, unsigned* nbx, unsigned* nby) {
   const int nx = p.nx, ny = p.ny, nz = p.nz, ns = 2 * nz + 2;
-  const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
   ReconRec* recon = (ReconRec*)sc.recon;
-  fu = FaceFluxArgs<F>{};  // u faces i = 2..nx (1-based), every row
-  fu.nfa = nx + 1; fu.ilo = 1; fu.ihi = nx; fu.jlo = 0; fu.jhi = ny;
-  fu.di = 1; fu.dj = 0; fu.nx = nx; fu.ns = ns; fu.nzc = nz;
-  fu.wall0 = p.ww ? p.nghost : -1;
-  fu.wall1 = p.we ? p.nghost + p.nxp : -1;
-  fu.nf = (uint32_t)nu; fu.fa = face_fa(ns); fu.fb = face_fb(ns); fu.nfp = (uint32_t)fs.nup;
-  fu.dt = p.dt; fu.khtr = fs.khtr_u; fu.len = p.dy_cu; fu.inv = p.idxCu;
-  fu.rec = (const FaceRecF<F>*)fs.u; fu.recon = recon;
-  fu.tgt = sc.tgt_u; fu.flx = (R*)sc.flx_u;
-  fv = FaceFluxArgs<F>{};  // v faces j = 2..ny (1-based), every column
-  fv.nfa = nx; fv.ilo = 0; fv.ihi = nx; fv.jlo = 1; fv.jhi = ny;
-  fv.di = 0; fv.dj = 1; fv.nx = nx; fv.ns = ns; fv.nzc = nz;
-  fv.wall0 = p.ws ? p.nghost : -1;
-  fv.wall1 = p.wn ? p.nghost + p.nyp : -1;
-  fv.nf = (uint32_t)nv; fv.fa = face_fa(ns); fv.fb = face_fb(ns); fv.nfp = (uint32_t)fs.nvp;
-  fv.dt = p.dt; fv.khtr = fs.khtr_v; fv.len = p.dx_cv; fv.inv = p.idyCv;
-  fv.rec = (const FaceRecF<F>*)fs.v; fv.recon = recon;
-  fv.tgt = sc.tgt_v; fv.flx = (R*)sc.flx_v;
-
-  ca = CellFluxArgs<F>{};
-  ca.nx = nx; ca.ny = ny; ca.nz = nz;
-  ca.wuf_w = p.nghost + 1; ca.wuf_e = p.nghost + p.nxp + 1;
-  ca.wvf_s = p.nghost + 1; ca.wvf_n = p.nghost + p.nyp + 1;
-  ca.wall_w = p.ww; ca.wall_e = p.we; ca.wall_s = p.ws; ca.wall_n = p.wn;
-  ca.areaT = p.areaT;
-  ca.tgt_u = sc.tgt_u; ca.tgt_v = sc.tgt_v;
-  ca.flx_u = (const R*)sc.flx_u; ca.flx_v = (const R*)sc.flx_v;
-  ca.nu = (uint32_t)fs.nup; ca.nv = (uint32_t)fs.nvp;
-}
-
-// Smallest accumulator depth class of k_cell_flux that holds nz layers.
-int cell_kcap(int nz) {
-  int kcap = 0;
-  while (kCellKCap[kcap] < nz) ++kcap;
-  return kcap;
+  fx = FluxArgs<F>{};  // u faces: runs along rows
+  fx.nx = nx; fx.ny = ny; fx.nz = nz;
+  fx.wall0 = p.ww ? p.nghost : -1;
+  fx.wall1 = p.we ? p.nghost + p.nxp : -1;
+  fx.seg = seg;
+  *nbx = flux_blocks(nx, ny, seg, false, &fx.nseg);
+  fx.fa = face_fa(ns); fx.fb = face_fb(ns);
+  fx.dt = p.dt; fx.khtr = fs.khtr_u; fx.len = p.dy_cu; fx.inv = p.idxCu;
+  fx.rec = (const FaceRecF<F>*)fs.u; fx.flags = fs.mono_u;
+  fx.recon = recon; fx.acc = sc.acc;
+  fy = fx;  // v faces: runs up columns
+  fy.wall0 = p.ws ? p.nghost : -1;
+  fy.wall1 = p.wn ? p.nghost + p.nyp : -1;
+  *nby = flux_blocks(nx, ny, seg, true, &fy.nseg);
+  fy.khtr = fs.khtr_v; fy.len = p.dx_cv; fy.inv = p.idyCv;
+  fy.rec = (const FaceRecF<F>*)fs.v; fy.flags = fs.mono_v;
+  ua = UpdateArgs<F>{};
+  ua.N2 = nx * ny; ua.nz = nz; ua.areaT = p.areaT; ua.acc = sc.acc;
 }
 
 // Reconstruction layout of a group of k tracers.
@@ -396,42 +373,34 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   const int slots = recon_slots(nt);
   const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
   ReconRec* recon = (ReconRec*)sc.recon;
-  FaceFluxArgs<F> fu, fv;
-  CellFluxArgs<F> ca;
-  phase_b_args<F>(p, fs, sc, fu, fv, ca);
-  ca.vertical = tiles.vertical;
-
-  const int kcap = cell_kcap(nz);
-  const unsigned nbu = nx > 1 ? (unsigned)(fs.nup / kFaceGroup) : 0;
-  const unsigned nbv = ny > 1 ? (unsigned)(fs.nvp / kFaceGroup) : 0;
+  FluxArgs<F> fx, fy;
+  UpdateArgs<F> ua;
+  unsigned nbx = 0, nby = 0;
+  phase_b_args<F>(p, fs, sc, tiles.seg, fx, fy, ua, &nbx, &nby);
   const int G = std::max(1, std::min(tiles.group, slots));
   for (int g0 = 0; g0 < nt; g0 += G) {
     const int gn = std::min(G, nt - g0);
     // A lone tracer goes through the two-tracer kernels: slot 1 repeats it
-    // and its result lands in the discard field.  (A one-tracer build of
-    // the flux kernels gave results that differ from the oracle at -O1 and
-    // above while its source is exact in a thread-by-thread CPU build and
-    // at -O0, so none is built.)
+    // and its result lands in the discard field.
     const int kn = gn == 1 ? 2 : gn;
     TracerSetT<R> ts{};
     for (int t = 0; t < kMaxNT; ++t) {
       const int tt = std::min(t, gn - 1);
       ts.q[t] = tracers[g0 + tt];
-      ca.htr[t] = tracers[g0 + tt];
-      ca.tend[t] = tend ? tend[g0 + tt] : nullptr;
+      ua.htr[t] = tracers[g0 + tt];
+      ua.tend[t] = tend ? tend[g0 + tt] : nullptr;
     }
     if (gn == 1) {
-      ca.htr[1] = (R*)sc.discard;
-      ca.tend[1] = nullptr;
+      ua.htr[1] = (R*)sc.discard;
+      ua.tend[1] = nullptr;
     }
+    ua.nt = kn;
     const ReconLayout ry = recon_layout(tiles.recon_layer_major, N2, nz, kn);
-    fu.ry = fv.ry = ry;
-    const unsigned nbc = kn == 2 ? cell_flux_blocks<2>(nx, ny)
-                                 : (kn == 3 ? cell_flux_blocks<3>(nx, ny) : cell_flux_blocks<4>(nx, ny));
+    fx.ry = fy.ry = ry;
     switch (kn) {
-      case 2: launch_group<F, 2>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
-      case 3: launch_group<F, 3>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
-      default: launch_group<F, 4>(p, tiles, h, ts, recon, ry, fu, fv, nbu, nbv, ca, kcap, nbc, st, tm); break;
+      case 2: launch_group<F, 2>(p, tiles, h, ts, recon, ry, fx, fy, nbx, nby, ua, st, tm); break;
+      case 3: launch_group<F, 3>(p, tiles, h, ts, recon, ry, fx, fy, nbx, nby, ua, st, tm); break;
+      default: launch_group<F, 4>(p, tiles, h, ts, recon, ry, fx, fy, nbx, nby, ua, st, tm); break;
     }
   }
   HIP_TRY(hipGetLastError());
@@ -621,7 +590,7 @@ extern "C" {
 
 const char* redi_last_error(void) { return g_err.c_str(); }
 
-int redi_version(void) { return 300; }
+int redi_version(void) { return 400; }
 
 long long redi_workspace_bytes(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return 0;
@@ -748,9 +717,11 @@ int redi_set_tile_order(redi_t* r, int vertical) {
   return REDI_OK;
 }
 
-int redi_set_face_staging(redi_t* r, int on) {
-  if (!r) return fail(REDI_ERR_ARG, "redi_set_face_staging: null handle");
-  r->tiles.face_stage = on ? 1 : 0;
+int redi_set_flux_segment(redi_t* r, int cells) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_flux_segment: null handle");
+  if (cells < 1 || cells > 4096)
+    return fail(REDI_ERR_ARG, "redi_set_flux_segment: %d cells per run not in 1..4096", cells);
+  r->tiles.seg = cells;
   return REDI_OK;
 }
 
@@ -767,8 +738,8 @@ int redi_nonmonotone_faces(const redi_t* r, long long* nu, long long* nv) {
   HIP_TRY(hipMemcpy(hu.data(), r->fs.mono_u, r->fs.nu, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(hv.data(), r->fs.mono_v, r->fs.nv, hipMemcpyDeviceToHost));
   *nu = 0; *nv = 0;
-  for (uint8_t f : hu) *nu += (f != (kMonoL | kMonoR));
-  for (uint8_t f : hv) *nv += (f != (kMonoL | kMonoR));
+  for (uint8_t f : hu) *nu += ((f & (kMonoL | kMonoR)) != (kMonoL | kMonoR));
+  for (uint8_t f : hv) *nv += ((f & (kMonoL | kMonoR)) != (kMonoL | kMonoR));
   return REDI_OK;
 }
 
@@ -911,8 +882,8 @@ int redi_timing_collect(redi_t* r, redi_kernel_times_t* out) {
 const char* redi_kernel_name(int kind) {
   static const char* names[REDI_NKERNELS] = {"k_column_state", "k_neutral_sweep_u",
                                              "k_neutral_sweep_v", "k_face_khtr",
-                                             "k_tracer_recon", "k_cell_flux",
-                                             "k_face_flux"};
+                                             "k_tracer_recon", "k_flux_x",
+                                             "k_flux_y", "k_tracer_update"};
   return (kind >= 0 && kind < REDI_NKERNELS) ? names[kind] : "";
 }
 

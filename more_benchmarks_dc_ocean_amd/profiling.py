@@ -101,10 +101,18 @@ def _base(name: str) -> str:
     """The byte model's name for a kernel (template arguments, the
     reconstruction's tiled variant and the batched kernels' kb_ prefix folded
     in)."""
-    b = _short(name).split("<")[0]
+    short = _short(name)
+    b = short.split("<")[0]
     if b.startswith("kb_"):
         b = "k_" + b[3:]
+    if b == "k_flux":  # k_flux<F, NT, NZC, y>: the x (u faces) or y (v faces) grid
+        b = "k_flux_y" if short.rstrip(">").split(",")[-1].strip() == "true" else "k_flux_x"
     return "k_tracer_recon" if b.startswith("k_tracer_recon") else b
+
+
+def _flux_class(name: str):
+    """The layer-capacity class of a k_flux / kb_flux kernel name."""
+    return int(_short(name).rstrip(">").split(",")[-2])
 
 
 def summarize_dir(d: str, workload: str, copy_mib: int = 2048, form: str = "fp64") -> dict:
@@ -115,12 +123,13 @@ def summarize_dir(d: str, workload: str, copy_mib: int = 2048, form: str = "fp64
     from .autotune import TUNED, TuneCache
     cases = _cases(workload, 1)
     alg: dict = {}
-    cell_by_cap: dict = {}  # batched: the cell-flux bytes per accumulator depth class
+    flux_by_cap: dict = {}  # batched: the flux kernels' bytes per (grid, layer-capacity class)
     for case in cases:
         if workload == BATCHED:
             cap = next(c for c in (32, 76, 128, 256) if c >= case.nz)
-            cell_by_cap[cap] = cell_by_cap.get(cap, 0.0) + sum(
-                byte_model.step_launches(case, 4, form)["k_cell_flux"])
+            sl = byte_model.step_launches(case, 4, form)
+            for kk in ("k_flux_x", "k_flux_y"):
+                flux_by_cap[(kk, cap)] = flux_by_cap.get((kk, cap), 0.0) + sum(sl[kk])
         group = 4 if workload == BATCHED else \
             TuneCache(TUNED).lookup(case.nx, case.ny, case.nz).get("group", 4)
         for k, v in byte_model.step_launches(case, group, form).items():
@@ -133,7 +142,6 @@ def summarize_dir(d: str, workload: str, copy_mib: int = 2048, form: str = "fp64
                 alg[k] = sum(v) / len(v)
     if workload == BATCHED:
         alg["k_neutral_sweep_u"] = alg.pop("k_neutral_sweep", 0.0)
-        alg["k_face_flux"] = alg.get("k_face_flux", 0.0)
     stats = {}
     for row in _rows(os.path.join(d, "kt", "*kernel_stats.csv")):
         stats[_short(row["Name"])] = (int(row["Calls"]), float(row["AverageNs"]) * 1e-6)
@@ -155,8 +163,8 @@ def summarize_dir(d: str, workload: str, copy_mib: int = 2048, form: str = "fp64
         cyc = ms * 1e-3 * GPU_CLOCK_HZ
         e = {"calls": calls, "avg_ms": ms}
         a = alg.get(_base(k) if _base(k) != "k_neutral_sweep" else "k_neutral_sweep_u")
-        if workload == BATCHED and k.startswith("kb_cell_flux"):
-            a = cell_by_cap.get(int(k.rstrip(">").split(",")[-1]))
+        if workload == BATCHED and k.startswith("kb_flux"):
+            a = flux_by_cap.get((_base(k), _flux_class(k)))
         if a is not None:
             e["alg_bytes"] = a
             e["alg_GBps"] = a / (ms * 1e-3) / 1e9

@@ -78,13 +78,13 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
             "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
-            "redi_set_face_staging", "redi_set_tile_order",
+            "redi_set_flux_segment", "redi_set_tile_order",
             # the batched form (bound in batch.py)
             "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
             "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
             "redi_batch_export_coeffs", "redi_batch_info")
 FORMS = {"fp64": 0, "fp32_store": 1}  # REDI_FORM_*
-NKERNELS = 7  # REDI_NKERNELS
+NKERNELS = 8  # REDI_NKERNELS
 
 
 class KernelTimes(ctypes.Structure):
@@ -113,7 +113,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_recon_order.argtypes = [_P, _I]
         L.redi_set_recon_chunk.argtypes = [_P, _I]
         L.redi_set_slope_taper.argtypes = [_P, _D]
-        L.redi_set_face_staging.argtypes = [_P, _I]
+        L.redi_set_flux_segment.argtypes = [_P, _I]
         L.redi_set_tile_order.argtypes = [_P, _I]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
         L.redi_set_tracer_group.argtypes = [_P, _I]
@@ -358,13 +358,12 @@ class OceanRedi:
         _check(lib().redi_set_slope_taper(self.handle, float(slope_max)), "redi_set_slope_taper")
 
     def set_tile_order(self, vertical: bool) -> None:
-        """Number sweep tiles / cell-flux segments down the rows first
-        (schedule only, same bits)."""
+        """Number sweep tiles down the rows first (schedule only, same bits)."""
         _check(lib().redi_set_tile_order(self.handle, int(bool(vertical))), "redi_set_tile_order")
 
-    def set_face_staging(self, on: bool) -> None:
-        """Face-flux record reads through LDS (schedule only, same bits)."""
-        _check(lib().redi_set_face_staging(self.handle, int(bool(on))), "redi_set_face_staging")
+    def set_flux_segment(self, cells: int) -> None:
+        """Cells per run of the Phase-B flux kernels (schedule only, same bits)."""
+        _check(lib().redi_set_flux_segment(self.handle, int(cells)), "redi_set_flux_segment")
 
     def set_recon_chunk(self, kt: int) -> None:
         """Layers per tiled-reconstruction workgroup (8, 16 or 32)."""

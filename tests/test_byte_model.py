@@ -45,8 +45,9 @@ def test_launch_split_adds_up(group, nt, form):
     assert total == pytest.approx(bm.step_bytes(c, form), rel=1e-12)
     ngroups = -(-nt // max(1, min(group, nt)))
     assert len(launches["k_tracer_recon"]) == ngroups
-    assert len(launches["k_cell_flux"]) == ngroups
-    assert len(launches["k_face_flux"]) == 2 * ngroups
+    assert len(launches["k_tracer_update"]) == ngroups
+    assert len(launches["k_flux_x"]) == ngroups
+    assert len(launches["k_flux_y"]) == ngroups
 
 
 def test_ceiling():

@@ -176,47 +176,51 @@ def test_hip_deterministic(gpu_device):
 
 
 @pytest.mark.parametrize("name", ["roquet_two_fronts_mini", "wright_vanished_mini", "nz1_mini"])
-def test_hip_face_staging_same_bits(name, gpu_device):
-    """Face-flux records read through LDS: the oracle's bits in the FP64 form,
-    the gathered path's bits in the FP32-store form."""
+def test_hip_flux_segment_same_bits(name, gpu_device):
+    """Runs of 1, 3, 7 and 1000 cells in the flux kernels (every run's end
+    faces computed twice, or whole rows and columns in one run): the
+    oracle's bits in the FP64 form, the default run's bits in the
+    FP32-store form."""
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     case, ref = load_golden(name)
-    print(f"[gpu] face staging {name}", flush=True)
-    outs = {}
+    print(f"[gpu] flux segment {name}", flush=True)
     for form in ("fp64", "fp32_store"):
-        for staged in (False, True):
+        outs = []
+        for seg in (32, 1, 3, 7, 1000):
             r, dom = make_redi(case, gpu_device, form)
-            r.set_face_staging(staged)
+            r.set_flux_segment(seg)
             r.step(dom, case.nsteps)
             torch.cuda.synchronize()
-            outs[(form, staged)] = dom.tracers_host()
+            outs.append(dom.tracers_host())
             r.close()
-    assert np.array_equal(outs[("fp64", True)], ref["hTrN"])
-    assert np.array_equal(outs[("fp64", False)], ref["hTrN"])
-    assert np.array_equal(outs[("fp32_store", True)], outs[("fp32_store", False)])
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0]), form
+        if form == "fp64":
+            assert np.array_equal(outs[0], ref["hTrN"])
 
 
-def test_hip_face_staging_front_473x297x75(gpu_device):
+def test_hip_flux_segment_front_473x297x75(gpu_device):
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     case = inputs.case_front(nsteps=1)
-    print("[gpu] face staging 473x297x75", flush=True)
+    print("[gpu] flux segment 473x297x75", flush=True)
     got = []
-    for staged in (False, True):
+    for seg in (32, 5, 64):
         r, dom = make_redi(case, gpu_device, "fp64")
-        r.set_face_staging(staged)
+        r.set_flux_segment(seg)
         r.step(dom, 1)
         torch.cuda.synchronize()
         got.append(dom.tracers_host())
         r.close()
     assert np.array_equal(got[0], got[1])
+    assert np.array_equal(got[0], got[2])
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32_store"])
 def test_hip_tile_order_same_bits(form, gpu_device):
-    """Vertical numbering of sweep tiles and cell-flux segments: same bits
-    (the oracle's in the FP64 form)."""
+    """Vertical numbering of the sweep tiles: same bits (the oracle's in the
+    FP64 form)."""
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     case, ref = load_golden("roquet_two_fronts_mini")
