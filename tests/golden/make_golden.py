@@ -3,6 +3,7 @@
 Run in the build container (where the reference tree exists):
 
     make -C oracle ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py --full     # the long / large digests
 
 Each small case stores its inputs AND the reference outputs in full:
 ``<name>.npz`` with the input arrays (h, hTr, metrics, wet masks, face
@@ -74,7 +75,75 @@ def out_digest(a: np.ndarray) -> dict:
             "sum": float(b.astype(np.float64).sum())}
 
 
+def stream_digests(path: str) -> dict:
+    """out_digest of every array of a REDO output file, read in chunks (the
+    1440x1080x75 outputs do not fit in memory twice)."""
+    import struct
+    res = {}
+    with open(path, "rb") as f:
+        magic, ver, nx, ny, nz, nt, ns, nsteps = struct.unpack("<8i", f.read(32))
+        specs = []
+        for p, fa, fb in (("u", ny, nx + 1), ("v", ny + 1, nx)):
+            specs += [(p + "PoL", (ns, fa, fb), "<f8"), (p + "PoR", (ns, fa, fb), "<f8"),
+                      (p + "KoL", (ns, fa, fb), "<i4"), (p + "KoR", (ns, fa, fb), "<i4"),
+                      (p + "hEff", (ns - 1, fa, fb), "<f8")]
+        specs += [("hTr1", (nt, nz, ny, nx), "<f8"), ("hTrN", (nt, nz, ny, nx), "<f8")]
+        for name, shape, dt in specs:
+            n = int(np.prod(shape))
+            item = np.dtype(dt).itemsize
+            h = hashlib.sha256()
+            mn, mx, sm = np.inf, -np.inf, 0.0
+            left = n
+            while left:
+                k = min(left, 1 << 25)
+                a = np.frombuffer(f.read(k * item), dtype=dt, count=k)
+                h.update(a.tobytes())
+                mn, mx = min(mn, float(a.min())), max(mx, float(a.max()))
+                sm += float(a.astype(np.float64).sum())
+                left -= k
+            res[name] = {"sha256": h.hexdigest(), "shape": list(shape),
+                         "dtype": str(np.dtype(dt).newbyteorder("=")), "min": mn, "max": mx, "sum": sm}
+        if f.read(1):
+            raise ValueError(f"{path}: trailing bytes")
+    return res
+
+
+def full_size(names) -> None:
+    """Digests of the long and large workloads, added to digests.json: the
+    473x297x75 front over 100 steps and the 1440x1080x75 two-front domain
+    over 1 step (compiled reference, every host core)."""
+    import tempfile
+    from more_benchmarks_dc_ocean_amd.inputs import write_input
+    work = {"front_473x297x75_100steps": lambda: inputs.case_front(nsteps=100),
+            "two_fronts_1440x1080x75_1step": lambda: inputs.case_two_fronts(nsteps=1)}
+    path = os.path.join(HERE, "digests.json")
+    with open(path) as f:
+        digests = json.load(f)
+    for name in names:
+        case = work[name]()
+        with tempfile.TemporaryDirectory(dir=os.environ.get("GOLDEN_TMP")) as td:
+            fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+            write_input(fin, case)
+            d = {"generator": case.name, "nsteps": case.nsteps, "input_sha256": input_digest(case),
+                 "land_fraction": case.land_fraction}
+            del case
+            env = dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 1))
+            import subprocess
+            p = subprocess.run([redi_oracle.REF_BIN, fin, fout, "0", "0"], env=env,
+                               capture_output=True, text=True)
+            if p.returncode != 0:
+                raise RuntimeError(p.stdout + p.stderr)
+            d["outputs"] = stream_digests(fout)
+        digests[name] = d
+        print(name, "done", flush=True)
+        with open(path, "w") as f:
+            json.dump(digests, f, indent=1, sort_keys=True)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--full":
+        full_size(sys.argv[2:] or ["front_473x297x75_100steps", "two_fronts_1440x1080x75_1step"])
+        return
     if not redi_oracle.have_reference():
         redi_oracle.build_reference()
     for case in small_cases():
