@@ -327,6 +327,30 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp,
                    const double *dx_cv, const double *idxCu,
                    const double *idyCv, const double *areaT, void *stream);
 
+/* redi_launch_nt without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * redi_launch_nt keeps its face state and workspace in a per-device cache
+ * that it hipMallocs on first use and grows; this entry keeps them in one
+ * static device array of REDI_NOALLOC_SCRATCH_BYTES (1 GiB) instead,
+ * allocated with the library's code on each device, and calls no hipMalloc,
+ * hipFree or allocating routine.  Domains whose face state and workspace
+ * need more (about 216 B per cell at 2 tracers: 473x297x30 plus ghosts
+ * fits) return REDI_ERR_BUDGET.  Every call on one device shares its array:
+ * calls that overlap on a device race.  Same arguments, results and
+ * stream semantics as redi_launch_nt. */
+int redi_launch_nt_noalloc(int nx, int ny, int nz, double dt, int nghost,
+                           int nxp, int nyp, int ww, int we, int wsz, int wn,
+                           const redi_eos_t *eos, const double *h_layer,
+                           int nt, double *const *tracers, int idx_T,
+                           int idx_S, const double *wet_u,
+                           const double *wet_v, const redi_coeffs_t *coeffs,
+                           const double *khtr_u_ext, const double *khtr_v_ext,
+                           const double *dy_cu, const double *dx_cv,
+                           const double *idxCu, const double *idyCv,
+                           const double *areaT, void *stream);
+#define REDI_NOALLOC_SCRATCH_BYTES (1ull << 30)
+
 /* ---- batched form: many independent domains in one launch per kernel -----
  * No counterpart in the reference, which steps one domain per call.  A batch
  * holds up to max_domains independent domains of any size, each with its own

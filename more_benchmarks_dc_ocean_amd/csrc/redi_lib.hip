@@ -481,6 +481,81 @@ int flat_state(int nx, int ny, int nz, int nt, FlatSlot** out) {
   return REDI_OK;
 }
 
+
+int check_flat(const char* who, int nx, int ny, int nz, const redi_eos_t* eos, int nt,
+               double* const* tracers, int idx_T, int idx_S, const redi_coeffs_t* cf) {
+  if (!eos || !cf || !tracers || nt < 1 || nx < 1 || ny < 1 || nz < 1)
+    return fail(REDI_ERR_ARG, "%s: bad argument", who);
+  if (nz > REDI_NZ_MAX) return fail(REDI_ERR_NZ, "%s: nz %d > %d", who, nz, REDI_NZ_MAX);
+  if ((double)(nz + 1) * nx * ny * kTracerGroup >= 2147483648.0)
+    return fail(REDI_ERR_ARG, "%s: domain exceeds 2^29 cell interfaces", who);
+  if (idx_T < 1 || idx_T > nt || idx_S < 1 || idx_S > nt)
+    return fail(REDI_ERR_ARG, "%s: tracer index out of range", who);
+  if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "%s: unknown EOS variant", who);
+  if (!cf->khtr_u || !cf->khtr_v) return fail(REDI_ERR_ARG, "%s: null khtr_u/khtr_v", who);
+  return REDI_OK;
+}
+
+// One flat Redi step (FP64) over caller-owned fields and the face state /
+// workspace `fs`, `ws`: Phase A, the reference-layout coefficients of the
+// computed faces, Phase B.  The caller's khtr arrays take the resolved face
+// diffusivities.
+int launch_flat(int nx, int ny, int nz, double dt, int nghost, int nxp, int nyp, int ww, int we,
+                int wsz, int wn, const redi_eos_t& eos, const double* h_layer, int nt,
+                double* const* tracers, int idx_T, int idx_S, const double* wet_u,
+                const double* wet_v, const redi_coeffs_t& cf, const double* kue, const double* kve,
+                const double* dy_cu, const double* dx_cv, const double* idxCu, const double* idyCv,
+                const double* areaT, FaceState fs, void* ws, hipStream_t st) {
+  fs.khtr_u = cf.khtr_u;
+  fs.khtr_v = cf.khtr_v;
+  const Tiles tiles;
+  int rc = launch_phase_a<Fp64>(nx, ny, nz, to_eos(eos), h_layer, tracers[idx_T - 1],
+                                tracers[idx_S - 1], wet_u, wet_v, fs, ws, tiles, st, g_no_timer);
+  if (rc) return rc;
+  rc = export_coeffs<Fp64>(fs, nx, ny, nz, cf, false, false, st);
+  if (rc) return rc;
+  PhaseB p;
+  p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;
+  p.ww = ww; p.we = we; p.ws = wsz; p.wn = wn;
+  p.khtr_u_ext = kue; p.khtr_v_ext = kve; p.khtr = 0.0;
+  p.dy_cu = dy_cu; p.dx_cv = dx_cv; p.idxCu = idxCu; p.idyCv = idyCv; p.areaT = areaT;
+  return launch_phase_b<Fp64>(p, h_layer, nt, tracers, fs, ws, tiles, st, g_no_timer);
+}
+
+// The allocation-free flat entry's scratch: one static array per device,
+// holding the FP64 face records and flags of both face grids and the step
+// workspace.  Its host-side name is not a device address; each device's
+// address comes from hipGetSymbolAddress, once, with that device current.
+// Every value a call reads from it, the call first writes (the sweep writes
+// every computed face, Phase B every cell), so nothing in it needs clearing
+// between calls; calls that overlap on one device race on it.
+constexpr size_t kNoAllocScratchBytes = REDI_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) char scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_ptr;  // per device
+
+size_t noalloc_bytes(int nx, int ny, int nz, int nt) {
+  const size_t ns = 2 * (size_t)nz + 2, nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return up(padded_faces(nu) * ns * sizeof(FaceRec64)) + up(padded_faces(nv) * ns * sizeof(FaceRec64)) +
+         up(nu) + up(nv) + up(workspace_bytes(nx, ny, nz, nt, REDI_FORM_FP64));
+}
+
+int noalloc_scratch(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(REDI_ERR_HIP, "redi_launch_nt_noalloc: no device address for the scratch");
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return REDI_OK;
+}
+
 }  // namespace
 
 struct redi_s {
@@ -827,34 +902,54 @@ int redi_launch_nt(int nx, int ny, int nz, double dt, int nghost, int nxp, int n
                    const double* wet_v, const redi_coeffs_t* cf, const double* kue,
                    const double* kve, const double* dy_cu, const double* dx_cv, const double* idxCu,
                    const double* idyCv, const double* areaT, void* stream) {
-  if (!eos || !cf || !tracers || nt < 1 || nx < 1 || ny < 1 || nz < 1)
-    return fail(REDI_ERR_ARG, "redi_launch_nt: bad argument");
-  if (nz > REDI_NZ_MAX) return fail(REDI_ERR_NZ, "redi_launch_nt: nz %d > %d", nz, REDI_NZ_MAX);
-  if ((double)(nz + 1) * nx * ny * kTracerGroup >= 2147483648.0)
-    return fail(REDI_ERR_ARG, "redi_launch_nt: domain exceeds 2^29 cell interfaces");
-  if (idx_T < 1 || idx_T > nt || idx_S < 1 || idx_S > nt)
-    return fail(REDI_ERR_ARG, "redi_launch_nt: tracer index out of range");
-  if (!valid_eos(eos->variant)) return fail(REDI_ERR_ARG, "redi_launch_nt: unknown EOS variant");
-  if (!cf->khtr_u || !cf->khtr_v) return fail(REDI_ERR_ARG, "redi_launch_nt: null khtr_u/khtr_v");
+  int rc = check_flat("redi_launch_nt", nx, ny, nz, eos, nt, tracers, idx_T, idx_S, cf);
+  if (rc) return rc;
   FlatSlot* fsl = nullptr;
-  int rc = flat_state(nx, ny, nz, nt, &fsl);
+  rc = flat_state(nx, ny, nz, nt, &fsl);
   if (rc) return rc;
-  FaceState fs = fsl->fs;
-  fs.khtr_u = cf->khtr_u;  // the caller's khtr arrays are the resolved-diffusivity slots
-  fs.khtr_v = cf->khtr_v;
-  const hipStream_t st = (hipStream_t)stream;
-  const Tiles tiles;
-  rc = launch_phase_a<Fp64>(nx, ny, nz, to_eos(*eos), h_layer, tracers[idx_T - 1],
-                            tracers[idx_S - 1], wet_u, wet_v, fs, fsl->ws, tiles, st, g_no_timer);
+  return launch_flat(nx, ny, nz, dt, nghost, nxp, nyp, ww, we, wsz, wn, *eos, h_layer, nt, tracers,
+                     idx_T, idx_S, wet_u, wet_v, *cf, kue, kve, dy_cu, dx_cv, idxCu, idyCv, areaT,
+                     fsl->fs, fsl->ws, (hipStream_t)stream);
+}
+
+int redi_launch_nt_noalloc(int nx, int ny, int nz, double dt, int nghost, int nxp, int nyp, int ww,
+                           int we, int wsz, int wn, const redi_eos_t* eos, const double* h_layer,
+                           int nt, double* const* tracers, int idx_T, int idx_S,
+                           const double* wet_u, const double* wet_v, const redi_coeffs_t* cf,
+                           const double* kue, const double* kve, const double* dy_cu,
+                           const double* dx_cv, const double* idxCu, const double* idyCv,
+                           const double* areaT, void* stream) {
+  int rc = check_flat("redi_launch_nt_noalloc", nx, ny, nz, eos, nt, tracers, idx_T, idx_S, cf);
   if (rc) return rc;
-  rc = export_coeffs<Fp64>(fs, nx, ny, nz, *cf, false, false, st);
+  const size_t need = noalloc_bytes(nx, ny, nz, nt);
+  if (need > kNoAllocScratchBytes)
+    return fail(REDI_ERR_BUDGET,
+                "redi_launch_nt_noalloc: %d x %d x %d with %d tracers needs %zu B of scratch; the "
+                "static scratch holds %zu B",
+                nx, ny, nz, nt, need, kNoAllocScratchBytes);
+  char* base = nullptr;
+  rc = noalloc_scratch(&base);
   if (rc) return rc;
-  PhaseB p;
-  p.nx = nx; p.ny = ny; p.nz = nz; p.nghost = nghost; p.nxp = nxp; p.nyp = nyp; p.dt = dt;
-  p.ww = ww; p.we = we; p.ws = wsz; p.wn = wn;
-  p.khtr_u_ext = kue; p.khtr_v_ext = kve; p.khtr = 0.0;
-  p.dy_cu = dy_cu; p.dx_cv = dx_cv; p.idxCu = idxCu; p.idyCv = idyCv; p.areaT = areaT;
-  return launch_phase_b<Fp64>(p, h_layer, nt, tracers, fs, fsl->ws, tiles, st, g_no_timer);
+  FaceState fs;
+  const size_t ns = 2 * (size_t)nz + 2, rb = sizeof(FaceRec64);
+  fs.nu = (size_t)(nx + 1) * ny;
+  fs.nv = (size_t)nx * (ny + 1);
+  fs.nup = padded_faces(fs.nu);
+  fs.nvp = padded_faces(fs.nv);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  fs.u = take(fs.nup * ns * rb);
+  fs.v = take(fs.nvp * ns * rb);
+  fs.mono_u = (uint8_t*)take(fs.nu);
+  fs.mono_v = (uint8_t*)take(fs.nv);
+  void* ws = take(workspace_bytes(nx, ny, nz, nt, REDI_FORM_FP64));
+  return launch_flat(nx, ny, nz, dt, nghost, nxp, nyp, ww, we, wsz, wn, *eos, h_layer, nt, tracers,
+                     idx_T, idx_S, wet_u, wet_v, *cf, kue, kve, dy_cu, dx_cv, idxCu, idyCv, areaT,
+                     fs, ws, (hipStream_t)stream);
 }
 
 int redi_timing_enable(redi_t* r, int on) {

@@ -74,7 +74,8 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_set_tracer_group",
             "redi_calc_coeffs", "redi_apply_flux", "redi_apply_flux_hoist", "redi_step",
             "redi_workspace_bytes", "redi_last_error", "redi_version", "redi_opt_launch",
-            "redi_cuda_launch_", "redi_launch_nt", "redi_timing_enable", "redi_timing_collect",
+            "redi_cuda_launch_", "redi_launch_nt", "redi_launch_nt_noalloc",
+            "redi_timing_enable", "redi_timing_collect",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
             "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
@@ -140,6 +141,7 @@ def lib() -> ctypes.CDLL:
         L.redi_launch_nt.argtypes = [_I, _I, _I, _D, _I, _I, _I, _I, _I, _I, _I,
                                      ctypes.POINTER(Eos), _P, _I, ctypes.POINTER(_P), _I, _I,
                                      _P, _P, ctypes.POINTER(Coeffs), _P, _P, _P, _P, _P, _P, _P, _P]
+        L.redi_launch_nt_noalloc.argtypes = L.redi_launch_nt.argtypes
         L.redi_timing_enable.argtypes = [_P, _I]
         L.redi_timing_collect.argtypes = [_P, ctypes.POINTER(KernelTimes)]
         L.redi_kernel_name.restype = ctypes.c_char_p
