@@ -80,14 +80,32 @@ def _digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def _input_digest(case):
+    h = hashlib.sha256()
+    for a in ("h", "hTr", "dy_cu", "dx_cv", "idxCu", "idyCv", "areaT", "wet_u", "wet_v",
+              "khtr_u_ext", "khtr_v_ext"):
+        h.update(np.ascontiguousarray(getattr(case, a), dtype="<f8").tobytes())
+    return h.hexdigest()
+
+
+def _assert_digests(got, d, case, label, names=OUTS):
+    """Every output in `names` is byte-identical to the reference build's
+    (SHA-256 of the little-endian bytes), on the inputs the reference ran
+    (the generator's digest pins them)."""
+    assert _input_digest(case) == d["input_sha256"], f"{label}: generated inputs differ"
+    bad = [k for k in names if _digest(got[k]) != d["outputs"][k]["sha256"]]
+    print(f"[gpu] {label}: {len(names) - len(bad)}/{len(names)} outputs match the reference's "
+          f"digests", flush=True)
+    assert not bad, f"{label}: outputs differ from the reference build: {bad}"
+
+
 def test_hip_full_seamount_64x64x25_against_reference_digests(gpu_device):
     d = load_digests()["seamount_64x64x25_10steps"]
     case = inputs.case_seamount(nsteps=10)
     got = _run_hip(case, 10, gpu_device)
     orc = redi_oracle.run_oracle(case)
     _compare(got, orc, "seamount 64x64x25 x10 vs oracle")
-    same = [k for k in OUTS if _digest(got[k]) == d["outputs"][k]["sha256"]]
-    print(f"[gpu] seamount 64x64x25 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
+    _assert_digests(got, d, case, "seamount 64x64x25 x10")
 
 
 def test_hip_full_front_473x297x75_one_step(gpu_device):
@@ -99,8 +117,7 @@ def test_hip_full_front_473x297x75_one_step(gpu_device):
     redi_oracle.set_threads(16)
     orc = redi_oracle.run_oracle(case, nsteps=1)
     _compare(got, orc, "front 473x297x75 x1 vs oracle")
-    same = [k for k in OUTS if _digest(got[k]) == d["outputs"][k]["sha256"]]
-    print(f"[gpu] front 473x297x75 digests matching the reference: {len(same)}/{len(OUTS)}", flush=True)
+    _assert_digests(got, d, case, "front 473x297x75 x1")
 
 
 @pytest.mark.parametrize("seed,eos,nz,walls,use_ext", [
