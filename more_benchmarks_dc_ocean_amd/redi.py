@@ -169,6 +169,17 @@ def _stream(device) -> _P:
     return _P(torch.cuda.current_stream(device).cuda_stream)
 
 
+def sanity(hTr_T) -> str:
+    """The reference driver's sanity rule for a temperature field after a
+    run (redi/drivers/dc_main.F90:149-153): "garbage" when its minimum is
+    NaN or the field is all zero, else "ok"."""
+    a = np.asarray(hTr_T)
+    lo, hi = a.min(), a.max()
+    if np.isnan(lo) or (lo == 0.0 and hi == 0.0):
+        return "garbage"
+    return "ok"
+
+
 class DeviceDomain:
     """One ``RediCase`` resident in HBM plus its ctypes descriptors.  In the
     FP32-store form h and the tracers are held in FP32 (rounded from the

@@ -88,3 +88,41 @@ def gpu_device():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     return "cuda:0"
+
+
+def nan_case():
+    """A small Roquet domain whose temperature holds one NaN in a wet
+    interior cell (the input the reference's sanity stop exists for,
+    redi/drivers/dc_main.F90:149-153); returns (case, (j, i) of the NaN's
+    column)."""
+    import dataclasses
+    from more_benchmarks_dc_ocean_amd import inputs
+    c = inputs.case_front(14, 12, 25, nt=2, nsteps=2, seed=6)
+    c = dataclasses.replace(c, name="nan_input_14x12x25", hTr=c.hTr.copy())
+    g = c.nghost
+    wet = np.argwhere(c.h[c.nz - 3, g + 3:-g - 3, g + 3:-g - 3] > 0)
+    j, i = wet[len(wet) // 2] + g + 3
+    c.hTr[0, c.nz - 3, j, i] = np.nan  # top-down layer 3 of a wet interior column
+    return c, (int(j), int(i))
+
+
+def away_from(shape, where, r, face=None):
+    """Mask of a (..., ny', nx') array: True where the column (for u / v
+    faces: both columns of the face) is more than r columns from `where` in
+    i or j."""
+    ny, nx = shape[-2], shape[-1]
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    j0, i0 = where
+    far = lambda j, i: (np.abs(j - j0) > r) | (np.abs(i - i0) > r)
+    m = far(jj, ii)
+    if face == "u":
+        m = far(jj, ii) & far(jj, ii - 1)
+    elif face == "v":
+        m = far(jj, ii) & far(jj - 1, ii)
+    return np.broadcast_to(m, shape)
+
+
+def same_with_nan(a: np.ndarray, b: np.ndarray) -> bool:
+    """Same NaN positions and the same bits everywhere else."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return bool(np.array_equal(na, nb) and np.array_equal(np.where(na, 0, a), np.where(nb, 0, b)))

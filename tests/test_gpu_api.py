@@ -134,7 +134,9 @@ def test_full_size_invariants_two_fronts_1440x1080x75(gpu_device):
     size.  Checks the properties that hold at any size: all values finite,
     the total of each tracer over the interior conserved (the walls stop
     exchange with the ghosts) to rounding, a spatially uniform passive
-    tracer left bit-identical, and run-to-run determinism."""
+    tracer left bit-identical, the other tracers changed on many wet cells,
+    and run-to-run determinism.  (The bits of one step at this size are
+    pinned to the reference build's digests in test_gpu_parity.py.)"""
     import torch
     from more_benchmarks_dc_ocean_amd.redi import make_redi
     case = inputs.case_two_fronts(nsteps=2)
@@ -153,6 +155,12 @@ def test_full_size_invariants_two_fronts_1440x1080x75(gpu_device):
         tot = (a[t] * area)[:, g:-g, g:-g].sum()
         assert abs(tot - tot0[t]) <= 1e-12 * abs(tot0[t]), (t, tot, tot0[t])
     assert np.array_equal(a[3], case.hTr[3])
+    # a step that changed nothing would pass every check above: T, S and the
+    # non-uniform passive tracer must actually move on many wet cells
+    wet = np.broadcast_to(case.h > 0, a[0].shape)
+    for t in range(3):
+        moved = np.count_nonzero((a[t] != case.hTr[t]) & wet)
+        assert moved > 0.05 * np.count_nonzero(wet), (t, moved)
     dom.set_tracers(case.hTr)
     r.step(dom, 2)
     torch.cuda.synchronize()

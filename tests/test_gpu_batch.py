@@ -44,8 +44,8 @@ def _batch(form, cases, device, persist=1, arena=None):
     from more_benchmarks_dc_ocean_amd.batch import RediBatch, domain_bytes
     from more_benchmarks_dc_ocean_amd.redi import DeviceDomain
     need = sum(domain_bytes(c.nx, c.ny, c.nz, form, 4) for c, _ in cases)
-    b = RediBatch(form=form, group=4, max_domains=16, arena_bytes=arena or need + (1 << 20),
-                  persist=persist, device=device)
+    b = RediBatch(form=form, group=4, max_domains=max(16, len(cases)),
+                  arena_bytes=arena or need + (1 << 20), persist=persist, device=device)
     doms, slots = [], []
     for c, shape in cases:
         d = DeviceDomain(c, device, form)
@@ -223,4 +223,82 @@ def test_slope_taper_matches_oracle_single_and_batched(cases, gpu_device):
         assert np.array_equal(dom.tracers_host(), ref["hTrN"]), f"{c.name}: single tapered step"
         assert np.array_equal(d.tracers_host(), ref["hTrN"]), f"{c.name}: batched tapered step"
         r.close()
+    b.close()
+
+
+def _interior_totals(case, hTr):
+    """sum(areaT * hTr) over the interior of every tracer, in FP64."""
+    g, nxp, nyp = case.nghost, case.nx_phys, case.ny_phys
+    a = case.areaT[g:g + nyp, g:g + nxp]
+    return np.array([(hTr[t][:, g:g + nyp, g:g + nxp].astype(np.float64) * a).sum()
+                     for t in range(hTr.shape[0])])
+
+
+def test_batch_96_domain_workload(gpu_device):
+    """The batched workload as specified -- inputs.case_batch(96): 96 domains,
+    square sides log-uniform in [48, 512], nz in {25, 50, 75}, each with its
+    own seeded bathymetry and T/S, Roquet EOS, 4 tracers -- in the
+    FP32-store form with the tuned per-domain launch shapes, 2 steps.  For
+    every domain: the batched (one hipGraph launch) bits equal the one
+    launch sequence per domain's, values stay finite, each tracer's
+    interior total is conserved, and T actually changes."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
+    cases = inputs.case_batch(96, seed=0, nsteps=2)
+    sides = [c.nx_phys for c in cases]
+    assert len(cases) == 96 and min(sides) >= 48 and max(sides) <= 512
+    assert {c.nz for c in cases} == {25, 50, 75}
+    cache = TuneCache(TUNED)
+    shapes = [cache.batch_shape(c.nx, c.ny, c.nz, "fp32_store") for c in cases]
+    print(f"[gpu] batch 96 domains, {sum(c.interior_cells for c in cases) / 1e6:.0f} M cells", flush=True)
+    b, doms, _ = _batch("fp32_store", list(zip(cases, shapes)), gpu_device)
+    b.step(2)
+    torch.cuda.synchronize()
+    batched = [d.tracers_host() for d in doms]
+    for c, d in zip(cases, doms):
+        d.set_tracers(c.hTr)
+    b.step_serial(2)
+    torch.cuda.synchronize()
+    for i, (c, d, x) in enumerate(zip(cases, doms, batched)):
+        y = d.tracers_host()
+        assert np.array_equal(x, y), f"domain {i} {c.name}: batched != one launch sequence per domain"
+        assert np.isfinite(x).all(), f"domain {i}: non-finite values"
+        h0 = c.hTr.astype(np.float32)
+        t0, t1 = _interior_totals(c, h0), _interior_totals(c, x)
+        rel = np.abs(t1 - t0) / np.abs(t0)
+        assert (rel < 1e-6).all(), f"domain {i}: interior totals moved by {rel}"
+        assert np.count_nonzero(x[0] != h0[0]) > 0.01 * x[0].size, f"domain {i}: T did not change"
+    b.close()
+
+
+# six domains of the 96-domain workload: the smallest and the two largest
+# sides, and a mid-size domain of each layer count
+CFG3_SUBSET = [54, 32, 58, 94, 16, 48]
+
+
+def test_batch_fp64_subset_of_96_domain_workload_matches_oracle(gpu_device):
+    """Six domains of inputs.case_batch(96) (48^2 to 506^2, nz 25 / 50 / 75)
+    batched in the plain FP64 form: Phase-A arrays after step 1 and hTr
+    after steps 1 and 2 are the oracle's bits."""
+    import torch
+    every = inputs.case_batch(96, seed=0, nsteps=2)
+    cases = [(every[i], None) for i in CFG3_SUBSET]
+    del every
+    assert {c.nz for c, _ in cases} == {25, 50, 75}
+    assert min(c.nx_phys for c, _ in cases) == 48 and max(c.nx_phys for c, _ in cases) >= 490
+    print(f"[gpu] batch fp64: 6 domains of the 96-domain workload vs oracle", flush=True)
+    b, doms, slots = _batch("fp64", cases, gpu_device)
+    b.step(1)
+    torch.cuda.synchronize()
+    got1 = [(d.tracers_host(), {k: v.cpu().numpy() for k, v in b.coeffs(s).items()})
+            for d, s in zip(doms, slots)]
+    b.step(1)
+    torch.cuda.synchronize()
+    redi_oracle.set_threads(16)
+    for (c, _), d, (h1, co) in zip(cases, doms, got1):
+        ref = redi_oracle.run_oracle(c, nsteps=2)
+        for k, v in co.items():
+            assert np.array_equal(v, ref[k]), f"{c.name}: {k} differs from the oracle"
+        assert np.array_equal(h1, ref["hTr1"]), f"{c.name}: hTr after step 1 differs"
+        assert np.array_equal(d.tracers_host(), ref["hTrN"]), f"{c.name}: hTr after step 2 differs"
     b.close()

@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import OUTS, golden_names, load_digests, load_golden, rel_err
+from conftest import OUTS, golden_names, load_digests, load_golden, nan_case, rel_err, same_with_nan
 
 import redi_oracle
 from more_benchmarks_dc_ocean_amd import inputs
@@ -92,8 +92,11 @@ def _assert_digests(got, d, case, label, names=OUTS):
     """Every output in `names` is byte-identical to the reference build's
     (SHA-256 of the little-endian bytes), on the inputs the reference ran
     (the generator's digest pins them)."""
+    from concurrent.futures import ThreadPoolExecutor
     assert _input_digest(case) == d["input_sha256"], f"{label}: generated inputs differ"
-    bad = [k for k in names if _digest(got[k]) != d["outputs"][k]["sha256"]]
+    with ThreadPoolExecutor(8) as ex:  # hashlib releases the GIL on large buffers
+        dig = dict(zip(names, ex.map(lambda k: _digest(got[k]), names)))
+    bad = [k for k in names if dig[k] != d["outputs"][k]["sha256"]]
     print(f"[gpu] {label}: {len(names) - len(bad)}/{len(names)} outputs match the reference's "
           f"digests", flush=True)
     assert not bad, f"{label}: outputs differ from the reference build: {bad}"
@@ -118,6 +121,26 @@ def test_hip_full_front_473x297x75_one_step(gpu_device):
     orc = redi_oracle.run_oracle(case, nsteps=1)
     _compare(got, orc, "front 473x297x75 x1 vs oracle")
     _assert_digests(got, d, case, "front 473x297x75 x1")
+
+
+def test_hip_front_473x297x75_100_steps_reference_digests(gpu_device):
+    """The 473x297x75 front workload as specified (Roquet, 4 tracers, 100
+    steps): the Phase-A arrays and hTr after step 1, and hTr after step 100,
+    byte-identical to the reference build's over the same 100 steps."""
+    d = load_digests()["front_473x297x75_100steps"]
+    case = inputs.case_front(nsteps=100)
+    got = _run_hip(case, 100, gpu_device)
+    _assert_digests(got, d, case, "front 473x297x75 x100")
+
+
+def test_hip_two_fronts_1440x1080x75_one_step_reference_digests(gpu_device):
+    """The headline 1440x1080x75 two-front workload (Roquet, 4 tracers): the
+    Phase-A arrays and hTr after one step byte-identical to the reference
+    build's."""
+    d = load_digests()["two_fronts_1440x1080x75_1step"]
+    case = inputs.case_two_fronts(nsteps=1)
+    got = _run_hip(case, 1, gpu_device)
+    _assert_digests(got, d, case, "two fronts 1440x1080x75 x1")
 
 
 @pytest.mark.parametrize("seed,eos,nz,walls,use_ext", [
@@ -254,3 +277,52 @@ def test_hip_tile_order_same_bits(form, gpu_device):
     assert np.array_equal(got[0], got[1])
     if form == "fp64":
         assert np.array_equal(got[1], ref["hTrN"])
+
+
+@pytest.mark.parametrize("nz", [100, 256])
+def test_hip_deep_columns_match_oracle_alone_and_batched(nz, gpu_device):
+    """The two deepest layer-capacity classes of the flux kernels (nz up to
+    128 and up to 256, the library's limit): a 12x10 Roquet domain with
+    vanished layers, 2 steps, alone and in a batch, against the oracle."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.batch import RediBatch, domain_bytes
+    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain
+    c = inputs.case_fractal(9, 12, 10, nz, 4, 2, seed=31 + nz, land=0.25,
+                            fronts=[("y", 0.5, 1.0, 40_000.0)], noise_sigma=2.0, noise_T=0.5,
+                            noise_S=0.05, eos_variant=inputs.EOS_ROQUET_SPV, vanished=True)
+    ref = redi_oracle.run_oracle(c, nsteps=2)
+    got = _run_hip(c, 2, gpu_device)
+    _compare(got, ref, f"deep nz={nz} alone")
+    assert all(np.array_equal(got[k], ref[k]) for k in OUTS)
+    b = RediBatch(form="fp64", group=4, max_domains=2,
+                  arena_bytes=domain_bytes(c.nx, c.ny, c.nz, "fp64", 4), device=gpu_device)
+    d = DeviceDomain(c, gpu_device, "fp64")
+    b.add(d)
+    b.step(2)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.tracers_host(), ref["hTrN"]), f"deep nz={nz} batched"
+    b.close()
+
+
+def test_hip_nan_input_same_as_oracle(gpu_device):
+    """One NaN in the temperature input (the case the reference driver's
+    sanity stop catches, redi/drivers/dc_main.F90:149-153).  NaN spreads
+    through the stencil and, where a branch compares a NaN's sign (the PLM
+    and guard signums use copysign), the NaN's sign bit decides it; x86 and
+    the GPU give arithmetic on a NaN different sign bits, so the cells near
+    the NaN may take other branches than the oracle's.  Bar: the sanity rule
+    flags the result as garbage, as it does the reference's, and every
+    array agrees with the oracle bit for bit beyond 2 columns per step from
+    the NaN's column."""
+    from conftest import away_from
+    from more_benchmarks_dc_ocean_amd.redi import sanity
+    c, where = nan_case()
+    got = _run_hip(c, 2, gpu_device)
+    ref = redi_oracle.run_oracle(c)
+    assert sanity(got["hTrN"][0]) == "garbage"
+    for k in OUTS:
+        face = "u" if k.startswith("u") else ("v" if k.startswith("v") else None)
+        r = 2 if k in ("hTr1",) or face else 4
+        m = away_from(got[k].shape, where, r, face)
+        assert same_with_nan(got[k][m], ref[k][m]), k
+        assert not np.isnan(got[k][m]).any(), k

@@ -13,7 +13,8 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import OUTS, golden_names, input_digest, load_digests, load_golden, rel_err
+from conftest import (OUTS, golden_names, input_digest, load_digests, load_golden, nan_case,
+                      rel_err, same_with_nan)
 
 import redi_oracle
 from more_benchmarks_dc_ocean_amd import inputs
@@ -83,3 +84,19 @@ def test_verifier_detects_one_ulp():
     assert not np.array_equal(bad, ref["hTrN"])
     pc, fr = rel_err(bad, ref["hTrN"])
     assert 0.0 < pc < 1e-15 and fr > 0.0
+
+
+@pytest.mark.skipif(not redi_oracle.have_reference(), reason="oracle/_ref not built")
+def test_oracle_nan_input_matches_reference():
+    """One NaN in the temperature input: the reference build and the oracle
+    spread it to the same cells and agree bit for bit everywhere else, and
+    the reference driver's sanity rule calls the result garbage."""
+    from more_benchmarks_dc_ocean_amd.redi import sanity
+    c, _ = nan_case()
+    ref, _ = redi_oracle.run_reference(c, threads=4)
+    got = redi_oracle.run_oracle(c)
+    for k in OUTS:
+        assert same_with_nan(got[k], ref[k]), k
+    assert np.isnan(ref["hTrN"][0]).any()
+    assert sanity(ref["hTrN"][0]) == "garbage"
+    assert sanity(c.hTr[1]) == "ok"
