@@ -166,6 +166,19 @@ int redi_set_recon_order(redi_t *self, int layer_major);
  * into the FP32 state. */
 int redi_set_tendency_output(redi_t *self, int ntracers, double *const *tend);
 
+/* Optional diagnostic output of the down-gradient guard's decisions
+ * (redi/ocean_redi.F90:913-924) in every later redi_apply_flux: for tracer
+ * t, face f and chunk c of 63 sublayers (sublayers 63c .. 63c+62), bit b of
+ * u_bits[(t*nu + f)*nch + c] is set when sublayer 63c+b contributes
+ * (hEff != 0) and the guard zeroed its flux; nu = (nx+1)*ny u faces (v_bits:
+ * nv = nx*(ny+1)), nch = ceil((2*nz+1)/63), device arrays of uint64 sized
+ * ntracers*nu*nch and ntracers*nv*nch.  Words of faces Phase B does not
+ * compute (edges, walls, faces without a contributing sublayer) are not
+ * written.  NULL, NULL turns it off.  No counterpart in the reference: it
+ * lets the number of guard decisions that differ between two runs (the
+ * precision forms, rounded inputs) be counted. */
+int redi_set_guard_output(redi_t *self, uint64_t *u_bits, uint64_t *v_bits);
+
 /* Opt-in neutral-slope taper (no counterpart in the reference, which does
  * not taper): in every later redi_calc_coeffs, a sublayer whose neutral
  * slope |S| = |zR - zL| / dx (z the sublayer's mid-depth in the right / left

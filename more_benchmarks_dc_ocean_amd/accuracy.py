@@ -21,6 +21,10 @@ the reference field's max / RMS:
   hTr[t]      every tracer after N steps: C vs A
   PoL..vhEff  the step-1 Phase-A coefficients: C vs B; KoL/KoR as the
               fraction of (face, surface) entries whose layer differs
+  guard       per tracer, the step-1 sublayer fluxes the down-gradient
+              guard (redi/ocean_redi.F90:913-924) zeroed in each run, and
+              how many of those decisions differ between the runs
+              (redi_set_guard_output)
 
 Run: python -m more_benchmarks_dc_ocean_amd.accuracy [--steps 50] [--out file.md]
 """
@@ -43,13 +47,19 @@ def _run(case, form: str, nsteps: int, dev: str, want_coeffs: bool):
     r, dom = make_redi(case, dev, form)
     tend = torch.zeros(dom.hTr.shape, dtype=torch.float64, device=dev)
     r.set_tendency_output(tend)
+    nch = r.guard_chunks()
+    gu = torch.zeros((case.nt, (case.nx + 1) * case.ny, nch), dtype=torch.int64, device=dev)
+    gv = torch.zeros((case.nt, case.nx * (case.ny + 1), nch), dtype=torch.int64, device=dev)
+    r.set_guard_output(gu, gv)
     r.step(dom, 1)
     r.set_tendency_output(None)
+    r.set_guard_output(None, None)
+    guard = [torch.cat([gu[t].flatten(), gv[t].flatten()]) for t in range(case.nt)]
     co = r.coeffs() if want_coeffs else None
     r.step(dom, nsteps - 1)
     torch.cuda.synchronize()
     h = dom.h.double()
-    out = {"tend": tend, "hTr": dom.hTr.double(), "h": h, "coeffs": co}
+    out = {"tend": tend, "hTr": dom.hTr.double(), "h": h, "coeffs": co, "guard": guard}
     r.close()
     del dom
     torch.cuda.empty_cache()
@@ -99,7 +109,25 @@ def report(case, nsteps: int, dev: str = "cuda:0", log=print) -> dict:
             F[f"{k} (C vs B)"] = {"mismatch_frac": float((a != b).double().mean())}
         else:
             F[f"{k} (C vs B)"] = _err(a, b)
+    res["guard"] = guard_flips(A["guard"], B["guard"], C["guard"], names)
     return res
+
+
+def _popcount(x) -> int:
+    return int(np.bitwise_count(x.cpu().numpy().view(np.uint64)).sum())
+
+
+def guard_flips(ga, gb, gc, names) -> dict:
+    """Per tracer: sublayers the guard zeroed in runs A, B, C and the
+    decisions that differ between them (popcounts of XORed bit masks)."""
+    out = {}
+    for t, n in enumerate(names):
+        out[n] = {"zeroed_A": _popcount(ga[t]), "zeroed_B": _popcount(gb[t]),
+                  "zeroed_C": _popcount(gc[t]),
+                  "flips_B_vs_A": _popcount(ga[t] ^ gb[t]),
+                  "flips_C_vs_A": _popcount(ga[t] ^ gc[t]),
+                  "flips_C_vs_B": _popcount(gb[t] ^ gc[t])}
+    return out
 
 
 def to_markdown(res: dict, cmd: str) -> str:
@@ -117,6 +145,14 @@ def to_markdown(res: dict, cmd: str) -> str:
         else:
             lines.append(f"| {k} | {e['max_abs']:.3e} | {e['rms_abs']:.3e} | {e['max_rel']:.3e} | "
                          f"{e['rms_rel']:.3e} |")
+    if res.get("guard"):
+        lines += ["", "Down-gradient guard decisions of step 1 (sublayer fluxes zeroed, and "
+                  "decisions that differ between runs):", "",
+                  "| tracer | zeroed A | zeroed B | zeroed C | differ B vs A | differ C vs A | "
+                  "differ C vs B |", "|---|---|---|---|---|---|---|"]
+        for n, g in res["guard"].items():
+            lines.append(f"| {n} | {g['zeroed_A']} | {g['zeroed_B']} | {g['zeroed_C']} | "
+                         f"{g['flips_B_vs_A']} | {g['flips_C_vs_A']} | {g['flips_C_vs_B']} |")
     return "\n".join(lines) + "\n"
 
 

@@ -216,7 +216,7 @@ struct BatchOps {
       const R* h = (const R*)d.h;
       R* const* tr = (R* const*)d.tr.data();
       // Phase A (launch_phase_a)
-      ColRec* rec = (ColRec*)d.ws;
+      typename F::Col* rec = (typename F::Col*)d.ws;
       double* P = (double*)(rec + (size_t)N2 * (nz + 1));
       w[kNodeColumn].add(ColumnArgs<F>{N2, nz, nx, h, tr[d.idx_t - 1], tr[d.idx_s - 1], d.eos, rec, P},
                          blocks_for(N2, 128));
@@ -260,7 +260,7 @@ struct BatchOps {
       ua.nt = NT;
       const ReconLayout ry = recon_layout(0, N2, nz, NT);
       fx.ry = fy.ry = ry;
-      w[kNodeRecon].add(Recon{ReconArgs<F, NT>{N2, nz, h, ts, (ReconRec*)sc.recon, ry}, d.tiles.recon_kt, 0},
+      w[kNodeRecon].add(Recon{ReconArgs<F, NT>{N2, nz, h, ts, (typename F::Recon*)sc.recon, ry}, d.tiles.recon_kt, 0},
                         recon_tiled_blocks(N2, nz, d.tiles.recon_kt));
       const int fc = flux_class(nz);
       w[kNodeFluxX0 + fc].add(fx, nbx);

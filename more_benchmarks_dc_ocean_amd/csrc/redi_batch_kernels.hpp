@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconIt
 // no barrier between blocks.
 template <class F, int NT, int NZC, bool kY>
 __global__ void __launch_bounds__(64) kb_flux(BatchView<FluxArgs<F>> v) {
-  __shared__ FluxLds<NT, NZC> lds;
+  __shared__ FluxLds<typename F::C, NT, NZC> lds;
   for_each_block<false>(v, [&](const FluxArgs<F>& a, int bid, int nb) {
     flux_body<F, NT, NZC, kY>(a, bid, nb, lds);
   });

@@ -53,13 +53,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "redi_column.hpp"
 #include "redi_eos.hpp"
 
 namespace redi {
 
-struct alignas(16) ColRec {  // one interface of a column (K1 output)
-  double T, S, dT, dS;        // interface T, S, dR/dT, dR/dS
+// One interface of a column (K1 output): interface T and S (FP64 in both
+// forms: the sweep's density differences are differences of them) and
+// dR/dT, dR/dS in the form's EOS type E (32 bytes in FP64, 24 in FP32).
+template <class E>
+struct alignas(sizeof(E) == 8 ? 16 : 8) ColRecT {
+  double T, S;
+  E dT, dS;
 };
 
 // One neutral surface s of a face (K2 output), as the kernels use it:
@@ -106,40 +113,67 @@ struct alignas(16) FaceRec32 {
 // One layer of a tracer column, as the face kernel needs it: layer mean,
 // limited top / bottom edge values, interface values at the layer's top /
 // bottom.
+template <class C>
 struct LayerVals {
-  double Tl, aL, aR, Tt, Tb;
+  C Tl, aL, aR, Tt, Tb;
 };
 
-// One layer of a tracer column (K3 output), FP64 in both forms: the layer
-// mean, the PPM interface values at its top and bottom, and the branch the
-// PPM limiter took.  Every branch yields the limited edges from exactly
-// these three values (kept, flattened to the mean, or one edge rebuilt from
-// the mean and the other), so the reader replays the branch on the same
-// operands and obtains the same bits.  Every value a guard compares across
-// a face is then exact in both forms -- the sweep places neutral surfaces
-// where the T edges of the two columns nearly agree, so rounding them
-// would flip guards.
-struct alignas(16) ReconRec {
+// One layer of a tracer column (K3 output) in the plain FP64 form: the
+// layer mean, the PPM interface values at its top and bottom, and the
+// branch the PPM limiter took.  Every branch yields the limited edges from
+// exactly these three values (kept, flattened to the mean, or one edge
+// rebuilt from the mean and the other), so the reader replays the branch on
+// the same operands and obtains the same bits.  32 bytes, all written.
+struct alignas(16) ReconRec64 {
   double Tl, Tt, Tb;
   int32_t branch, pad;
-  static __device__ __forceinline__ ReconRec make(double tl, double et, double eb) {
+  static __device__ __forceinline__ ReconRec64 make(double tl, double et, double eb) {
     double al = et, ar = eb;
-    return ReconRec{tl, et, eb, ppm_limit<double>(tl, al, ar), 0};
+    return ReconRec64{tl, et, eb, ppm_limit<double>(tl, al, ar), 0};
   }
-  __device__ __forceinline__ LayerVals get() const {
+  __device__ __forceinline__ LayerVals<double> get() const {
     double al, ar;
     ppm_limited_edges<double>(branch, Tl, Tt, Tb, al, ar);
-    return LayerVals{Tl, al, ar, Tt, Tb};
+    return LayerVals<double>{Tl, al, ar, Tt, Tb};
   }
 };
 
+// The same layer in the FP32-store form, in 16 bytes: the FP64 layer mean
+// and the interface values at its top and bottom as FP32 offsets from it.
+// Differences between columns and layers are then formed as an FP64
+// difference of means plus an FP32 difference of small offsets, so the
+// down-gradient guard sees them to near-FP64 accuracy while the rest of
+// the sublayer arithmetic runs in FP32 (the PPM limiter too, rerun on the
+// offsets by the reader).
+struct alignas(16) ReconRecOff {
+  double Tl;
+  float oT, oB;
+  static __device__ __forceinline__ ReconRecOff make(double tl, double et, double eb) {
+    return ReconRecOff{tl, (float)(et - tl), (float)(eb - tl)};
+  }
+};
+
+// The precision forms.  R: stored fields (h, hTr); E: the EOS and its
+// stored derivatives; C: the sublayer-flux arithmetic and the stored fluxes.
+// In both forms the T/S interface values and the layer means are FP64 (the
+// density differences that place the neutral surfaces, and the tracer
+// differences the down-gradient guard tests, are differences of them), as
+// are positions, pressures and the tendency accumulation.
 struct Fp64 {
-  using R = double;  // stored fields and fluxes
+  using R = double;
+  using E = double;
+  using C = double;
   using Face = FaceRec64;
+  using Col = ColRecT<double>;
+  using Recon = ReconRec64;
 };
 struct Fp32 {
   using R = float;
+  using E = float;
+  using C = float;
   using Face = FaceRec32;
+  using Col = ColRecT<float>;
+  using Recon = ReconRecOff;
 };
 
 template <class F> using FaceRecF = typename F::Face;
@@ -193,10 +227,9 @@ struct TileMap {
 // ---------------------------------------------------------------------------
 // K1: column interface state (redi_build_column, redi/ocean_redi.F90:609-654)
 // ---------------------------------------------------------------------------
-// FP64 in both forms, the EOS included (the FP32-store form reads FP32
-// fields): the sweep's density differences are differences of these
-// derivatives' products, and FP32 derivatives move neutral surfaces into
-// other layers.
+// The layer values and PPM edges in FP64 and the EOS derivatives in the
+// form's EOS type; the interface pressure is accumulated in FP64 in both
+// forms (the sweep's positions and thicknesses are differences of it).
 // Column state layout: interface-major within each row of columns, the
 // record of interface kt (top-down) of column (i, j) at (j*(nz+1) + kt)*nx + i.
 // Neighbouring faces sit at similar depths, so the sweep's lanes (adjacent
@@ -214,7 +247,7 @@ struct ColumnArgs {
   const typename F::R* thtr;
   const typename F::R* shtr;
   EosParams eos;
-  ColRec* rec;
+  typename F::Col* rec;
   double* P;
 };
 
@@ -229,26 +262,28 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
   const typename F::R* __restrict__ thtr = a.thtr;
   const typename F::R* __restrict__ shtr = a.shtr;
   const EosParams eos = a.eos;
+  using C = double;
+  using E = typename F::E;
   const double g_rho0 = kGravity * eos.rho0;
   const size_t cb = colstate_base((size_t)c, a.nx, nz), cs = (size_t)a.nx;
-  ColRec* r = a.rec + cb;
+  typename F::Col* r = a.rec + cb;
   double* pc = a.P + cb;
   double P = 0.0;  // Pint(1) = 0
-  auto load = [&](int kt, double& hh, double* s) {
+  auto load = [&](int kt, C& hh, C* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
-    hh = (double)h[o];
-    const double he = hh > kHDivEps ? hh : kHDivEps;
-    s[0] = (double)thtr[o] / he;
-    s[1] = (double)shtr[o] / he;
+    hh = (C)h[o];
+    const C he = hh > C(kHDivEps) ? hh : C(kHDivEps);
+    s[0] = (C)thtr[o] / he;
+    s[1] = (C)shtr[o] / he;
   };
-  auto emit = [&](int k, const double* e, const double*) {
+  auto emit = [&](int k, const C* e, const C*) {
     if (k > 0) P = P + g_rho0 * (double)h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
-    double dRdT, dRdS;
-    interface_drho<double>(eos, e[0], e[1], P, dRdT, dRdS);
-    r[(size_t)k * cs] = ColRec{e[0], e[1], dRdT, dRdS};
+    E dRdT, dRdS;
+    interface_drho<E>(eos, (E)e[0], (E)e[1], (E)P, dRdT, dRdS);
+    r[(size_t)k * cs] = typename F::Col{e[0], e[1], dRdT, dRdS};
     pc[(size_t)k * cs] = P;
   };
-  stream_edges<double, 2>(nz, load, emit);
+  stream_edges<C, 2>(nz, load, emit);
 }
 
 template <class F>
@@ -263,28 +298,32 @@ __global__ void __launch_bounds__(128) k_column_state(ColumnArgs<F> a) {
 // ---------------------------------------------------------------------------
 
 // secant density difference rho(a) - rho(b) of two interface records
-// (ocean_redi.F90:344-345, 367-371, 406-410)
-__device__ __forceinline__ double sec_drho(const ColRec& a, const ColRec& b) {
-  return 0.5 * ((a.dT + b.dT) * (a.T - b.T) + (a.dS + b.dS) * (a.S - b.S));
+// (ocean_redi.F90:344-345, 367-371, 406-410), formed in FP64 from the
+// stored values in both forms
+template <class E>
+__device__ __forceinline__ double sec_drho(const ColRecT<E>& a, const ColRecT<E>& b) {
+  return 0.5 * (((double)a.dT + (double)b.dT) * (a.T - b.T) +
+                ((double)a.dS + (double)b.dS) * (a.S - b.S));
 }
 
 // Sliding window over one column of the sweep.  `k` is the reference's
 // kl / kr (1-based, 1..nk+1); the window holds the records at m = max(k-1,1)
 // and m+1 and the interface pressures at m, m+1, min(m+2, nk+1), which is
 // every index the sweep reads while the pointer sits at k.
+template <class Col>
 struct SweepSide {
-  const ColRec* rec;
+  const Col* rec;
   const double* P;
   size_t st;  // elements between consecutive interfaces of the column
   int k, m;
-  ColRec w0, w1;
+  Col w0, w1;
   double p0, p1, p2;
-  __device__ __forceinline__ void init(const ColRec* r, const double* p, size_t stride, int nk) {
+  __device__ __forceinline__ void init(const Col* r, const double* p, size_t stride, int nk) {
     rec = r; P = p; st = stride; k = 1; m = 1;
     w0 = r[0]; w1 = r[stride];
     p0 = p[0]; p1 = p[stride]; p2 = p[(size_t)(nk < 2 ? nk : 2) * stride];
   }
-  __device__ __forceinline__ ColRec cur() const { return k == 1 ? w0 : w1; }  // record at k
+  __device__ __forceinline__ Col cur() const { return k == 1 ? w0 : w1; }  // record at k
   __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }        // P(k)
   __device__ __forceinline__ double pk1() const { return k == 1 ? p1 : p2; }       // P(k+1)
   __device__ __forceinline__ void advance(int nk) {  // k -> k+1, k <= nk
@@ -305,7 +344,7 @@ struct SweepArgs {
   int i0, j0;              // first computed face
   int di, dj;              // left column = (i - di, j - dj)
   const double* wet;
-  const ColRec* rec;
+  const typename F::Col* rec;
   const double* P;
   FaceRecF<F>* out;        // [s][face]
   uint8_t* mono;           // [face]
@@ -384,7 +423,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   };
   const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
   const size_t cr = (size_t)i + (size_t)a.nx * j;
-  SweepSide L{}, Rt{};
+  SweepSide<typename F::Col> L{}, Rt{};
   if (wet) {
     const size_t bl = colstate_base(cl, a.nx, nk), br = colstate_base(cr, a.nx, nk);
     L.init(a.rec + bl, a.P + bl, (size_t)a.nx, nk);
@@ -411,7 +450,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
     double PoL, PoR, posL, posR;
     int KoL, KoR;
     if (search_l) {
-      const ColRec cr_ = Rt.cur();
+      const typename F::Col cr_ = Rt.cur();
       const double top = sec_drho(L.w0, cr_);
       const double bot = sec_drho(L.w1, cr_);
       double Po;
@@ -445,7 +484,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         reached_bottom = true; search_l = false;
       }
     } else {
-      const ColRec cl_ = L.cur();
+      const typename F::Col cl_ = L.cur();
       const double top = sec_drho(Rt.w0, cl_);
       const double bot = sec_drho(Rt.w1, cl_);
       double Po;
@@ -546,38 +585,41 @@ struct ReconArgs {
   int N2, nz;
   const typename F::R* h;
   TracerSetT<typename F::R> tr;
-  ReconRec* rec;
+  typename F::Recon* rec;
   ReconLayout y;
 };
 
-// The reconstruction is computed in FP64 in both forms, from the stored
-// fields.
+// The reconstruction is computed in FP64 in both forms from the stored
+// fields (the FP32-store form keeps its edge values as FP32 offsets from
+// the FP64 layer means).
 // Streaming form: one lane per column walks it top-down with a register
 // window (stream_edges).
 template <class F, int NT>
 __global__ void __launch_bounds__(128) k_tracer_recon(ReconArgs<F, NT> a) {
+  using C = double;
   const int N2 = a.N2, nz = a.nz;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
   const size_t n2 = (size_t)N2;
   const typename F::R* __restrict__ h = a.h;
-  auto load = [&](int kt, double& hh, double* s) {
+  auto load = [&](int kt, C& hh, C* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;
-    hh = (double)h[o];
-    const double he = hh > kHDivEps ? hh : kHDivEps;
+    hh = (C)h[o];
+    const C he = hh > C(kHDivEps) ? hh : C(kHDivEps);
 #pragma unroll
-    for (int t = 0; t < NT; ++t) s[t] = (double)a.tr.q[t][o] / he;
+    for (int t = 0; t < NT; ++t) s[t] = (C)a.tr.q[t][o] / he;
   };
-  double e_prev[NT];
-  ReconRec* rc = a.rec + (size_t)c * a.y.ca;
-  auto emit = [&](int k, const double* e, const double* s_above) {
+  C e_prev[NT];
+  typename F::Recon* rc = a.rec + (size_t)c * a.y.ca;
+  auto emit = [&](int k, const C* e, const C* s_above) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (k > 0) rc[(size_t)(k - 1) * a.y.cb + (size_t)t * a.y.ct] = ReconRec::make(s_above[t], e_prev[t], e[t]);
+      if (k > 0)
+        rc[(size_t)(k - 1) * a.y.cb + (size_t)t * a.y.ct] = F::Recon::make(s_above[t], e_prev[t], e[t]);
       e_prev[t] = e[t];
     }
   };
-  stream_edges<double, NT>(nz, load, emit);
+  stream_edges<C, NT>(nz, load, emit);
 }
 
 // Tiled form: a workgroup of kReconLanes lanes takes CT = kReconLanes / KT
@@ -674,10 +716,10 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
   const int kk = tid % KT, cc = tid / KT;
   const int kl = k0 + kk, c = c0 + cc;
   if (kl < nz && c < N2) {
-    ReconRec* rc = a.rec + (size_t)c * a.y.ca + (size_t)kl * a.y.cb;
+    typename F::Recon* rc = a.rec + (size_t)c * a.y.ca + (size_t)kl * a.y.cb;
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-      rc[(size_t)t * a.y.ct] = ReconRec::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
+      rc[(size_t)t * a.y.ct] = F::Recon::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
   }
 }
 
@@ -702,22 +744,31 @@ inline unsigned recon_tiled_blocks(int N2, int nz, int kt) {
 // Pieces of redi_sublayer_dT (:893-925).  The interface value at position
 // Po in layer K of a column, (1 - Po) * T(K-1) + Po * T(K), from layer K's
 // record (the tlt / tlb / trt / trb expressions, :905-912):
-__device__ __forceinline__ double edge_value(const LayerVals& r, double Po) {
-  return (1.0 - Po) * r.Tt + Po * r.Tb;
+template <class C>
+__device__ __forceinline__ C edge_value(const LayerVals<C>& r, C Po) {
+  return (C(1) - Po) * r.Tt + Po * r.Tb;
 }
 
-// The down-gradient guard (:913-924): the layer difference where the
-// top, bottom, mean and layer differences agree in sign, else 0.
+// The down-gradient guard (:913-924): the flux is zeroed unless the top
+// and bottom differences agree in sign and the mean and layer differences
+// agree in sign.
 template <class R>
-__device__ __forceinline__ R guarded_dT(R tlt, R tlb, R trt, R trb, R tlay, R trlay) {
+__device__ __forceinline__ bool guard_zero(R dT_top, R dT_bot, R dT_ave, R dT_layer) {
+  return signum1<R>(dT_top) * signum1<R>(dT_bot) <= R(0) ||
+         signum1<R>(dT_ave) * signum1<R>(dT_layer) <= R(0);
+}
+
+// redi_sublayer_dT's result from the interface values at the sublayer's
+// top (tlt, trt) and bottom (tlb, trb) and the layer means: the layer
+// difference, or 0 where the guard zeroes it (`zero`).
+template <class R>
+__device__ __forceinline__ R guarded_dT(R tlt, R tlb, R trt, R trb, R tlay, R trlay, bool& zero) {
   const R dT_top = trt - tlt;
   const R dT_bot = trb - tlb;
   const R dT_ave = R(0.5) * (dT_top + dT_bot);
   const R dT_layer = trlay - tlay;
-  if (signum1<R>(dT_top) * signum1<R>(dT_bot) <= R(0) ||
-      signum1<R>(dT_ave) * signum1<R>(dT_layer) <= R(0))
-    return R(0);
-  return dT_layer;
+  zero = guard_zero<R>(dT_top, dT_bot, dT_ave, dT_layer);
+  return zero ? R(0) : dT_layer;
 }
 
 constexpr int kMaxNT = 4;
@@ -791,14 +842,21 @@ struct FluxArgs {
   const FaceRecF<F>* rec;
   const uint8_t* flags;   // face flags (kHasFlux)
   ReconLayout ry;
-  const ReconRec* recon;  // tracer t's records from recon + t*ry.ct
+  const typename F::Recon* recon;  // tracer t's records from recon + t*ry.ct
   double* acc;            // dTr, (cell, K, t) at cell*nz*NT + (K-1)*NT + t
+  // optional diagnostic: for tracer t of the group, face fo and chunk c of
+  // 63 sublayers, the bits of the contributing sublayers the guard zeroed,
+  // at guard[t*guard_ts + fo*guard_fs + c] (null: not written)
+  uint64_t* guard;
+  size_t guard_ts, guard_fs;
+  int guard_nt;           // tracers of the group that have a block (a lone tracer's copy has none)
 };
 
-// LDS of one flux wavefront: the compacted contributions of one face.
-template <int NT, int NZC>
+// LDS of one flux wavefront: the compacted contributions of one face
+// (fluxes in the form's compute type).
+template <class X, int NT, int NZC>
 struct FluxLds {
-  double x[(2 * NZC + 1) * NT];      // fluxes, entry e's tracer t at e*NT + t
+  X x[(2 * NZC + 1) * NT];           // fluxes, entry e's tracer t at e*NT + t
   uint16_t tgt[2][2 * NZC + 1];      // entry e's target layer, left / right column
   uint16_t start[2][NZC + 2];        // first entry whose target >= K (K = 1..nz+1)
 };
@@ -807,13 +865,13 @@ struct FluxLds {
 // (right), compacted into L in s order; returns their count (wave-uniform).
 template <class F, int NT, int NZC>
 __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t fo, uint32_t cl,
-                                                  uint32_t cr, FluxLds<NT, NZC>& L) {
-  using R = typename F::R;
+                                                  uint32_t cr, FluxLds<typename F::C, NT, NZC>& L) {
+  using C = typename F::C;
   using Face = FaceRecF<F>;
   const int lane = (int)(threadIdx.x & 63);
-  const double coef = ((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo];
-  const ReconRec* recL = a.recon + cl * a.ry.ca;
-  const ReconRec* recR = a.recon + cr * a.ry.ca;
+  const C coef = (C)(((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo]);
+  const typename F::Recon* recL = a.recon + cl * a.ry.ca;
+  const typename F::Recon* recR = a.recon + cr * a.ry.ca;
   const uint32_t cb = a.ry.cb, ct = a.ry.ct;
   const size_t fb = a.fb;
   const int ns = 2 * a.nz + 2, nsub = ns - 1;
@@ -825,29 +883,63 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
     SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
     if (surf) Fs = r[(size_t)s * fb].get();
     const int KoL = Fs.KoL, KoR = Fs.KoR;
-    const double PoL = Fs.PoL, PoR = Fs.PoR;
+    const C PoL = (C)Fs.PoL, PoR = (C)Fs.PoR;
     // surface s+1, the bottom of sublayer s (lane 63's value is never used)
-    const double PoLb = __shfl_down(PoL, 1, 64), PoRb = __shfl_down(PoR, 1, 64);
+    const C PoLb = __shfl_down(PoL, 1, 64), PoRb = __shfl_down(PoR, 1, 64);
     const int KoLb = __shfl_down(KoL, 1, 64), KoRb = __shfl_down(KoR, 1, 64);
     const bool contrib = surf && lane < kChunk && s < nsub && Fs.hEff != 0.0;
-    const double hEff = Fs.hEff;
-    double x[NT];
+    const C hEff = (C)Fs.hEff;
+    C x[NT];
     // one tracer's records at a time: a wave holds two gathered records
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      LayerVals qL{0.0, 0.0, 0.0, 0.0, 0.0}, qR = qL;
-      if (surf) {
-        qL = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
-        qR = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
+      x[t] = C(0);
+      bool zero = false;
+      if constexpr (std::is_same<typename F::Recon, ReconRecOff>::value) {
+        // FP32-store form: FP64 means, FP32 offsets.  The sublayer's top
+        // difference is D + (eR - eL) with D the FP64 difference of the two
+        // layer means and eL, eR FP32 edge offsets; its bottom difference is
+        // the next surface's top one; its layer difference is D plus the
+        // difference of the FP32 parabola means about the layer means.
+        ReconRecOff ql{0.0, 0.f, 0.f}, qr = ql;
+        if (surf) {
+          ql = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
+          qr = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
+        }
+        const double D = qr.Tl - ql.Tl;
+        const float eL = (1.f - PoL) * ql.oT + PoL * ql.oB;
+        const float eR = (1.f - PoR) * qr.oT + PoR * qr.oB;
+        const double dtop = D + (double)(eR - eL);
+        const double dbot = __shfl_down(dtop, 1, 64);
+        if (contrib) {
+          float aLl = ql.oT, aRl = ql.oB, aLr = qr.oT, aRr = qr.oB;
+          (void)ppm_limit<float>(0.f, aLl, aRl);
+          (void)ppm_limit<float>(0.f, aLr, aRr);
+          const float pl = ppm_ave<float>(PoL, PoLb + (float)(KoLb - KoL), aLl, aRl, 0.f);
+          const float pr = ppm_ave<float>(PoR, PoRb + (float)(KoRb - KoR), aLr, aRr, 0.f);
+          const double dlay = D + (double)(pr - pl);
+          const double dave = 0.5 * (dtop + dbot);
+          zero = guard_zero<double>(dtop, dbot, dave, dlay);
+          x[t] = zero ? C(0) : ((C)dlay * hEff) * coef;
+        }
+      } else {
+        LayerVals<C> qL{C(0), C(0), C(0), C(0), C(0)}, qR = qL;
+        if (surf) {
+          qL = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
+          qR = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
+        }
+        const C eL = edge_value(qL, PoL), eR = edge_value(qR, PoR);
+        const C eLb = __shfl_down(eL, 1, 64), eRb = __shfl_down(eR, 1, 64);
+        if (contrib) {
+          const C tlay = ppm_ave<C>(PoL, PoLb + (C)(KoLb - KoL), qL.aL, qL.aR, qL.Tl);
+          const C trlay = ppm_ave<C>(PoR, PoRb + (C)(KoRb - KoR), qR.aL, qR.aR, qR.Tl);
+          const C d = guarded_dT<C>(eL, eLb, eR, eRb, tlay, trlay, zero);
+          x[t] = (d * hEff) * coef;
+        }
       }
-      const double eL = edge_value(qL, PoL), eR = edge_value(qR, PoR);
-      const double eLb = __shfl_down(eL, 1, 64), eRb = __shfl_down(eR, 1, 64);
-      x[t] = 0.0;
-      if (contrib) {
-        const double tlay = ppm_ave<double>(PoL, PoLb + (double)(KoLb - KoL), qL.aL, qL.aR, qL.Tl);
-        const double trlay = ppm_ave<double>(PoR, PoRb + (double)(KoRb - KoR), qR.aL, qR.aR, qR.Tl);
-        const double d = guarded_dT<double>(eL, eLb, eR, eRb, tlay, trlay);
-        x[t] = (double)(R)((d * hEff) * coef);  // the form's stored flux
+      if (a.guard && t < a.guard_nt) {  // wave-uniform
+        const uint64_t gm = __ballot(contrib && zero);
+        if (lane == 0) a.guard[(size_t)t * a.guard_ts + fo * a.guard_fs + (size_t)(c0 / kChunk)] = gm;
       }
     }
     const uint64_t m = __ballot(contrib);
@@ -869,15 +961,21 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
 //
 // v + (sign) the contributions of one side (0 left, 1 right) of the face in
 // L to layer K, tracer t, in s order.
-template <int NT, int NZC>
-__device__ __forceinline__ double fold_side(double v, const FluxLds<NT, NZC>& L, int side, bool mono,
+template <class X, int NT, int NZC>
+__device__ __forceinline__ double fold_side(double v, const FluxLds<X, NT, NZC>& L, int side, bool mono,
                                             int nc, int K, int t, bool plus) {
   if (mono) {
     const int lo = L.start[side][K], hi = L.start[side][K + 1];
-    for (int e = lo; e < hi; ++e) v = plus ? v + L.x[e * NT + t] : v - L.x[e * NT + t];
+    for (int e = lo; e < hi; ++e) {
+      const double x = (double)L.x[e * NT + t];
+      v = plus ? v + x : v - x;
+    }
   } else {
     for (int e = 0; e < nc; ++e)
-      if ((int)L.tgt[side][e] == K) v = plus ? v + L.x[e * NT + t] : v - L.x[e * NT + t];
+      if ((int)L.tgt[side][e] == K) {
+        const double x = (double)L.x[e * NT + t];
+        v = plus ? v + x : v - x;
+      }
   }
   return v;
 }
@@ -885,7 +983,7 @@ __device__ __forceinline__ double fold_side(double v, const FluxLds<NT, NZC>& L,
 // Run `w` of the x (kY = false) or y (kY = true) face grid.
 template <class F, int NT, int NZC, bool kY>
 __device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
-                                          FluxLds<NT, NZC>& L) {
+                                          FluxLds<typename F::C, NT, NZC>& L) {
   constexpr int P = (NZC * NT + 63) / 64;  // passes over a cell's dTr block
   const int lane = (int)(threadIdx.x & 63);
   const int w = xcd_band(bid, nb);
@@ -962,12 +1060,12 @@ __device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
       const int K = idx / NT + 1, t = idx % NT;
       const bool ok = idx < nk;
       if (doL) {  // E (x) / N (y): the left cell's second / last face
-        if (ok && nc) prev[p] = fold_side<NT, NZC>(prev[p], L, 0, monoL, nc, K, t, true);
+        if (ok && nc) prev[p] = fold_side(prev[p], L, 0, monoL, nc, K, t, true);
         if (ok) acc[(size_t)cl * nk + idx] = prev[p];
       }
       if (doR) {  // W (x, added to 0) / S (y, added to the W + E partial)
         double v = kY ? cur[p] : 0.0;
-        if (ok && nc) v = fold_side<NT, NZC>(v, L, 1, monoR, nc, K, t, false);
+        if (ok && nc) v = fold_side(v, L, 1, monoR, nc, K, t, false);
         prev[p] = v;
       }
     }
@@ -977,7 +1075,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
 
 template <class F, int NT, int NZC, bool kY>
 __global__ void __launch_bounds__(64) k_flux(FluxArgs<F> a) {
-  __shared__ FluxLds<NT, NZC> lds;
+  __shared__ FluxLds<typename F::C, NT, NZC> lds;
   flux_body<F, NT, NZC, kY>(a, (int)blockIdx.x, (int)gridDim.x, lds);
 }
 

@@ -88,11 +88,12 @@ size_t padded_faces(size_t nf) { return (nf + kFacePad - 1) / kFacePad * kFacePa
 // Bytes of the per-form records and values (the column state is FP64 in
 // both forms).
 struct FormSizes {
-  size_t face, recon, val;  // FaceRec, ReconRec, stored value
+  size_t face, recon, val, col;  // FaceRec, ReconRec, stored value, ColRec
 };
 template <class F>
 constexpr FormSizes form_sizes() {
-  return FormSizes{sizeof(FaceRecF<F>), sizeof(ReconRec), sizeof(typename F::R)};
+  return FormSizes{sizeof(FaceRecF<F>), sizeof(typename F::Recon), sizeof(typename F::R),
+                   sizeof(typename F::Col)};
 }
 FormSizes sizes_of(int form) {
   return form == REDI_FORM_FP32_STORE ? form_sizes<Fp32>() : form_sizes<Fp64>();
@@ -131,7 +132,7 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const
 
 size_t workspace_bytes(int nx, int ny, int nz, int nt, int form) {
   const FormSizes z = sizes_of(form);
-  const size_t phase_a = (sizeof(ColRec) + sizeof(double)) * (nz + 1) * (size_t)nx * ny;
+  const size_t phase_a = (z.col + sizeof(double)) * (nz + 1) * (size_t)nx * ny;
   return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt), z).bytes);
 }
 
@@ -229,7 +230,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
                    const double* wet_v, const FaceState& fs, void* ws, const Tiles& tiles,
                    hipStream_t st, Timer& tm, const Taper& tp = Taper{}) {
   const int N2 = nx * ny;
-  ColRec* rec = (ColRec*)ws;
+  typename F::Col* rec = (typename F::Col*)ws;
   double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
   k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, nx, h, t, s, eos, rec, P});
@@ -269,7 +270,7 @@ struct PhaseB {
 
 template <class F, int NT>
 void launch_recon(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
-                  const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
+                  const TracerSetT<typename F::R>& ts, typename F::Recon* recon, const ReconLayout& ry,
                   hipStream_t st) {
   const int N2 = p.nx * p.ny, nz = p.nz;
   const ReconArgs<F, NT> ra{N2, nz, h, ts, recon, ry};
@@ -307,7 +308,7 @@ int flux_class(int nz) {
 // grids with their divergence, the tracer update.
 template <class F, int NT>
 void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
-                  const TracerSetT<typename F::R>& ts, ReconRec* recon, const ReconLayout& ry,
+                  const TracerSetT<typename F::R>& ts, typename F::Recon* recon, const ReconLayout& ry,
                   const FluxArgs<F>& fx, const FluxArgs<F>& fy, unsigned nbx, unsigned nby,
                   const UpdateArgs<F>& ua, hipStream_t st, Timer& tm) {
   int slot = tm.begin(REDI_K_TRACER_RECON, st);
@@ -331,7 +332,7 @@ template <class F>
 void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc, int seg,
                   FluxArgs<F>& fx, FluxArgs<F>& fy, UpdateArgs<F>& ua, unsigned* nbx, unsigned* nby) {
   const int nx = p.nx, ny = p.ny, nz = p.nz, ns = 2 * nz + 2;
-  ReconRec* recon = (ReconRec*)sc.recon;
+  typename F::Recon* recon = (typename F::Recon*)sc.recon;
   fx = FluxArgs<F>{};  // u faces: runs along rows
   fx.nx = nx; fx.ny = ny; fx.nz = nz;
   fx.wall0 = p.ww ? p.nghost : -1;
@@ -359,10 +360,14 @@ ReconLayout recon_layout(int layer_major, int N2, int nz, int k) {
                      : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
 }
 
+// Sublayer chunks per face of the guard-decision output.
+size_t guard_chunks(int nz) { return (size_t)(2 * nz + 1 + kChunk - 1) / kChunk; }
+
 template <class F>
 int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::R* const* tracers,
                    const FaceState& fs, void* ws, const Tiles& tiles, hipStream_t st, Timer& tm,
-                   double* const* tend = nullptr) {
+                   double* const* tend = nullptr, uint64_t* guard_u = nullptr,
+                   uint64_t* guard_v = nullptr) {
   using R = typename F::R;
   const int nx = p.nx, ny = p.ny, nz = p.nz, N2 = nx * ny;
   const int nu = (nx + 1) * ny, nv = nx * (ny + 1);
@@ -372,7 +377,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
   tm.end(slot, st);
   const int slots = recon_slots(nt);
   const PhaseBScratch sc = carve_phase_b(ws, nx, ny, nz, slots, form_sizes<F>());
-  ReconRec* recon = (ReconRec*)sc.recon;
+  typename F::Recon* recon = (typename F::Recon*)sc.recon;
   FluxArgs<F> fx, fy;
   UpdateArgs<F> ua;
   unsigned nbx = 0, nby = 0;
@@ -397,6 +402,15 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
     ua.nt = kn;
     const ReconLayout ry = recon_layout(tiles.recon_layer_major, N2, nz, kn);
     fx.ry = fy.ry = ry;
+    if (guard_u && guard_v) {
+      const size_t nch = guard_chunks(nz);
+      fx.guard = guard_u + (size_t)g0 * nu * nch;
+      fy.guard = guard_v + (size_t)g0 * nv * nch;
+      fx.guard_ts = (size_t)nu * nch;
+      fy.guard_ts = (size_t)nv * nch;
+      fx.guard_fs = fy.guard_fs = nch;
+      fx.guard_nt = fy.guard_nt = gn;
+    }
     switch (kn) {
       case 2: launch_group<F, 2>(p, tiles, h, ts, recon, ry, fx, fy, nbx, nby, ua, st, tm); break;
       case 3: launch_group<F, 3>(p, tiles, h, ts, recon, ry, fx, fy, nbx, nby, ua, st, tm); break;
@@ -570,6 +584,8 @@ struct redi_s {
   Tiles tiles;
   Timer timer;
   std::vector<double*> tend;  // optional tendency outputs, one per tracer
+  uint64_t* guard_u = nullptr;  // optional guard-decision outputs (redi_set_guard_output)
+  uint64_t* guard_v = nullptr;
   double slope_max = 0.0;     // opt-in slope taper (0 = off)
   bool is_init = false;
 };
@@ -643,7 +659,8 @@ int apply_flux_impl(const char* who, const redi_hgrid_t* grid, const redi_metric
   if (!r->tend.empty() && (int)r->tend.size() != ms.nt)
     return fail(REDI_ERR_ARG, "%s: %zu tendency outputs for %d tracers", who, r->tend.size(), ms.nt);
   return launch_phase_b<F>(p, ms.h, ms.nt, ms.tracers, r->fs, r->ws, r->tiles,
-                           (hipStream_t)stream, r->timer, r->tend.empty() ? nullptr : r->tend.data());
+                           (hipStream_t)stream, r->timer, r->tend.empty() ? nullptr : r->tend.data(),
+                           r->guard_u, r->guard_v);
 }
 
 StateView<double> view64(const redi_state_t& s) {
@@ -768,6 +785,14 @@ int redi_set_tendency_output(redi_t* r, int ntracers, double* const* tend) {
   if (ntracers < 0 || (ntracers > 0 && !tend))
     return fail(REDI_ERR_ARG, "redi_set_tendency_output: bad arguments");
   r->tend.assign(tend, tend + ntracers);
+  return REDI_OK;
+}
+
+int redi_set_guard_output(redi_t* r, uint64_t* u_bits, uint64_t* v_bits) {
+  if (!r) return fail(REDI_ERR_ARG, "redi_set_guard_output: null handle");
+  if (!u_bits != !v_bits) return fail(REDI_ERR_ARG, "redi_set_guard_output: give both arrays or neither");
+  r->guard_u = u_bits;
+  r->guard_v = v_bits;
   return REDI_OK;
 }
 

@@ -79,6 +79,7 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_kernel_name", "redi_init_form", "redi_calc_coeffs_fp32",
             "redi_apply_flux_fp32", "redi_step_fp32", "redi_state_bytes_form",
             "redi_set_tendency_output", "redi_set_recon_chunk", "redi_set_slope_taper",
+            "redi_set_guard_output",
             "redi_set_flux_segment", "redi_set_tile_order",
             # the batched form (bound in batch.py)
             "redi_batch_create", "redi_batch_destroy", "redi_batch_domain_bytes",
@@ -117,6 +118,7 @@ def lib() -> ctypes.CDLL:
         L.redi_set_flux_segment.argtypes = [_P, _I]
         L.redi_set_tile_order.argtypes = [_P, _I]
         L.redi_set_tendency_output.argtypes = [_P, _I, ctypes.POINTER(_P)]
+        L.redi_set_guard_output.argtypes = [_P, _P, _P]
         L.redi_set_tracer_group.argtypes = [_P, _I]
         L.redi_nonmonotone_faces.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong),
                                              ctypes.POINTER(ctypes.c_longlong)]
@@ -360,6 +362,18 @@ class OceanRedi:
         _check(lib().redi_set_tendency_output(self.handle, nt,
                                               ctypes.cast(self._tend_ptrs, ctypes.POINTER(_P))),
                "redi_set_tendency_output")
+
+    def guard_chunks(self) -> int:
+        """Sublayer chunks (of 63) per face of the guard-decision output."""
+        return (2 * self.nz + 1 + 62) // 63
+
+    def set_guard_output(self, u_bits, v_bits) -> None:
+        """Record the down-gradient guard's zeroed sublayers of every later
+        apply_flux in ``u_bits`` / ``v_bits`` (int64 tensors (nt, nu, nch) /
+        (nt, nv, nch) on the device, see redi_set_guard_output), or stop
+        with ``None, None``."""
+        _check(lib().redi_set_guard_output(self.handle, _ptr(u_bits), _ptr(v_bits)),
+               "redi_set_guard_output")
 
     def set_recon_order(self, layer_major: bool) -> None:
         _check(lib().redi_set_recon_order(self.handle, int(bool(layer_major))),
