@@ -679,6 +679,14 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
     for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)a.tr.q[t][o] / he;
   }
   __syncthreads();
+  // The FP32-store form (ReconRecOff records) runs stages 2-4 in FP32 on
+  // differences of the FP64 layer means: PLM slopes and PPM edges use only
+  // differences of their layer values (plus, for an edge, one of them), so
+  // shifting every value by one layer's mean changes them by exactly that
+  // shift; each slope is formed about its own layer's mean, each edge about
+  // the mean of the layer above it, and the records keep the edges as
+  // offsets from their layer's mean.
+  constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
   // 2: slopes of layers kl = k0-1+r; 0 for the first and last layer (the
   // slope stream_edges starts from and the one it skips)
   for (int q = tid; q < NSL * CT; q += kReconLanes) {
@@ -686,10 +694,18 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
     const int kl = k0 - 1 + r;  // rows r, r+1, r+2 of sh / sv hold layers kl-1, kl, kl+1
     const bool in = kl >= 1 && kl <= nz - 2;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-      sd[t][r][cc] = in ? plm_slope<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
-                                       sv[t][r + 1][cc], sv[t][r + 2][cc])
-                        : R(0);
+    for (int t = 0; t < NT; ++t) {
+      if constexpr (kOff) {
+        sd[t][r][cc] = in ? (R)plm_slope<float>((float)sh[r][cc], (float)sh[r + 1][cc], (float)sh[r + 2][cc],
+                                                (float)(sv[t][r][cc] - sv[t][r + 1][cc]), 0.f,
+                                                (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]))
+                          : R(0);
+      } else {
+        sd[t][r][cc] = in ? plm_slope<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
+                                         sv[t][r + 1][cc], sv[t][r + 2][cc])
+                          : R(0);
+      }
+    }
   }
   __syncthreads();
   // 3: edges at interfaces ki = k0+r
@@ -700,7 +716,15 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       R e = R(0);
-      if (ki == 0) {
+      if constexpr (kOff) {
+        // offset from the mean of layer ki-1 (ki = 0 and ki = nz: 0, the edge
+        // being the adjacent layer's mean there)
+        if (ki > 0 && ki < nz)
+          e = (R)ppm_edge<float>((float)sh[r][cc], (float)sh[r + 1][cc], (float)sh[r + 2][cc],
+                                 (float)sh[r + 3][cc], 0.f,
+                                 (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]), (float)sd[t][r][cc],
+                                 (float)sd[t][r + 1][cc]);
+      } else if (ki == 0) {
         e = sv[t][r + 2][cc] - R(0.5) * R(0);  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
       } else if (ki == nz) {
         e = sv[t][r + 1][cc] + R(0.5) * R(0);  // diff(nk) = 0
@@ -718,8 +742,16 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
   if (kl < nz && c < N2) {
     typename F::Recon* rc = a.rec + (size_t)c * a.y.ca + (size_t)kl * a.y.cb;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-      rc[(size_t)t * a.y.ct] = F::Recon::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
+    for (int t = 0; t < NT; ++t) {
+      if constexpr (kOff) {
+        // top edge: offset from layer kl-1's mean, moved to layer kl's; bottom
+        // edge: already about layer kl's mean
+        const float oT = kl == 0 ? 0.f : (float)((sv[t][kk + 1][cc] - sv[t][kk + 2][cc]) + se[t][kk][cc]);
+        rc[(size_t)t * a.y.ct] = ReconRecOff{sv[t][kk + 2][cc], oT, (float)se[t][kk + 1][cc]};
+      } else {
+        rc[(size_t)t * a.y.ct] = F::Recon::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
+      }
+    }
   }
 }
 
