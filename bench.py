@@ -48,6 +48,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -277,39 +279,36 @@ def time_batched(cases, form: str, steps: int, warmup: int, persist: int = 1,
             "shapes": hist, "arena_GiB": round(arena / 2**30, 2)}
 
 
-def run_served(rate: float, duration: float, npool: int, budget_gb: float, seed: int = 0) -> dict:
-    """Served workload: Poisson step requests against the continuous-batching
-    server (FP32-store), and the same requests as serial plain-FP64 solves."""
+def _copy_requests(reqs):
+    from more_benchmarks_dc_ocean_amd import serve
+    return [serve.Request(rid=r.rid, arrival=r.arrival, nsteps=r.nsteps, domain=r.domain, seed=r.seed)
+            for r in reqs]
+
+
+def run_served(rate: float, duration: float, npool: int, budget_gb: float, seed: int = 0,
+               serial_seconds: float = None) -> dict:
+    """Served workload: Poisson step requests, each carrying its domain's
+    state (uploaded inside its latency), against the continuous-batching
+    server (FP32-store); and the same requests -- those arriving in the first
+    ``serial_seconds`` (default: all) -- served one at a time by the plain
+    FP64 single-domain entry, in real time.  With the serial baseline on a
+    prefix, the served figures over that same prefix are reported beside it
+    (``prefix``)."""
     import torch
     from more_benchmarks_dc_ocean_amd import serve
     from more_benchmarks_dc_ocean_amd.autotune import TUNED, TuneCache
-    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain, make_redi
 
     dev = "cuda:0"
     t0 = time.time()
     cases = serve.pool_cases(npool, seed)
-    log(f"[served] pool of {npool} domains generated in {time.time() - t0:.1f}s")
-    # serial plain-FP64 step time of every pool domain (the baseline's service times)
-    step_s = []
-    for c in cases:
-        r, d = make_redi(c, dev, "fp64")
-        r.step(d, 1)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r.step(d, 3)
-        torch.cuda.synchronize()
-        step_s.append((time.perf_counter() - t1) / 3)
-        r.close()
-        del d
-    torch.cuda.empty_cache()
+    pool32 = [serve.PinnedDomain(c, "fp32_store") for c in cases]
+    log(f"[served] pool of {npool} domains generated and pinned in {time.time() - t0:.1f}s")
     cache = TuneCache(TUNED)
-    pool = [DeviceDomain(c, dev, "fp32_store") for c in cases]
     shapes = [cache.batch_shape(c.nx, c.ny, c.nz, "fp32_store") for c in cases]
-    srv = serve.RediServer(pool, int(budget_gb * 1e9), "fp32_store", max_slots=256, shapes=shapes,
+    srv = serve.RediServer(pool32, int(budget_gb * 1e9), "fp32_store", max_slots=256, shapes=shapes,
                            device=dev)
     # warm the graph and the kernels on one request of each domain
-    warm = [serve.Request(rid=i, arrival=0.0, nsteps=1, domain=i) for i in range(npool)]
-    srv.run(warm)
+    srv.run([serve.Request(rid=i, arrival=0.0, nsteps=1, domain=i) for i in range(npool)])
     reqs = serve.poisson_requests(rate, duration, npool, seed)
     log(f"[served] {len(reqs)} requests over {duration:.0f}s at {rate:.0f}/s")
     st = srv.run(reqs, log=log)
@@ -317,15 +316,30 @@ def run_served(rate: float, duration: float, npool: int, budget_gb: float, seed:
                "budget_GB": budget_gb, "iterations": srv.iterations, "max_active": srv.max_active,
                "form": "fp32_store"})
     srv.close()
-    del pool
+    del pool32
     torch.cuda.empty_cache()
-    base = serve.fifo_serial(reqs, step_s, cases)
-    base["form"] = "fp64"
     log(f"[served] batched p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
-        f"{st['value']:.1f} Mcell-steps/s; serial FP64 p50 {base['p50_ms']:.1f} ms "
-        f"p99 {base['p99_ms']:.1f} ms {base['value']:.1f} Mcell-steps/s")
-    return {**{k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
-            "serial_fp64": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in base.items()}}
+        f"{st['value']:.1f} Mcell-steps/s")
+    rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    out = rnd(st)
+    pool64 = [serve.PinnedDomain(c, "fp64") for c in cases]
+    serve.run_serial([serve.Request(rid=i, arrival=0.0, nsteps=1, domain=i)
+                      for i in range(npool)], pool64, dev)  # warm-up
+    window = duration if serial_seconds is None else min(serial_seconds, duration)
+    sub = [r for r in reqs if r.arrival < window]
+    base = serve.run_serial(_copy_requests(sub), pool64, dev, log=log)
+    base.update({"form": "fp64", "arrival_window_s": window})
+    del pool64
+    torch.cuda.empty_cache()
+    log(f"[served] serial FP64 ({len(sub)} requests arriving in {window:.0f}s) p50 "
+        f"{base['p50_ms']:.1f} ms p99 {base['p99_ms']:.1f} ms {base['value']:.1f} Mcell-steps/s")
+    out["serial_fp64"] = rnd(base)
+    if len(sub) < len(reqs):
+        lat = np.array([r.latency for r in sub])
+        out["prefix"] = {"arrival_window_s": window, "requests": len(sub),
+                         "p50_ms": round(float(np.percentile(lat, 50) * 1e3), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99) * 1e3), 3)}
+    return out
 
 
 def main() -> int:
@@ -339,8 +353,13 @@ def main() -> int:
     ap.add_argument("--batched", type=int, default=96,
                     help="domains of the batched leg (0 = skip it)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-seconds", type=float, default=5.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="arrival window of the served leg (0 = skip it)")
+    ap.add_argument("--serve-serial-seconds", type=float, default=20.0,
+                    help="arrival window of the requests the serial FP64 baseline serves (0 = all)")
+    ap.add_argument("--serve-sustained-rate", type=float, default=50.0,
+                    help="request rate of the second, sustainable served point (0 = skip it)")
+    ap.add_argument("--serve-sustained-seconds", type=float, default=20.0)
     ap.add_argument("--serve-pool", type=int, default=24)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--reference-steps", type=int, default=20,
@@ -401,7 +420,11 @@ def main() -> int:
         del bcases
     if args.serve_seconds > 0:
         line["served"] = run_served(args.serve_rate, args.serve_seconds, args.serve_pool,
-                                    args.serve_budget_gb)
+                                    args.serve_budget_gb,
+                                    serial_seconds=args.serve_serial_seconds or None)
+    if args.serve_sustained_rate > 0 and args.serve_sustained_seconds > 0:
+        line["served_sustained"] = run_served(args.serve_sustained_rate, args.serve_sustained_seconds,
+                                              args.serve_pool, args.serve_budget_gb, seed=1)
     print(json.dumps(line), flush=True)
     return 0
 

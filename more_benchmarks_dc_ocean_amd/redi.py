@@ -187,22 +187,38 @@ class DeviceDomain:
     FP32-store form h and the tracers are held in FP32 (rounded from the
     case's FP64 fields); the metrics stay FP64."""
 
+    METRIC_NAMES = ("dy_cu", "dx_cv", "idxCu", "idyCv", "areaT", "wet_u", "wet_v", "khtr_u_ext",
+                    "khtr_v_ext")
+
     def __init__(self, case: RediCase, device: str = "cuda", form: str = "fp64"):
         import torch
         if form not in FORMS:
             raise ValueError(f"unknown form {form!r}")
-        self.case = case
-        self.form = form
-        self.device = torch.device(device)
-        dev = self.device
+        dev = torch.device(device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
         fdt = torch.float32 if form == "fp32_store" else torch.float64
-        self.h = t(case.h).to(fdt)
-        self.hTr = t(case.hTr).to(fdt)  # (nt, nz, ny, nx)
-        self.dy_cu, self.dx_cv = t(case.dy_cu), t(case.dx_cv)
-        self.idxCu, self.idyCv = t(case.idxCu), t(case.idyCv)
-        self.areaT, self.wet_u, self.wet_v = t(case.areaT), t(case.wet_u), t(case.wet_v)
-        self.khtr_u_ext, self.khtr_v_ext = t(case.khtr_u_ext), t(case.khtr_v_ext)
+        self._bind(case, form, dev, t(case.h).to(fdt), t(case.hTr).to(fdt),
+                   [t(getattr(case, n)) for n in self.METRIC_NAMES])
+
+    @classmethod
+    def from_tensors(cls, case: RediCase, form: str, device, h, hTr, metrics) -> "DeviceDomain":
+        """A domain over device tensors the caller filled: h (nz, ny, nx) and
+        hTr (nt, nz, ny, nx) in the form's storage type, and the FP64 2-D
+        arrays of METRIC_NAMES, in that order.  ``case`` supplies the
+        scalars and shapes."""
+        import torch
+        d = cls.__new__(cls)
+        d._bind(case, form, torch.device(device), h, hTr, list(metrics))
+        return d
+
+    def _bind(self, case, form, dev, h, hTr, metrics) -> None:
+        self.case = case
+        self.form = form
+        self.device = dev
+        self.h = h
+        self.hTr = hTr  # (nt, nz, ny, nx)
+        (self.dy_cu, self.dx_cv, self.idxCu, self.idyCv, self.areaT, self.wet_u, self.wet_v,
+         self.khtr_u_ext, self.khtr_v_ext) = metrics
         self.grid = HGrid(case.nx, case.ny, case.nx_phys, case.ny_phys, case.nghost, 0, 0.0, 0.0)
         self.metrics = Metrics(*(_ptr(x).value for x in (self.dy_cu, self.dx_cv, self.idxCu,
                                                            self.idyCv, self.areaT, self.wet_u,

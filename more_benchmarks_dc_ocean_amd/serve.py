@@ -1,6 +1,9 @@
 """Served Redi steps: continuous batching of step requests on one GPU.
 
-A request asks for ``nsteps`` Redi steps of one regional domain.  The
+A request asks for ``nsteps`` Redi steps of one regional domain and carries
+that domain's state: its seeded bathymetry (h) and T/S, its metrics and face
+diffusivities -- held in pinned host memory, as a client would hand them
+over -- and a seed from which its two passive tracers are drawn.  The
 server keeps every admitted request's state resident in HBM and advances
 all of them together, one batched step (``RediBatch.step``: one hipGraph
 launch) per iteration:
@@ -9,25 +12,29 @@ launch) per iteration:
   admit    before every iteration, waiting requests are admitted in arrival
            order while a batch slot is free and the request fits the HBM
            budget: its Phase-A state and step scratch in the batch's arena
-           (carved, nothing allocated) and its tracers in the tracer budget;
+           (carved, nothing allocated) and its fields in the field budget;
            a request that does not fit yet waits while later, smaller ones
            may pass it, until the budget frees;
+  upload   an admitted request's fields are copied host -> HBM on a copy
+           stream (overlapping the batched step already running) and its
+           passive tracers drawn on the device from its seed; the compute
+           stream waits for that copy before the request's first step;
   step     one batched step of every admitted request: requests that arrived
            while others were on the GPU join at the next iteration, so their
            work overlaps on the device with the requests already running;
   retire   a request leaves the batch after its last step is enqueued; its
            completion time is taken when that step's event has completed,
-           and its latency is completion - arrival.
+           and its latency is completion - arrival (so it includes the wait,
+           the upload and every step).
 
-Requests carry their inputs as a reference to one of a pool of seeded
-regional domains resident on the device (h, metrics and face diffusivities
-are read-only in a Redi step and are shared; each request gets its own copy
-of the tracers), so the host does no per-request input generation or PCIe
-transfer inside the served loop.
+The baseline (``run_serial``) serves the same requests one at a time, first
+come first served, each uploaded and stepped alone by the plain FP64
+single-domain entry, in real time against the same arrival times.
 
-Baseline: the same request sequence served one request at a time by the
-plain FP64 single-domain step (FIFO, no batching), with each request's
-service time = its steps x the measured FP64 step time of its domain.
+The requests' domains come from a pool of seeded regional domains generated
+before the run (``pool_cases``), so that the host generates no fractal
+bathymetry inside the served loop; each request copies its pool entry's
+fields into HBM of its own and draws its own passive tracers.
 """
 from __future__ import annotations
 
@@ -41,7 +48,7 @@ import numpy as np
 
 from . import inputs
 from .batch import RediBatch, domain_bytes
-from .redi import DeviceDomain
+from .redi import DeviceDomain, OceanRedi
 
 
 @dataclasses.dataclass
@@ -50,12 +57,19 @@ class Request:
     arrival: float        # s after the start of the run
     nsteps: int
     domain: int           # index into the domain pool
+    seed: int = -1        # seed of its passive tracers (default: rid)
     slot: int = -1
     remaining: int = 0
     admitted: float = math.nan
     done: float = math.nan
     state: Optional[DeviceDomain] = None
+    ready: object = None  # event: its upload has landed
+    initial: Optional[np.ndarray] = None
     result: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        if self.seed < 0:
+            self.seed = self.rid
 
     @property
     def latency(self) -> float:
@@ -82,14 +96,70 @@ def poisson_requests(rate: float, duration: float, npool: int, seed: int = 0,
         if t >= duration:
             break
         out.append(Request(rid=len(out), arrival=t, nsteps=int(rng.integers(1, max_steps + 1)),
-                           domain=int(rng.integers(0, npool))))
+                           domain=int(rng.integers(0, npool)), seed=1_000_000 * (seed + 1) + len(out)))
     return out
 
 
-class RediServer:
-    """Continuous-batching Redi server over a pool of device-resident domains."""
+class PinnedDomain:
+    """One pool domain's fields in pinned host memory, in the storage type of
+    ``form``: h, hTr of T and S, and the FP64 2-D arrays
+    (DeviceDomain.METRIC_NAMES)."""
 
-    def __init__(self, pool: Sequence[DeviceDomain], budget_bytes: int, form: str = "fp32_store",
+    def __init__(self, case: inputs.RediCase, form: str):
+        import torch
+        if (case.idx_T, case.idx_S) != (1, 2) or case.nt < 2:
+            raise ValueError("served domains carry T and S as tracers 1 and 2")
+        self.case = case
+        self.form = form
+        fdt = torch.float32 if form == "fp32_store" else torch.float64
+        self.h = torch.from_numpy(case.h).to(fdt).pin_memory()
+        self.ts = torch.from_numpy(np.ascontiguousarray(case.hTr[:2])).to(fdt).pin_memory()
+        self.metrics = [torch.from_numpy(np.ascontiguousarray(getattr(case, n))).pin_memory()
+                        for n in DeviceDomain.METRIC_NAMES]
+
+    def device_bytes(self) -> int:
+        """HBM a request of this domain holds for its fields."""
+        c = self.case
+        return (self.h.numel() * self.h.element_size() * (1 + c.nt)
+                + sum(m.numel() * m.element_size() for m in self.metrics))
+
+
+def upload(pd: PinnedDomain, seed: int, device, stream=None) -> DeviceDomain:
+    """The request's own copy of the domain on the device: h, T, S and the
+    metrics copied from pinned host memory (asynchronously, on ``stream``),
+    and tracers 3.. drawn on the device as U[0,1) x h from a generator seeded
+    with ``seed`` (land, where h = 0, gets 0).  Tensors used on another
+    stream must be recorded on it by the caller."""
+    import torch
+    c = pd.case
+    dev = torch.device(device)
+    with torch.cuda.stream(stream) if stream is not None else _nullcontext():
+        h = pd.h.to(dev, non_blocking=True)
+        hTr = torch.empty((c.nt,) + tuple(pd.h.shape), dtype=pd.h.dtype, device=dev)
+        hTr[:2].copy_(pd.ts, non_blocking=True)
+        if c.nt > 2:
+            g = torch.Generator(device=dev)
+            g.manual_seed(int(seed))
+            u = torch.rand((c.nt - 2,) + tuple(pd.h.shape), generator=g, dtype=pd.h.dtype, device=dev)
+            torch.mul(u, h, out=hTr[2:])
+        metrics = [m.to(dev, non_blocking=True) for m in pd.metrics]
+    return DeviceDomain.from_tensors(c, pd.form, dev, h, hTr, metrics)
+
+
+class _nullcontext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
+
+
+class RediServer:
+    """Continuous-batching Redi server: requests carry their domain's state
+    (a PinnedDomain and a seed); admitted requests are uploaded and stepped
+    together."""
+
+    def __init__(self, pool: Sequence[PinnedDomain], budget_bytes: int, form: str = "fp32_store",
                  max_slots: int = 256, shapes: Optional[Sequence[Dict]] = None,
                  device: str = "cuda", keep_results: bool = False):
         import torch
@@ -98,22 +168,21 @@ class RediServer:
         self.device = torch.device(device)
         self.keep_results = keep_results
         self.shapes = list(shapes) if shapes else [None] * len(self.pool)
-        pool_bytes = sum(_field_bytes(d) for d in self.pool)
-        free = budget_bytes - pool_bytes
-        if free <= 0:
-            raise ValueError("the domain pool alone exceeds the HBM budget")
-        # arena (Phase-A state + scratch, ~280 B per cell in the FP32-store form)
-        # and per-request tracer copies (16 B per cell) share what is left
-        self.tracer_cap = int(free * 0.06)
-        self.arena_bytes = free - self.tracer_cap
-        self.need = [domain_bytes(d.case.nx, d.case.ny, d.case.nz, form, 4) for d in self.pool]
-        self.tracer_bytes = [d.hTr.numel() * d.hTr.element_size() for d in self.pool]
+        self.need = [domain_bytes(p.case.nx, p.case.ny, p.case.nz, form, 4) for p in self.pool]
+        self.field_bytes = [p.device_bytes() for p in self.pool]
+        # the budget is split between the arena (Phase-A state + scratch) and
+        # the requests' fields in the pool's mean proportion, with a margin
+        # for the fields
+        share = float(np.mean([f / (f + a) for f, a in zip(self.field_bytes, self.need)]))
+        self.field_cap = int(budget_bytes * min(0.5, 1.25 * share))
+        self.arena_bytes = int(budget_bytes - self.field_cap)
         self.batch = RediBatch(form=form, group=4, max_domains=max_slots,
                                arena_bytes=self.arena_bytes, persist=1, device=device)
         self.max_slots = max_slots
-        self.tracer_used = 0
+        self.field_used = 0
         self.iterations = 0
         self.max_active = 0
+        self.copy_stream = torch.cuda.Stream(device=self.device)
 
     def close(self) -> None:
         self.batch.close()
@@ -121,17 +190,24 @@ class RediServer:
     def _fits(self, r: Request, active: int) -> bool:
         if active >= self.max_slots:
             return False
-        if self.tracer_used + self.tracer_bytes[r.domain] > self.tracer_cap:
+        if self.field_used + self.field_bytes[r.domain] > self.field_cap:
             return False
         return self.batch.info()["largest_free_bytes"] >= self.need[r.domain]
 
-    def _admit(self, r: Request, now: float) -> None:
-        st = self.pool[r.domain].clone_tracers()
+    def _admit(self, r: Request, now: float, stream) -> None:
+        import torch
+        st = upload(self.pool[r.domain], r.seed, self.device, self.copy_stream)
+        ev = torch.cuda.Event()
+        ev.record(self.copy_stream)
+        for t in (st.h, st.hTr, st.dy_cu, st.dx_cv, st.idxCu, st.idyCv, st.areaT, st.wet_u,
+                  st.wet_v, st.khtr_u_ext, st.khtr_v_ext):
+            t.record_stream(stream)  # used by the compute stream until the request retires
+        r.ready = ev
         r.slot = self.batch.add(st, self.shapes[r.domain])
         r.state = st
         r.remaining = r.nsteps
         r.admitted = now
-        self.tracer_used += self.tracer_bytes[r.domain]
+        self.field_used += self.field_bytes[r.domain]
 
     def run(self, requests: Sequence[Request], log=None) -> Dict:
         """Serve ``requests`` (arrival times in s from now); returns stats."""
@@ -155,7 +231,7 @@ class RediServer:
                     if self.keep_results:
                         r.result = r.state.tracers_host()
                     r.state = None
-                    self.tracer_used -= self.tracer_bytes[r.domain]
+                    self.field_used -= self.field_bytes[r.domain]
                 block = False
 
         while pending or waiting or active or inflight:
@@ -165,11 +241,15 @@ class RediServer:
             admitted = []
             for r in waiting:
                 if self._fits(r, len(active)):
-                    self._admit(r, now)
+                    self._admit(r, now, stream)
                     active[r.slot] = r
                     admitted.append(r)
             if admitted:
                 waiting = [r for r in waiting if r.slot < 0]
+                for r in admitted:  # the step waits for the new requests' uploads
+                    stream.wait_event(r.ready)
+                    if self.keep_results:
+                        r.initial = r.state.tracers_host()
             if active:
                 self.batch.step(1)
                 ev = torch.cuda.Event()
@@ -199,14 +279,50 @@ class RediServer:
         return summarize(requests, self.pool, clock())
 
 
-def _field_bytes(d: DeviceDomain) -> int:
-    ts = [d.h, d.hTr, d.dy_cu, d.dx_cv, d.idxCu, d.idyCv, d.areaT, d.wet_u, d.wet_v,
-          d.khtr_u_ext, d.khtr_v_ext]
-    return sum(t.numel() * t.element_size() for t in ts)
+def run_serial(requests: Sequence[Request], pool: Sequence[PinnedDomain], device: str = "cuda",
+               log=None, keep_results: bool = False) -> Dict:
+    """The baseline: the same requests served one at a time in arrival order
+    (first come, first served) by the plain FP64 single-domain entry, in
+    real time: each is uploaded and stepped alone once it has arrived and
+    the previous one is done.  One ocean_redi_t per pool domain is created
+    beforehand (its state sized for the domain), as a serial solver would
+    keep one per domain size."""
+    import torch
+    if any(p.form != "fp64" for p in pool):
+        raise ValueError("the serial baseline runs the plain FP64 form")
+    solvers: Dict[int, OceanRedi] = {}
+    for i in sorted({r.domain for r in requests}):
+        o = OceanRedi(pool[i].case, device=device)
+        o.configure(enable=True, khtr=pool[i].case.khtr)
+        solvers[i] = o
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    clock = lambda: time.perf_counter() - t0
+    next_log = 5.0
+    for r in sorted(requests, key=lambda r: r.arrival):
+        wait = r.arrival - clock()
+        if wait > 0:
+            time.sleep(wait)
+        r.admitted = clock()
+        dom = upload(pool[r.domain], r.seed, device)
+        if keep_results:
+            r.initial = dom.tracers_host()
+        solvers[r.domain].step(dom, r.nsteps)
+        torch.cuda.synchronize()
+        r.done = clock()
+        if keep_results:
+            r.result = dom.tracers_host()
+        del dom
+        if log and clock() >= next_log:
+            log(f"serial t={clock():6.1f}s done={r.rid + 1}")
+            next_log += 5.0
+    for o in solvers.values():
+        o.close()
+    return summarize(requests, pool, clock())
 
 
 def cell_steps(r: Request, pool) -> int:
-    c = pool[r.domain].case if hasattr(pool[r.domain], "case") else pool[r.domain]
+    c = pool[r.domain].case
     return c.interior_cells * r.nsteps
 
 
@@ -218,20 +334,3 @@ def summarize(requests: Sequence[Request], pool, wall: float) -> Dict:
             "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
             "max_ms": float(lat.max() * 1e3), "cell_steps": int(work),
             "value": work / span / 1e6, "unit": "Mcell-steps/s", "makespan_s": span, "wall_s": wall}
-
-
-def fifo_serial(requests: Sequence[Request], step_s: Sequence[float], pool) -> Dict:
-    """Latencies of serving ``requests`` one at a time in arrival order, each
-    taking nsteps x step_s[domain] seconds (Lindley's recursion)."""
-    free_at, lat = 0.0, []
-    for r in sorted(requests, key=lambda r: r.arrival):
-        start = max(free_at, r.arrival)
-        free_at = start + r.nsteps * step_s[r.domain]
-        lat.append(free_at - r.arrival)
-    lat = np.array(lat)
-    work = sum(cell_steps(r, pool) for r in requests)
-    span = free_at - min(r.arrival for r in requests)
-    return {"requests": len(requests), "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3),
-            "cell_steps": int(work), "value": work / span / 1e6, "unit": "Mcell-steps/s",
-            "makespan_s": span}

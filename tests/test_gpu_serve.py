@@ -1,53 +1,108 @@
 """Served form: continuous batching of Redi step requests (MI355X).
 
-Bar: every request's result from the served loop -- admitted while others
-are on the GPU, stepped in a changing batch, under an HBM budget that makes
-later requests wait -- is bit-identical to the same domain stepped alone by
-the single-domain FP32-store entry for the same number of steps.
+Requests carry their domain's state (pinned host fields, uploaded inside the
+request's latency, and passive tracers drawn on the device from the
+request's own seed).  Bars:
+
+  * every request's result from the served loop -- admitted while others
+    are on the GPU, stepped in a changing batch, under an HBM budget that
+    makes later requests wait -- is bit-identical to the same state stepped
+    alone by the single-domain entry of the same form for the same number
+    of steps, with domains of the served workload's own sizes (sides 64 to
+    720, nz 25 / 50 / 75);
+  * a served run of the plain FP64 form, and the serial FP64 baseline, give
+    the oracle's bits for every request.
 """
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
-from more_benchmarks_dc_ocean_amd import inputs
+import redi_oracle
 
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_same_alone_and_in_the_mix(gpu_device):
+def _pick(cases):
+    """The smallest, a middle and the largest domain of the pool."""
+    order = sorted(range(len(cases)), key=lambda i: cases[i].nx_phys)
+    return [order[0], order[len(order) // 2], order[-1]]
+
+
+def _solo(pd, r, device, form):
     import torch
     from more_benchmarks_dc_ocean_amd import serve
-    from more_benchmarks_dc_ocean_amd.batch import domain_bytes
-    from more_benchmarks_dc_ocean_amd.redi import DeviceDomain, make_redi
+    from more_benchmarks_dc_ocean_amd.redi import OceanRedi
+    dom = serve.upload(pd, r.seed, device)
+    o = OceanRedi(pd.case, device=device, form=form)
+    o.configure(enable=True, khtr=pd.case.khtr)
+    o.step(dom, r.nsteps)
+    torch.cuda.synchronize()
+    out = dom.tracers_host()
+    o.close()
+    return out
 
-    print("[gpu] served loop: 12 requests over 3 pool domains", flush=True)
-    cases = [inputs.case_domain(4, 201, 30, 22, 25), inputs.case_domain(4, 202, 18, 40, 50),
-             inputs.case_domain(4, 203, 26, 26, 75)]
-    pool = [DeviceDomain(c, gpu_device, "fp32_store") for c in cases]
-    pool_bytes = sum(serve._field_bytes(d) for d in pool)
+
+def test_served_results_same_alone_and_in_the_mix(gpu_device):
+    from more_benchmarks_dc_ocean_amd import serve
+    from more_benchmarks_dc_ocean_amd.batch import domain_bytes
+
+    every = serve.pool_cases(24, 0)
+    cases = [every[i] for i in _pick(every)]
+    del every
+    sides = [c.nx_phys for c in cases]
+    print(f"[gpu] served loop: pool sides {sides}, nz {[c.nz for c in cases]}", flush=True)
+    assert min(sides) <= 80 and max(sides) >= 600
+    pool = [serve.PinnedDomain(c, "fp32_store") for c in cases]
     need = max(domain_bytes(c.nx, c.ny, c.nz, "fp32_store", 4) for c in cases)
-    # room for about three requests at a time (arena and tracer budget): of
-    # the 12 requests, arriving in two bursts, most wait for the budget
-    budget = pool_bytes + int(3.3 * need / 0.94)
-    srv = serve.RediServer(pool, budget, "fp32_store", max_slots=8, device=gpu_device,
-                           keep_results=True)
-    rng = np.random.default_rng(5)
-    reqs = [serve.Request(rid=i, arrival=0.0 if i < 6 else 0.01, nsteps=int(rng.integers(1, 5)),
-                          domain=int(rng.integers(0, 3))) for i in range(12)]
+    fields = max(p.device_bytes() for p in pool)
+    # room for about three of the largest requests at a time: of the
+    # requests, arriving as a Poisson stream, some wait for the budget
+    srv = serve.RediServer(pool, int(3.3 * (need + fields)), "fp32_store", max_slots=8,
+                           device=gpu_device, keep_results=True)
+    reqs = serve.poisson_requests(60.0, 0.25, len(pool), seed=5, max_steps=4)
+    assert len(reqs) >= 8
     stats = srv.run(reqs)
     srv.close()
-    assert stats["requests"] == 12
+    assert stats["requests"] == len(reqs)
     assert srv.max_active >= 2, "requests never shared a batched step"
-    assert any(r.admitted > r.arrival + 1e-4 for r in reqs), "the budget never made a request wait"
+    assert any(r.admitted > r.arrival + 1e-4 for r in reqs), "no request ever waited"
     assert all(r.done >= r.admitted >= r.arrival for r in reqs)
-    alone = {}
     for r in reqs:
-        key = (r.domain, r.nsteps)
-        if key not in alone:
-            ro, d = make_redi(cases[r.domain], gpu_device, "fp32_store")
-            ro.step(d, r.nsteps)
-            torch.cuda.synchronize()
-            alone[key] = d.tracers_host()
-            ro.close()
-        assert np.array_equal(r.result, alone[key]), f"request {r.rid} differs from its solo run"
+        alone = _solo(pool[r.domain], r, gpu_device, "fp32_store")
+        assert np.array_equal(r.result, alone), f"request {r.rid} differs from its solo run"
+        assert not np.array_equal(r.result, r.initial), f"request {r.rid}: nothing changed"
+    seeds = {r.seed for r in reqs}
+    assert len(seeds) == len(reqs)
+    same_dom = [r for r in reqs if r.domain == reqs[0].domain]
+    if len(same_dom) > 1:  # each request draws its own passive tracers
+        assert not np.array_equal(same_dom[0].initial[2], same_dom[1].initial[2])
+
+
+def test_served_fp64_and_serial_baseline_match_oracle(gpu_device):
+    """A served run in the plain FP64 form, and the serial FP64 baseline on
+    the same requests: every result is the oracle's step of the request's
+    own uploaded state, bit for bit (the smallest pool domains, so that the
+    oracle finishes quickly)."""
+    import dataclasses
+    from more_benchmarks_dc_ocean_amd import serve
+    every = serve.pool_cases(24, 0)
+    order = sorted(range(len(every)), key=lambda i: every[i].nx_phys)
+    cases = [every[i] for i in order[:3]]
+    del every
+    print(f"[gpu] served fp64 vs oracle: sides {[c.nx_phys for c in cases]}", flush=True)
+    pool = [serve.PinnedDomain(c, "fp64") for c in cases]
+    reqs = serve.poisson_requests(200.0, 0.05, len(pool), seed=9, max_steps=3)
+    assert len(reqs) >= 4
+    srv = serve.RediServer(pool, 8 << 30, "fp64", max_slots=16, device=gpu_device, keep_results=True)
+    srv.run(reqs)
+    srv.close()
+    ser = [serve.Request(rid=r.rid, arrival=r.arrival, nsteps=r.nsteps, domain=r.domain, seed=r.seed)
+           for r in reqs]
+    serve.run_serial(ser, pool, gpu_device, keep_results=True)
+    for r, q in zip(reqs, ser):
+        c = dataclasses.replace(pool[r.domain].case, hTr=r.initial.copy())
+        ref = redi_oracle.run_oracle(c, nsteps=r.nsteps)["hTrN"]
+        assert np.array_equal(r.result, ref), f"served request {r.rid} differs from the oracle"
+        assert np.array_equal(q.initial, r.initial), f"request {r.rid}: serial state differs"
+        assert np.array_equal(q.result, ref), f"serial request {r.rid} differs from the oracle"
