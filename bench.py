@@ -81,34 +81,72 @@ def measured_traffic(kernel: str, workload: str, form: str = "fp64"):
     return None
 
 
-def cpu_baseline(case, sample_steps: int = 2):
-    """Time the reference build (or the C restatement) on a cropped sample of
-    the workload: a 480x360 interior window of the same domain, all layers and
-    tracers, 1 warm-up + ``sample_steps`` timed steps, on every host core."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_reference(case, steps: int, threads: int) -> float:
+    """ms per step of the reference build on ``case`` (1 warm-up step)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import redi_oracle
+    _, out = redi_oracle.run_reference(case, nsteps=steps, threads=threads, nwarm=1,
+                                       want_output=False)
+    return float(out.split("ms_per_step=")[1].split()[0])
+
+
+def cpu_baseline(case, sample_steps: int = 2, extra: bool = True):
+    """The reference's own Fortran build (oracle/_ref/redi_ref) on this
+    host's cores.  The headline figure: a 480x360 interior window of the
+    workload domain, all layers and tracers, 1 warm-up + ``sample_steps``
+    timed steps on every core.  With ``extra``, also the CPU-runnable
+    64x64x25 configuration (10 steps) and the 473x297x75 front (1 step), each
+    on 1 thread and on every core.  Falls back to the oracle's C restatement
+    (kind "port") only where the reference binary is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import redi_oracle
+    from more_benchmarks_dc_ocean_amd import inputs
     from more_benchmarks_dc_ocean_amd.inputs import crop
     nxp = min(480, case.nx_phys)
     nyp = min(360, case.ny_phys)
     i0 = (case.nx_phys - nxp) // 2
     j0 = (case.ny_phys - nyp) // 2
     sub = crop(case, i0, j0, nxp, nyp)
-    threads = min(os.cpu_count() or 1, 64)
+    nproc = os.cpu_count() or 1
+    threads = min(nproc, 64)
     cells = sub.nx_phys * sub.ny_phys * sub.nz
     sample = (f"{nxp}x{nyp}x{sub.nz} interior window of the workload domain, {sub.nt} tracers, "
               f"{sample_steps} timed steps after 1 warm-up")
+    prov = {"cpu_model": _cpu_model(), "nproc": nproc}
     if redi_oracle.have_reference():
         try:
-            _, out = redi_oracle.run_reference(sub, nsteps=sample_steps, threads=threads, nwarm=1,
-                                               want_output=False)
-            ms = float(out.split("ms_per_step=")[1].split()[0])
-            return {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
-                    "kind": "reference", "sample": sample, "ms_per_step": ms}
+            ms = _time_reference(sub, sample_steps, threads)
+            res = {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
+                   "kind": "reference", "sample": sample, "ms_per_step": ms, **prov}
+            if extra:
+                runs = []
+                for name, c, steps in (("seamount_64x64x25 (linear EOS, 2 tracers)",
+                                        inputs.case_seamount(nsteps=10), 10),
+                                       ("front_473x297x75 (Roquet, 4 tracers)",
+                                        inputs.case_front(nsteps=1), 1)):
+                    for th in (1, threads):
+                        m = _time_reference(c, steps, th)
+                        runs.append({"workload": name, "threads": th, "steps": steps,
+                                     "ms_per_step": round(m, 3),
+                                     "Mcell_steps_per_s": round(c.interior_cells / (m * 1e-3) / 1e6, 3)})
+                        log(f"cpu reference {name} on {th} thread(s): {m:.1f} ms/step")
+                res["runs"] = runs
+            return res
         except Exception as e:  # the binary exists but cannot run here
             log(f"reference CPU build failed ({e}); timing the C restatement instead")
     redi_oracle.set_threads(threads)
     o = redi_oracle.OracleRedi(sub)
-    import numpy as np
     h = np.ascontiguousarray(sub.h)
     hTr = sub.hTr.copy()
     o.step(h, hTr)
@@ -117,7 +155,7 @@ def cpu_baseline(case, sample_steps: int = 2):
         o.step(h, hTr)
     ms = (time.perf_counter() - t0) * 1e3 / sample_steps
     return {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
-            "kind": "port", "sample": sample, "ms_per_step": ms}
+            "kind": "port", "sample": sample, "ms_per_step": ms, **prov}
 
 
 def time_form(case, form: str, steps: int, warmup: int, workload: str) -> dict:
