@@ -11,7 +11,10 @@ before the timed region starts.
 Both precision forms are timed on the same inputs, one after the other:
 the plain FP64 form gives ``value``; the FP32-store form (fields, face
 records and fluxes stored in FP32) is reported under ``forms.fp32_store``
-with its own roofline and its ratio to the FP64 form.  ``--form`` times one
+with its own roofline, its ratio to the FP64 form and ``accuracy``: its
+error against the FP64 form after ``--accuracy-steps`` steps and how many
+down-gradient guard decisions it switches, beside what rounding the inputs
+to FP32 alone does to the FP64 form.  ``--form`` times one
 form only.
 
 Printed JSON line: metric "Redi Mcell-steps/s" (interior cells x steps per
@@ -75,8 +78,9 @@ def measured_traffic(kernel: str, workload: str, form: str = "fp64"):
         return None
     with open(p) as f:
         d = json.load(f)
+    from more_benchmarks_dc_ocean_amd.profiling import _base
     for k, e in d.get("kernels", {}).items():
-        if k.split("<")[0] == kernel and "traffic_bytes" in e:
+        if _base(k) == kernel and "traffic_bytes" in e:
             return e["traffic_bytes"]
     return None
 
@@ -156,6 +160,34 @@ def cpu_baseline(case, sample_steps: int = 2, extra: bool = True):
     ms = (time.perf_counter() - t0) * 1e3 / sample_steps
     return {"value": cells / (ms * 1e-3) / 1e6, "unit": "Mcell-steps/s", "cores": threads,
             "kind": "port", "sample": sample, "ms_per_step": ms, **prov}
+
+
+def fp32_accuracy(case, steps: int) -> dict:
+    """Error of the FP32-store form after ``steps`` steps of the workload,
+    against the FP64 form, and the down-gradient guard decisions of step 1
+    (more_benchmarks_dc_ocean_amd.accuracy: A = FP64 form on the FP64 inputs,
+    B = FP64 form on the inputs rounded to FP32, C = FP32-store form on the
+    rounded inputs).  Relative errors are to the reference field's max."""
+    from more_benchmarks_dc_ocean_amd import accuracy
+    res = accuracy.report(case, steps, log=lambda m: log(f"[accuracy] {m}"))
+    F, out = res["fields"], {}
+    for n, g in res["guard"].items():
+        e = lambda key: F[key]["max_rel"]
+        out[n] = {"tend_max_rel_C_vs_A": e(f"tend_{n} (C vs A)"),
+                  "tend_rms_rel_C_vs_A": F[f"tend_{n} (C vs A)"]["rms_rel"],
+                  "tend_max_rel_B_vs_A": e(f"tend_{n} (B vs A)"),
+                  "tend_max_rel_C_vs_B": e(f"tend_{n} (C vs B)"),
+                  f"hTr_max_rel_after_{steps}_C_vs_A": e(f"hTr_{n} after {steps} (C vs A)"),
+                  "guard_zeroed_A": g["zeroed_A"], "guard_flips_B_vs_A": g["flips_B_vs_A"],
+                  "guard_flips_C_vs_A": g["flips_C_vs_A"], "guard_flips_C_vs_B": g["flips_C_vs_B"]}
+    for n in ("T", "S"):
+        out[n][f"layer_value_max_rel_after_{steps}_C_vs_A"] = \
+            F[f"{n} layer value after {steps} (C vs A)"]["max_rel"]
+    return {"steps": steps,
+            "runs": "A fp64 on fp64 inputs, B fp64 on inputs rounded to fp32, C fp32_store on the "
+                    "rounded inputs; relative to the reference field's max",
+            "tracers": {n: {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in d.items()}
+                        for n, d in out.items()}}
 
 
 def time_form(case, form: str, steps: int, warmup: int, workload: str) -> dict:
@@ -402,6 +434,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--reference-steps", type=int, default=20,
                     help="timed steps of the reference's own benchmark configuration (0 = skip it)")
+    ap.add_argument("--accuracy-steps", type=int, default=20,
+                    help="steps of the FP32-store accuracy runs beside the FP64 form (0 = skip them)")
     ap.add_argument("--gabight-steps", type=int, default=10,
                     help="timed steps of the real-bathymetry workload (0 = skip it)")
     args = ap.parse_args()
@@ -444,6 +478,8 @@ def main() -> int:
     if "fp64" in res and "fp32_store" in res:
         f32 = dict(res["fp32_store"])
         f32["ratio_vs_fp64"] = round(f32["value"] / res["fp64"]["value"], 4)
+        if args.accuracy_steps > 0:
+            f32["accuracy"] = fp32_accuracy(case, args.accuracy_steps)
         line["forms"] = {"fp32_store": f32}
     if args.reference_steps > 0:
         line["reference_benchmark"] = time_reference_config(args.reference_steps)

@@ -194,12 +194,12 @@ int redi_set_slope_taper(redi_t *self, double slope_max);
  * result. */
 int redi_set_tile_order(redi_t *self, int vertical);
 
-/* Cells per run of the Phase-B flux kernels (1..4096, default 32): one
- * wavefront walks a run of cells along a row (u faces) or up a column
- * (v faces), face by face, and adds each face's sublayer fluxes into its
- * two cells in the reference's order.  A run's two end faces are also
- * computed by the neighbouring runs.  Changes only the schedule, never
- * the result. */
+/* Cells per run of the Phase-B flux kernels (1..4096; 0, the default, picks
+ * 32, or 16 or 8 on domains of under 640k or 320k columns): one wavefront
+ * walks a run of cells along a row (u faces) or up a column (v faces), face
+ * by face, and adds each face's sublayer fluxes into its two cells in the
+ * reference's order.  A run's two end faces are also computed by the
+ * neighbouring runs.  Changes only the schedule, never the result. */
 int redi_set_flux_segment(redi_t *self, int cells);
 
 /* Layers per workgroup of the per-column tiled reconstruction (8, 16 or 32,

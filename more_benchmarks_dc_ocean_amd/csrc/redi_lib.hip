@@ -201,7 +201,7 @@ struct Tiles {
   int recon_layer_major = 0; // reconstruction layer-major (1) or per column (0)
   int group = kTracerGroup;  // tracers per flux launch (1..4)
   int recon_kt = kReconChunkDefault;  // layers per tiled-reconstruction workgroup
-  int seg = kFluxSegDefault;  // cells per run of the flux kernels
+  int seg = 0;  // cells per run of the flux kernels (0: by domain size, flux_segment)
   int vertical = kTileVerticalDefault;  // sweep tiles numbered down rows first
 };
 
@@ -325,6 +325,18 @@ void launch_group(const PhaseB& p, const Tiles& tiles, const typename F::R* h,
   tm.end(slot, st);
 }
 
+// Cells per run of the flux kernels for an nx x ny domain: runs of
+// kFluxSegDefault cells where that still gives every SIMD several runs to
+// interleave, shorter runs (more of them, each end face computed twice) on
+// small domains.
+int flux_segment(int nx, int ny, int seg) {
+  if (seg > 0) return seg;
+  const double cells = (double)nx * ny;
+  int s = kFluxSegDefault;
+  while (s > 8 && cells / s < 20000.0) s /= 2;
+  return s;
+}
+
 // Flux arguments of both face grids and the update arguments of one
 // domain's Phase B over the scratch `sc` (the reconstruction layout, the
 // tracer pointers and the group size are set per tracer group).
@@ -337,6 +349,7 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   fx.nx = nx; fx.ny = ny; fx.nz = nz;
   fx.wall0 = p.ww ? p.nghost : -1;
   fx.wall1 = p.we ? p.nghost + p.nxp : -1;
+  seg = flux_segment(nx, ny, seg);
   fx.seg = seg;
   *nbx = flux_blocks(nx, ny, seg, false, &fx.nseg);
   fx.fa = face_fa(ns); fx.fb = face_fb(ns);
@@ -819,8 +832,8 @@ int redi_set_tile_order(redi_t* r, int vertical) {
 
 int redi_set_flux_segment(redi_t* r, int cells) {
   if (!r) return fail(REDI_ERR_ARG, "redi_set_flux_segment: null handle");
-  if (cells < 1 || cells > 4096)
-    return fail(REDI_ERR_ARG, "redi_set_flux_segment: %d cells per run not in 1..4096", cells);
+  if (cells < 0 || cells > 4096)
+    return fail(REDI_ERR_ARG, "redi_set_flux_segment: %d cells per run not in 0..4096", cells);
   r->tiles.seg = cells;
   return REDI_OK;
 }
