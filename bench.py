@@ -111,7 +111,8 @@ def cpu_baseline(case, sample_steps: int = 2, extra: bool = True):
     workload domain, all layers and tracers, 1 warm-up + ``sample_steps``
     timed steps on every core.  With ``extra``, also the CPU-runnable
     64x64x25 configuration (10 steps) and the 473x297x75 front (1 step), each
-    on 1 thread and on every core.  Falls back to the oracle's C restatement
+    on 1 thread and on every core, and one step of the whole workload domain
+    (after 1 warm-up) on every core.  Falls back to the oracle's C restatement
     (kind "port") only where the reference binary is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import redi_oracle
@@ -145,6 +146,12 @@ def cpu_baseline(case, sample_steps: int = 2, extra: bool = True):
                                      "ms_per_step": round(m, 3),
                                      "Mcell_steps_per_s": round(c.interior_cells / (m * 1e-3) / 1e6, 3)})
                         log(f"cpu reference {name} on {th} thread(s): {m:.1f} ms/step")
+                m = _time_reference(case, 1, threads)  # the whole workload domain
+                runs.append({"workload": f"{case.nx_phys}x{case.ny_phys}x{case.nz} whole domain "
+                                         f"({case.nt} tracers)", "threads": threads, "steps": 1,
+                             "ms_per_step": round(m, 3),
+                             "Mcell_steps_per_s": round(case.interior_cells / (m * 1e-3) / 1e6, 3)})
+                log(f"cpu reference whole domain on {threads} threads: {m:.1f} ms/step")
                 res["runs"] = runs
             return res
         except Exception as e:  # the binary exists but cannot run here
