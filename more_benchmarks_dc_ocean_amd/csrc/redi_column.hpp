@@ -34,25 +34,46 @@ __device__ __forceinline__ R fmin_sel(R a, R b) { return a < b ? a : b; }
 template <class R>
 __device__ __forceinline__ R fmax_sel(R a, R b) { return a > b ? a : b; }
 
-// redi_fv_diff (redi/ocean_redi.F90:135-150)
+// The PLM slope and the PPM edge are linear in the tracer values with
+// weights that depend on the layer thicknesses alone.  Every tracer sharing a
+// thickness column shares those weights, so they are formed once per layer
+// (plm_weights, ppm_weights) and applied per tracer (plm_slope_w,
+// ppm_edge_w).  Each weight is the reference's own subexpression, evaluated
+// in the reference's order, so weights + application give the same bits as
+// the one-shot helpers below them.
+
+// thickness part of redi_fv_diff (redi/ocean_redi.F90:135-150) and of the
+// PLM guard (:174-187)
 template <class R>
-__device__ __forceinline__ R fv_diff(R hkm1, R hk, R hkp1, R skm1, R sk, R skp1) {
+struct PlmW {
+  R wa, wb, wc;  // (2 hkm1 + hk) / (hkp1 + hk), (2 hkp1 + hk) / (hkm1 + hk), hk / h_sum
+  bool flat;     // the slope is 0 whatever the tracer values
+};
+template <class R>
+__device__ __forceinline__ PlmW<R> plm_weights(R hkm1, R hk, R hkp1) {
   R h_sum = (hkm1 + hkp1) + hk;
   if (h_sum != R(0)) h_sum = R(1) / h_sum;
   R hm = hkm1 + hk;
   if (hm != R(0)) hm = R(1) / hm;
   R hp = hkp1 + hk;
   if (hp != R(0)) hp = R(1) / hp;
-  const R a = ((R(2) * hkm1 + hk) * hp) * (skp1 - sk);
-  const R b = ((R(2) * hkp1 + hk) * hm) * (sk - skm1);
-  return (hk * h_sum) * (a + b);
+  return PlmW<R>{(R(2) * hkm1 + hk) * hp, (R(2) * hkp1 + hk) * hm, hk * h_sum,
+                 !((hkp1 + hk) * (hkm1 + hk) > R(0))};
+}
+
+// redi_fv_diff (redi/ocean_redi.F90:135-150) from its weights
+template <class R>
+__device__ __forceinline__ R fv_diff_w(const PlmW<R>& w, R skm1, R sk, R skp1) {
+  const R a = w.wa * (skp1 - sk);
+  const R b = w.wb * (sk - skm1);
+  return w.wc * (a + b);
 }
 
 // one interior entry of redi_plm_diff (redi/ocean_redi.F90:174-187)
 template <class R>
-__device__ __forceinline__ R plm_slope(R hm1, R h0, R hp1, R sm1, R s0, R sp1) {
-  if (!((hp1 + h0) * (hm1 + h0) > R(0))) return R(0);
-  const R dc = fv_diff<R>(hm1, h0, hp1, sm1, s0, sp1);
+__device__ __forceinline__ R plm_slope_w(const PlmW<R>& w, R sm1, R s0, R sp1) {
+  if (w.flat) return R(0);
+  const R dc = fv_diff_w<R>(w, sm1, s0, sp1);
   const R dl = R(2) * (s0 - sm1);
   const R dr = R(2) * (sp1 - s0);
   if (signum1<R>(dl) * signum1<R>(dr) <= R(0)) return R(0);
@@ -60,24 +81,51 @@ __device__ __forceinline__ R plm_slope(R hm1, R h0, R hp1, R sm1, R s0, R sp1) {
   return f_sign<R>(m, dc);
 }
 
-// redi_ppm_edge (redi/ocean_redi.F90:194-225)
 template <class R>
-__device__ __forceinline__ R ppm_edge(R hkm1, R hk, R hkp1, R hkp2, R ak, R akp1, R pk, R pkp1) {
+__device__ __forceinline__ R fv_diff(R hkm1, R hk, R hkp1, R skm1, R sk, R skp1) {
+  return fv_diff_w<R>(plm_weights<R>(hkm1, hk, hkp1), skm1, sk, skp1);
+}
+template <class R>
+__device__ __forceinline__ R plm_slope(R hm1, R h0, R hp1, R sm1, R s0, R sp1) {
+  return plm_slope_w<R>(plm_weights<R>(hm1, h0, hp1), sm1, s0, sp1);
+}
+
+// thickness part of redi_ppm_edge (redi/ocean_redi.F90:194-225)
+template <class R>
+struct PpmW {
+  R we;          // hk / (hk + hkp1) or hkp1 / (hk + hkp1), by `lower`
+  R f1, f2, f3, f4;
+  bool mean;     // hk + hkp1 <= 0: the edge is the mean of the two values
+  bool lower;    // hk < hkp1: the edge is built from ak
+};
+template <class R>
+__device__ __forceinline__ PpmW<R> ppm_weights(R hkm1, R hk, R hkp1, R hkp2) {
+  PpmW<R> w;
   R r = hk + hkp1;
-  if (r <= R(0)) return R(0.5) * (ak + akp1);
+  w.mean = r <= R(0);
   r = R(1) / r;
-  R e;
-  if (hk < hkp1)
-    e = ak + (hk * r) * (akp1 - ak);
-  else
-    e = akp1 + (hkp1 * r) * (ak - akp1);
+  w.lower = hk < hkp1;
+  w.we = w.lower ? hk * r : hkp1 * r;
   const R r2a = R(1) / ((R(2) * hk + hkp1) + R(kHNeglect));
   const R r2b = R(1) / ((hk + R(2) * hkp1) + R(kHNeglect));
-  const R f1 = R(1) / ((hk + hkp1) + (hkm1 + hkp2));
-  const R f2 = ((R(2) * (hkp1 * hk)) * r) * (((hkm1 + hk) * r2a) - ((hkp2 + hkp1) * r2b));
-  const R f3 = (hk * (hkm1 + hk)) * r2a;
-  const R f4 = (hkp1 * (hkp1 + hkp2)) * r2b;
-  return e + f1 * (f2 * (akp1 - ak) - (f3 * pkp1 - f4 * pk));
+  w.f1 = R(1) / ((hk + hkp1) + (hkm1 + hkp2));
+  w.f2 = ((R(2) * (hkp1 * hk)) * r) * (((hkm1 + hk) * r2a) - ((hkp2 + hkp1) * r2b));
+  w.f3 = (hk * (hkm1 + hk)) * r2a;
+  w.f4 = (hkp1 * (hkp1 + hkp2)) * r2b;
+  return w;
+}
+
+// redi_ppm_edge (redi/ocean_redi.F90:194-225) from its weights
+template <class R>
+__device__ __forceinline__ R ppm_edge_w(const PpmW<R>& w, R ak, R akp1, R pk, R pkp1) {
+  if (w.mean) return R(0.5) * (ak + akp1);
+  const R e = w.lower ? ak + w.we * (akp1 - ak) : akp1 + w.we * (ak - akp1);
+  return e + w.f1 * (w.f2 * (akp1 - ak) - (w.f3 * pkp1 - w.f4 * pk));
+}
+
+template <class R>
+__device__ __forceinline__ R ppm_edge(R hkm1, R hk, R hkp1, R hkp2, R ak, R akp1, R pk, R pkp1) {
+  return ppm_edge_w<R>(ppm_weights<R>(hkm1, hk, hkp1, hkp2), ak, akp1, pk, pkp1);
 }
 
 // Streaming form of redi_interface_scalar (redi/ocean_redi.F90:231-254) for
@@ -102,10 +150,12 @@ __device__ __forceinline__ void stream_edges(int nz, Load load, Emit emit) {
   emit(0, e, sw0);
   for (int k = 1; k < nz; ++k) {
     R dcur[NS];
+    const PlmW<R> pw = plm_weights<R>(hw1, hw2, hw3);
+    const PpmW<R> ew = ppm_weights<R>(hw0, hw1, hw2, hw3);
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      dcur[q] = (k <= nz - 2) ? plm_slope<R>(hw1, hw2, hw3, sw0[q], sw1[q], sw2[q]) : R(0);
-      e[q] = ppm_edge<R>(hw0, hw1, hw2, hw3, sw0[q], sw1[q], dprev[q], dcur[q]);
+      dcur[q] = (k <= nz - 2) ? plm_slope_w<R>(pw, sw0[q], sw1[q], sw2[q]) : R(0);
+      e[q] = ppm_edge_w<R>(ew, sw0[q], sw1[q], dprev[q], dcur[q]);
     }
     emit(k, e, sw0);
     R hn, sn[NS];

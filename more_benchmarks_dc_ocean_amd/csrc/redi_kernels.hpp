@@ -687,23 +687,23 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
   // the mean of the layer above it, and the records keep the edges as
   // offsets from their layer's mean.
   constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
+  using W = std::conditional_t<kOff, float, R>;  // arithmetic of stages 2-3
   // 2: slopes of layers kl = k0-1+r; 0 for the first and last layer (the
   // slope stream_edges starts from and the one it skips)
   for (int q = tid; q < NSL * CT; q += kReconLanes) {
     const int r = q / CT, cc = q % CT;
     const int kl = k0 - 1 + r;  // rows r, r+1, r+2 of sh / sv hold layers kl-1, kl, kl+1
     const bool in = kl >= 1 && kl <= nz - 2;
+    // thickness weights once for every tracer of the group
+    const PlmW<W> w = plm_weights<W>((W)sh[r][cc], (W)sh[r + 1][cc], (W)sh[r + 2][cc]);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if constexpr (kOff) {
-        sd[t][r][cc] = in ? (R)plm_slope<float>((float)sh[r][cc], (float)sh[r + 1][cc], (float)sh[r + 2][cc],
-                                                (float)(sv[t][r][cc] - sv[t][r + 1][cc]), 0.f,
-                                                (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]))
+        sd[t][r][cc] = in ? (R)plm_slope_w<float>(w, (float)(sv[t][r][cc] - sv[t][r + 1][cc]), 0.f,
+                                                  (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]))
                           : R(0);
       } else {
-        sd[t][r][cc] = in ? plm_slope<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sv[t][r][cc],
-                                         sv[t][r + 1][cc], sv[t][r + 2][cc])
-                          : R(0);
+        sd[t][r][cc] = in ? plm_slope_w<R>(w, sv[t][r][cc], sv[t][r + 1][cc], sv[t][r + 2][cc]) : R(0);
       }
     }
   }
@@ -713,6 +713,7 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
     const int r = q / CT, cc = q % CT;
     const int ki = k0 + r;  // sh / sv rows r..r+3: layers ki-2 .. ki+1 (clamped);
                             // sd rows r, r+1: slopes of layers ki-1, ki
+    const PpmW<W> w = ppm_weights<W>((W)sh[r][cc], (W)sh[r + 1][cc], (W)sh[r + 2][cc], (W)sh[r + 3][cc]);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       R e = R(0);
@@ -720,17 +721,14 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
         // offset from the mean of layer ki-1 (ki = 0 and ki = nz: 0, the edge
         // being the adjacent layer's mean there)
         if (ki > 0 && ki < nz)
-          e = (R)ppm_edge<float>((float)sh[r][cc], (float)sh[r + 1][cc], (float)sh[r + 2][cc],
-                                 (float)sh[r + 3][cc], 0.f,
-                                 (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]), (float)sd[t][r][cc],
-                                 (float)sd[t][r + 1][cc]);
+          e = (R)ppm_edge_w<float>(w, 0.f, (float)(sv[t][r + 2][cc] - sv[t][r + 1][cc]),
+                                   (float)sd[t][r][cc], (float)sd[t][r + 1][cc]);
       } else if (ki == 0) {
         e = sv[t][r + 2][cc] - R(0.5) * R(0);  // edge(1) = tr(1) - 0.5*diff(1), diff(1) = 0
       } else if (ki == nz) {
         e = sv[t][r + 1][cc] + R(0.5) * R(0);  // diff(nk) = 0
       } else if (ki < nz) {
-        e = ppm_edge<R>(sh[r][cc], sh[r + 1][cc], sh[r + 2][cc], sh[r + 3][cc], sv[t][r + 1][cc],
-                        sv[t][r + 2][cc], sd[t][r][cc], sd[t][r + 1][cc]);
+        e = ppm_edge_w<R>(w, sv[t][r + 1][cc], sv[t][r + 2][cc], sd[t][r][cc], sd[t][r + 1][cc]);
       }
       se[t][r][cc] = e;
     }
