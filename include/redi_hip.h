@@ -267,7 +267,7 @@ long long redi_state_bytes_form(int nx_total, int ny_total, int nz, int form);
  * by HIP events recorded on the launch's own stream.  redi_timing_collect
  * waits for those events, adds their elapsed times per kernel kind into
  * `out` and forgets them. */
-#define REDI_NKERNELS 8
+#define REDI_NKERNELS 9
 #define REDI_K_COLUMN_STATE 0
 #define REDI_K_SWEEP_U 1
 #define REDI_K_SWEEP_V 2
@@ -276,6 +276,7 @@ long long redi_state_bytes_form(int nx_total, int ny_total, int nz, int form);
 #define REDI_K_FLUX_X 5         /* u-face fluxes, added as W and E contributions */
 #define REDI_K_FLUX_Y 6         /* v-face fluxes, added as S and N contributions */
 #define REDI_K_TRACER_UPDATE 7  /* hTr += dTr / areaT */
+#define REDI_K_SWEEP_UV 8       /* both face grids' sweeps in one launch (64-lane tiles) */
 typedef struct {
     double ms[REDI_NKERNELS];           /* summed kernel time */
     long long launches[REDI_NKERNELS];  /* launches timed */

@@ -19,6 +19,8 @@ Each term is charged to the kernel that moves it:
 
     k_column_state      h
     k_neutral_sweep_u/v coefficient write of its face grid, wet of that grid
+                        (k_neutral_sweep_uv, which sweeps both grids in one
+                        launch: both)
     k_face_khtr         khtr_u / khtr_v
     k_tracer_recon      the group's tracer reads (one launch per tracer group)
     k_flux_x / k_flux_y coefficient read-back and dy_cu/idxCu (dx_cv/idyCv)
@@ -63,10 +65,12 @@ def bytes_per_cell_step(case, form: str = "fp64") -> float:
     return step_bytes(case, form) / case.interior_cells
 
 
-def step_launches(case, group: int, form: str = "fp64") -> Dict[str, list]:
+def step_launches(case, group: int, form: str = "fp64", merged_sweep: bool = True) -> Dict[str, list]:
     """Algorithmic bytes of every launch of one step, per kernel name, for
     tracer groups of ``group`` tracers, with fields and coefficients of
-    ``form``'s element sizes (the 2-D metrics are FP64 in every form)."""
+    ``form``'s element sizes (the 2-D metrics are FP64 in every form).
+    ``merged_sweep``: both face grids are swept by one launch (64-lane
+    sweep tiles, nx > 1 and ny > 1)."""
     d = _dims(case)
     nt = case.nt
     s = float(ELEMENT_BYTES[form][0])
@@ -77,6 +81,8 @@ def step_launches(case, group: int, form: str = "fp64") -> Dict[str, list]:
         out["k_neutral_sweep_u"] = [coeff_bytes(d["Nu"], d["ns"], form) + 8.0 * d["Nu"]]
     if d["ny"] > 1:
         out["k_neutral_sweep_v"] = [coeff_bytes(d["Nv"], d["ns"], form) + 8.0 * d["Nv"]]
+    if merged_sweep and d["nx"] > 1 and d["ny"] > 1:
+        out["k_neutral_sweep_uv"] = [out.pop("k_neutral_sweep_u")[0] + out.pop("k_neutral_sweep_v")[0]]
     fx, fy, cf, rc = [], [], [], []
     for gn in groups:
         share = gn / nt

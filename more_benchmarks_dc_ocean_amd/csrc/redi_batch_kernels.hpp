@@ -105,7 +105,7 @@ template <class F>
 __global__ void __launch_bounds__(kBatchSweepLanes) kb_neutral_sweep(BatchView<SweepItem<F>> v) {
   __shared__ uint4 stage[kSweepStageWords];
   for_each_block<false>(v, [&](const SweepItem<F>& it, int bid, int nb) {
-    neutral_sweep_body<F>(it.a, it.tm, bid, nb, stage);
+    neutral_sweep_body<F>(it.a, it.tm, bid, nb, stage, threadIdx.x);
   });
 }
 

@@ -214,12 +214,13 @@ struct TileMap {
   int tiles1;    // tiles along the slow axis
   int vertical;  // tile numbering: slow axis first
   // lanes of block `bid` of `nb` (tiles beyond the index space map outside it)
-  __device__ __forceinline__ bool map(int bid, int nb, int& a, int& b) const {
+  // (tid: the lane's index within its tile)
+  __device__ __forceinline__ bool map(int bid, int nb, int& a, int& b, unsigned tid) const {
     const int tile = xcd_band(bid, nb);
     const int t0 = vertical ? tile / tiles1 : tile % tiles0;
     const int t1 = vertical ? tile % tiles1 : tile / tiles0;
-    a = t0 * tx + (int)(threadIdx.x % (unsigned)tx);
-    b = t1 * ty + (int)(threadIdx.x / (unsigned)tx);
+    a = t0 * tx + (int)(tid % (unsigned)tx);
+    b = t1 * ty + (int)(tid / (unsigned)tx);
     return a < n0 && b < n1;
   }
 };
@@ -383,16 +384,18 @@ constexpr int kSweepRunWords = 8;                              // 16-byte words 
 constexpr int kSweepStageWords = 64 * kSweepRunWords;          // per wave
 constexpr size_t kSweepStageBytes = kSweepStageWords * sizeof(uint4);
 
+// `tid`: the lane's index within the tile; `stage`: the workgroup's LDS
+// stage (kSweepStageWords per wave).
 template <class F>
 __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const TileMap& tm,
-                                                   int bid, int nb, uint4* stage) {
+                                                   int bid, int nb, uint4* stage, unsigned tid) {
   using Face = FaceRecF<F>;
   constexpr int kW = (int)(sizeof(Face) / sizeof(uint4));
   constexpr int kRun = kSweepRunWords / kW;  // surfaces per run
   const int lane = (int)(threadIdx.x & 63);
   uint4* stw = stage + (threadIdx.x >> 6) * kSweepStageWords;
   int fi, fj;
-  const bool inr = tm.map(bid, nb, fi, fj);
+  const bool inr = tm.map(bid, nb, fi, fj, tid);
   const int i = a.i0 + fi, j = a.j0 + fj;
   const uint32_t fo = inr ? (uint32_t)i + (uint32_t)a.nfa * (uint32_t)j : 0u;
   const int nk = a.nz, ns = 2 * nk + 2;
@@ -556,7 +559,26 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
 template <class F>
 __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap tm) {
   extern __shared__ uint4 sweep_stage[];
-  neutral_sweep_body<F>(a, tm, (int)blockIdx.x, (int)gridDim.x, sweep_stage);
+  neutral_sweep_body<F>(a, tm, (int)blockIdx.x, (int)gridDim.x, sweep_stage, threadIdx.x);
+}
+
+// Both face grids in one launch, for 64-lane tiles: wave 0 of a workgroup
+// sweeps a tile of u faces, wave 1 the tile of v faces at the same tile
+// position.  The u tile reads the columns of row j, the v tile those of rows
+// j-1 and j, so the two waves read the columns of row j at about the same
+// time, from the same caches: the column state is fetched about once for
+// both grids where two launches fetch it once each.  Both maps must number
+// the same tile grid (tiles0, tiles1); tiles outside a grid's faces have no
+// lanes in range.  Each wave runs the same sequence of barriers (ns steps of
+// one domain), so the workgroup barriers of the record staging pair up.
+template <class F>
+__global__ void __launch_bounds__(128) k_neutral_sweep_uv(SweepArgs<F> au, TileMap mu, SweepArgs<F> av,
+                                                          TileMap mv) {
+  extern __shared__ uint4 sweep_stage[];
+  if (threadIdx.x < 64)
+    neutral_sweep_body<F>(au, mu, (int)blockIdx.x, (int)gridDim.x, sweep_stage, threadIdx.x);
+  else
+    neutral_sweep_body<F>(av, mv, (int)blockIdx.x, (int)gridDim.x, sweep_stage, threadIdx.x - 64);
 }
 
 // ---------------------------------------------------------------------------
@@ -920,22 +942,33 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
     const bool contrib = surf && lane < kChunk && s < nsub && Fs.hEff != 0.0;
     const C hEff = (C)Fs.hEff;
     C x[NT];
-    // one tracer's records at a time: a wave holds two gathered records
+    // The FP32-store form gathers every tracer's two 16-byte records before
+    // computing (one round trip per chunk, eight records in flight); the FP64
+    // form gathers one tracer's 32-byte records at a time, which keeps its
+    // registers, and so its waves per SIMD, at the faster setting.
+    constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
+    ReconRecOff gl[kOff ? NT : 1], gr[kOff ? NT : 1];
+    if constexpr (kOff) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        gl[t] = gr[t] = ReconRecOff{0.0, 0.f, 0.f};
+        if (surf) {
+          gl[t] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
+          gr[t] = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
+        }
+      }
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       x[t] = C(0);
       bool zero = false;
-      if constexpr (std::is_same<typename F::Recon, ReconRecOff>::value) {
+      if constexpr (kOff) {
         // FP32-store form: FP64 means, FP32 offsets.  The sublayer's top
         // difference is D + (eR - eL) with D the FP64 difference of the two
         // layer means and eL, eR FP32 edge offsets; its bottom difference is
         // the next surface's top one; its layer difference is D plus the
         // difference of the FP32 parabola means about the layer means.
-        ReconRecOff ql{0.0, 0.f, 0.f}, qr = ql;
-        if (surf) {
-          ql = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
-          qr = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
-        }
+        const ReconRecOff ql = gl[t], qr = gr[t];
         const double D = qr.Tl - ql.Tl;
         const float eL = (1.f - PoL) * ql.oT + PoL * ql.oB;
         const float eR = (1.f - PoR) * qr.oT + PoR * qr.oB;

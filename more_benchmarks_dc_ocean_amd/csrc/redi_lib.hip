@@ -237,6 +237,23 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
+  if (bs == 64 && nx > 1 && ny > 1) {  // both grids in one launch, one wave per grid
+    SweepArgs<F> au{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec, P,
+                    (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
+    SweepArgs<F> av{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec, P,
+                    (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
+    unsigned nbu, nbv;
+    TileMap mu = make_tiles(nx - 1, ny, tiles, &nbu);
+    TileMap mv = make_tiles(nx, ny - 1, tiles, &nbv);
+    mu.tiles0 = mv.tiles0 = mu.tiles0 > mv.tiles0 ? mu.tiles0 : mv.tiles0;
+    mu.tiles1 = mv.tiles1 = mu.tiles1 > mv.tiles1 ? mu.tiles1 : mv.tiles1;
+    slot = tm.begin(REDI_K_SWEEP_UV, st);
+    k_neutral_sweep_uv<F><<<(unsigned)mu.tiles0 * (unsigned)mu.tiles1, 128, 2 * kSweepStageBytes, st>>>(
+        au, mu, av, mv);
+    tm.end(slot, st);
+    HIP_TRY(hipGetLastError());
+    return REDI_OK;
+  }
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
     SweepArgs<F> a{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec, P,
                    (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
@@ -695,7 +712,7 @@ extern "C" {
 
 const char* redi_last_error(void) { return g_err.c_str(); }
 
-int redi_version(void) { return 400; }
+int redi_version(void) { return 500; }
 
 long long redi_workspace_bytes(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return 0;
@@ -1016,7 +1033,8 @@ const char* redi_kernel_name(int kind) {
   static const char* names[REDI_NKERNELS] = {"k_column_state", "k_neutral_sweep_u",
                                              "k_neutral_sweep_v", "k_face_khtr",
                                              "k_tracer_recon", "k_flux_x",
-                                             "k_flux_y", "k_tracer_update"};
+                                             "k_flux_y", "k_tracer_update",
+                                             "k_neutral_sweep_uv"};
   return (kind >= 0 && kind < REDI_NKERNELS) ? names[kind] : "";
 }
 

@@ -86,7 +86,7 @@ EXPORTED = ("redi_init", "redi_destroy", "redi_configure", "redi_export_coeffs",
             "redi_batch_add", "redi_batch_remove", "redi_batch_step", "redi_batch_step_serial",
             "redi_batch_export_coeffs", "redi_batch_info")
 FORMS = {"fp64": 0, "fp32_store": 1}  # REDI_FORM_*
-NKERNELS = 8  # REDI_NKERNELS
+NKERNELS = 9  # REDI_NKERNELS
 
 
 class KernelTimes(ctypes.Structure):

@@ -35,12 +35,15 @@ def test_hand_count_tiny_domain():
     assert bm.step_bytes(c) == 8 + 16 + 2 * 4 * (4 * 24 + 3 * 8) + 8 * 17
 
 
+@pytest.mark.parametrize("merged", [True, False])
 @pytest.mark.parametrize("form", ["fp64", "fp32_store"])
 @pytest.mark.parametrize("group", [1, 2, 3, 4])
 @pytest.mark.parametrize("nt", [1, 2, 3, 4])
-def test_launch_split_adds_up(group, nt, form):
+def test_launch_split_adds_up(group, nt, form, merged):
     c = _Sizes(100, 70, 30, nt)
-    launches = bm.step_launches(c, group, form)
+    launches = bm.step_launches(c, group, form, merged)
+    assert ("k_neutral_sweep_uv" in launches) == merged
+    assert ("k_neutral_sweep_u" in launches) == (not merged)
     total = sum(sum(v) for v in launches.values())
     assert total == pytest.approx(bm.step_bytes(c, form), rel=1e-12)
     ngroups = -(-nt // max(1, min(group, nt)))
