@@ -258,7 +258,7 @@ struct BatchOps {
         ua.tend[t] = nullptr;
       }
       ua.nt = NT;
-      const ReconLayout ry = recon_layout(0, N2, nz, NT);
+      const ReconLayout ry = recon_layout(0, N2, nz + F::Recon::kExtra, NT);
       fx.ry = fy.ry = ry;
       w[kNodeRecon].add(Recon{ReconArgs<F, NT>{N2, nz, h, ts, (typename F::Recon*)sc.recon, ry}, d.tiles.recon_kt, 0},
                         recon_tiled_blocks(N2, nz, d.tiles.recon_kt));

@@ -173,10 +173,11 @@ __device__ __forceinline__ void stream_edges(int nz, Load load, Emit emit) {
   emit(nz, e, sw0);
 }
 
-// PPM limiter of redi_tracer_column (redi/ocean_redi.F90:876-890).  Returns
-// the branch taken: 0 edges kept, 1 both flattened to the mean, 2 the top
-// edge rebuilt from the mean and the bottom one, 3 the bottom edge rebuilt
-// from the mean and the top one (ppm_limited_edges replays it).
+// PPM limiter of redi_tracer_column (redi/ocean_redi.F90:876-890): limits
+// the edges al / ar of a layer of mean tl in place.  Returns the branch
+// taken: 0 edges kept, 1 both flattened to the mean, 2 the top edge rebuilt
+// from the mean and the bottom one, 3 the bottom edge rebuilt from the mean
+// and the top one.
 template <class R>
 __device__ __forceinline__ int ppm_limit(R tl, R& al, R& ar) {
   if (signum1<R>(ar - tl) * signum1<R>(tl - al) <= R(0)) {
@@ -191,16 +192,6 @@ __device__ __forceinline__ int ppm_limit(R tl, R& al, R& ar) {
     return 3;
   }
   return 0;
-}
-
-// The limited edges of a layer from its mean, its unlimited edges and the
-// branch ppm_limit took on them: the same expressions on the same operands,
-// so the same bits.
-template <class R>
-__device__ __forceinline__ void ppm_limited_edges(int branch, R tl, R et, R eb, R& al, R& ar) {
-  const R x = tl + R(2) * (tl - (branch == 2 ? eb : et));
-  al = branch == 1 ? tl : (branch == 2 ? x : et);
-  ar = branch == 1 ? tl : (branch == 3 ? x : eb);
 }
 
 // redi_interpolate_position (redi/ocean_redi.F90:260-285)

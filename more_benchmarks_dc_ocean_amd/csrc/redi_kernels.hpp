@@ -118,23 +118,24 @@ struct LayerVals {
   C Tl, aL, aR, Tt, Tb;
 };
 
-// One layer of a tracer column (K3 output) in the plain FP64 form: the
-// layer mean, the PPM interface values at its top and bottom, and the
-// branch the PPM limiter took.  Every branch yields the limited edges from
-// exactly these three values (kept, flattened to the mean, or one edge
-// rebuilt from the mean and the other), so the reader replays the branch on
-// the same operands and obtains the same bits.  32 bytes, all written.
-struct alignas(16) ReconRec64 {
-  double Tl, Tt, Tb;
-  int32_t branch, pad;
-  static __device__ __forceinline__ ReconRec64 make(double tl, double et, double eb) {
-    double al = et, ar = eb;
-    return ReconRec64{tl, et, eb, ppm_limit<double>(tl, al, ar), 0};
-  }
-  __device__ __forceinline__ LayerVals<double> get() const {
-    double al, ar;
-    ppm_limited_edges<double>(branch, Tl, Tt, Tb, al, ar);
-    return LayerVals<double>{Tl, al, ar, Tt, Tb};
+// A tracer column (K3 output) in the plain FP64 form: per layer, its mean
+// and the PPM interface value at its bottom, 16 bytes; entry 0 of a column
+// holds the value at the top interface (in Tb), entry K (K = 1..nz, top-down)
+// layer K.  A layer's top interface value is the bottom one of the entry
+// above, since the unlimited PPM edge is one value per interface
+// (redi/ocean_redi.F90:231-254), so a column stores each value once.  The
+// reader gathers entries K-1 and K and runs the PPM limiter
+// (redi/ocean_redi.F90:876-890) on the same three operands as the
+// reference, so it obtains the same bits.
+struct alignas(16) ReconPair64 {
+  double Tl, Tb;
+  static constexpr int kExtra = 1;  // entries per column beyond one per layer
+  // layer K from entries K-1 (above) and K
+  static __device__ __forceinline__ LayerVals<double> layer(const ReconPair64& above,
+                                                           const ReconPair64& k) {
+    double al = above.Tb, ar = k.Tb;
+    (void)ppm_limit<double>(k.Tl, al, ar);
+    return LayerVals<double>{k.Tl, al, ar, above.Tb, k.Tb};
   }
 };
 
@@ -148,6 +149,7 @@ struct alignas(16) ReconRec64 {
 struct alignas(16) ReconRecOff {
   double Tl;
   float oT, oB;
+  static constexpr int kExtra = 0;  // one entry per layer
   static __device__ __forceinline__ ReconRecOff make(double tl, double et, double eb) {
     return ReconRecOff{tl, (float)(et - tl), (float)(eb - tl)};
   }
@@ -165,7 +167,7 @@ struct Fp64 {
   using C = double;
   using Face = FaceRec64;
   using Col = ColRecT<double>;
-  using Recon = ReconRec64;
+  using Recon = ReconPair64;
 };
 struct Fp32 {
   using R = float;
@@ -636,9 +638,13 @@ __global__ void __launch_bounds__(128) k_tracer_recon(ReconArgs<F, NT> a) {
   auto emit = [&](int k, const C* e, const C* s_above) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (k > 0)
-        rc[(size_t)(k - 1) * a.y.cb + (size_t)t * a.y.ct] = F::Recon::make(s_above[t], e_prev[t], e[t]);
-      e_prev[t] = e[t];
+      if constexpr (F::Recon::kExtra) {  // entry k: layer k's mean (none above the top), interface k
+        rc[(size_t)k * a.y.cb + (size_t)t * a.y.ct] = typename F::Recon{k > 0 ? s_above[t] : 0.0, e[t]};
+      } else {
+        if (k > 0)
+          rc[(size_t)(k - 1) * a.y.cb + (size_t)t * a.y.ct] = F::Recon::make(s_above[t], e_prev[t], e[t]);
+        e_prev[t] = e[t];
+      }
     }
   };
   stream_edges<C, NT>(nz, load, emit);
@@ -769,7 +775,9 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
         const float oT = kl == 0 ? 0.f : (float)((sv[t][kk + 1][cc] - sv[t][kk + 2][cc]) + se[t][kk][cc]);
         rc[(size_t)t * a.y.ct] = ReconRecOff{sv[t][kk + 2][cc], oT, (float)se[t][kk + 1][cc]};
       } else {
-        rc[(size_t)t * a.y.ct] = F::Recon::make(sv[t][kk + 2][cc], se[t][kk][cc], se[t][kk + 1][cc]);
+        // entry kl + 1: layer kl's mean and bottom value; entry 0: the top value
+        rc[(size_t)a.y.cb + (size_t)t * a.y.ct] = typename F::Recon{sv[t][kk + 2][cc], se[t][kk + 1][cc]};
+        if (kl == 0) rc[(size_t)t * a.y.ct] = typename F::Recon{0.0, se[t][kk][cc]};
       }
     }
   }
@@ -794,11 +802,12 @@ inline unsigned recon_tiled_blocks(int N2, int nz, int kt) {
 // ---------------------------------------------------------------------------
 
 // Pieces of redi_sublayer_dT (:893-925).  The interface value at position
-// Po in layer K of a column, (1 - Po) * T(K-1) + Po * T(K), from layer K's
-// record (the tlt / tlb / trt / trb expressions, :905-912):
+// Po in layer K of a column, (1 - Po) * T(K-1) + Po * T(K), from the values
+// at the layer's top (tt) and bottom (tb) interfaces (the tlt / tlb / trt /
+// trb expressions, :905-912):
 template <class C>
-__device__ __forceinline__ C edge_value(const LayerVals<C>& r, C Po) {
-  return (C(1) - Po) * r.Tt + Po * r.Tb;
+__device__ __forceinline__ C edge_value(C tt, C tb, C Po) {
+  return (C(1) - Po) * tt + Po * tb;
 }
 
 // The down-gradient guard (:913-924): the flux is zeroed unless the top
@@ -986,14 +995,19 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
           x[t] = zero ? C(0) : ((C)dlay * hEff) * coef;
         }
       } else {
-        LayerVals<C> qL{C(0), C(0), C(0), C(0), C(0)}, qR = qL;
+        // entries K-1 and K of each column (layer K's top value, mean and
+        // bottom value); the limited edges only where a flux is formed
+        ReconPair64 la{0.0, 0.0}, lk = la, ra = la, rk = la;
         if (surf) {
-          qL = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct].get();
-          qR = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct].get();
+          la = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
+          lk = recL[(uint32_t)KoL * cb + (uint32_t)t * ct];
+          ra = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
+          rk = recR[(uint32_t)KoR * cb + (uint32_t)t * ct];
         }
-        const C eL = edge_value(qL, PoL), eR = edge_value(qR, PoR);
+        const C eL = edge_value<C>(la.Tb, lk.Tb, PoL), eR = edge_value<C>(ra.Tb, rk.Tb, PoR);
         const C eLb = __shfl_down(eL, 1, 64), eRb = __shfl_down(eR, 1, 64);
         if (contrib) {
+          const LayerVals<C> qL = ReconPair64::layer(la, lk), qR = ReconPair64::layer(ra, rk);
           const C tlay = ppm_ave<C>(PoL, PoLb + (C)(KoLb - KoL), qL.aL, qL.aR, qL.Tl);
           const C trlay = ppm_ave<C>(PoR, PoRb + (C)(KoRb - KoR), qR.aL, qR.aR, qR.Tl);
           const C d = guarded_dT<C>(eL, eLb, eR, eRb, tlay, trlay, zero);

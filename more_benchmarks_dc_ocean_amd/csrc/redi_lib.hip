@@ -88,12 +88,13 @@ size_t padded_faces(size_t nf) { return (nf + kFacePad - 1) / kFacePad * kFacePa
 // Bytes of the per-form records and values (the column state is FP64 in
 // both forms).
 struct FormSizes {
-  size_t face, recon, val, col;  // FaceRec, ReconRec, stored value, ColRec
+  size_t face, recon, val, col;  // FaceRec, reconstruction entry, stored value, ColRec
+  int recon_extra;               // reconstruction entries per column beyond one per layer
 };
 template <class F>
 constexpr FormSizes form_sizes() {
   return FormSizes{sizeof(FaceRecF<F>), sizeof(typename F::Recon), sizeof(typename F::R),
-                   sizeof(typename F::Col)};
+                   sizeof(typename F::Col), F::Recon::kExtra};
 }
 FormSizes sizes_of(int form) {
   return form == REDI_FORM_FP32_STORE ? form_sizes<Fp32>() : form_sizes<Fp64>();
@@ -102,12 +103,13 @@ FormSizes sizes_of(int form) {
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
 //   Phase A: ColRec + P per column interface.
-//   Phase B, per tracer slot: a ReconRec and an FP64 dTr value per layer of
-//   every column; once: a discard layer field that takes the second output
-//   of a two-tracer launch run for a lone tracer.
+//   Phase B, per tracer slot: the reconstruction entries of every column
+//   (one per layer, plus recon_extra) and an FP64 dTr value per layer;
+//   once: a discard layer field that takes the second output of a
+//   two-tracer launch run for a lone tracer.
 // Every array starts on a 256-byte boundary.
 struct PhaseBScratch {
-  void* recon;       // N2 * nz * slots records (ReconLayout)
+  void* recon;       // N2 * (nz + recon_extra) * slots entries (ReconLayout)
   void* discard;     // N2 * nz values
   double* acc;       // N2 * nz * slots dTr values
   size_t bytes;
@@ -123,7 +125,7 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const
     return p;
   };
   PhaseBScratch s;
-  s.recon = take(z.recon * N2 * nz * slots);
+  s.recon = take(z.recon * N2 * (size_t)(nz + z.recon_extra) * slots);
   s.discard = take(z.val * N2 * nz);
   s.acc = (double*)take(sizeof(double) * N2 * nz * slots);
   s.bytes = off;
@@ -383,11 +385,12 @@ void phase_b_args(const PhaseB& p, const FaceState& fs, const PhaseBScratch& sc,
   ua.N2 = nx * ny; ua.nz = nz; ua.areaT = p.areaT; ua.acc = sc.acc;
 }
 
-// Reconstruction layout of a group of k tracers.
-ReconLayout recon_layout(int layer_major, int N2, int nz, int k) {
+// Reconstruction layout of a group of k tracers with ne entries per column
+// and tracer (nz layers plus the form's recon_extra).
+ReconLayout recon_layout(int layer_major, int N2, int ne, int k) {
   const uint32_t k32 = (uint32_t)k;
   return layer_major ? ReconLayout{k32, (uint32_t)N2 * k32, 1}
-                     : ReconLayout{(uint32_t)nz * k32, 1, (uint32_t)nz};
+                     : ReconLayout{(uint32_t)ne * k32, 1, (uint32_t)ne};
 }
 
 // Sublayer chunks per face of the guard-decision output.
@@ -430,7 +433,7 @@ int launch_phase_b(const PhaseB& p, const typename F::R* h, int nt, typename F::
       ua.tend[1] = nullptr;
     }
     ua.nt = kn;
-    const ReconLayout ry = recon_layout(tiles.recon_layer_major, N2, nz, kn);
+    const ReconLayout ry = recon_layout(tiles.recon_layer_major, N2, nz + F::Recon::kExtra, kn);
     fx.ry = fy.ry = ry;
     if (guard_u && guard_v) {
       const size_t nch = guard_chunks(nz);
