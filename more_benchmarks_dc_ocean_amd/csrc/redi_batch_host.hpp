@@ -217,13 +217,12 @@ struct BatchOps {
       R* const* tr = (R* const*)d.tr.data();
       // Phase A (launch_phase_a)
       typename F::Col* rec = (typename F::Col*)d.ws;
-      double* P = (double*)(rec + (size_t)N2 * (nz + 1));
-      w[kNodeColumn].add(ColumnArgs<F>{N2, nz, nx, h, tr[d.idx_t - 1], tr[d.idx_s - 1], d.eos, rec, P},
+      w[kNodeColumn].add(ColumnArgs<F>{N2, nz, nx, h, tr[d.idx_t - 1], tr[d.idx_s - 1], d.eos, rec},
                          blocks_for(N2, 128));
       const double pa_to_h = 1.0 / (kGravity * d.eos.rho0);
       if (nx > 1) {
         SweepItem<F> it{SweepArgs<F>{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0,
-                                     d.wet_u, rec, P,
+                                     d.wet_u, rec,
                                      (FaceRecF<F>*)d.fs.u, d.fs.mono_u, pa_to_h, d.p.idxCu, d.slope_max},
                         TileMap{}};
         unsigned nb;
@@ -232,7 +231,7 @@ struct BatchOps {
       }
       if (ny > 1) {
         SweepItem<F> it{SweepArgs<F>{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1,
-                                     d.wet_v, rec, P,
+                                     d.wet_v, rec,
                                      (FaceRecF<F>*)d.fs.v, d.fs.mono_v, pa_to_h, d.p.idyCv, d.slope_max},
                         TileMap{}};
         unsigned nb;

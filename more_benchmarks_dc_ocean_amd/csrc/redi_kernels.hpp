@@ -26,8 +26,9 @@
 //
 // Device memory layout (R = the form's storage type):
 //   cell fields h, hTr          [k][j][i] (reference layout, k native bottom-up)
-//   column state (K1 -> K2)     FP64 in both forms, interface-major within
-//                               each row: ColRec / P of column (i, j) at
+//   column state (K1 -> K2)     interface T, S and the next interface's
+//                               pressure FP64 in both forms, interface-major
+//                               within each row: ColRec of column (i, j) at
 //                               interface kt (top-down, 0..nz) at
 //                               (j*(nz+1) + kt)*nx + i
 //   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
@@ -60,12 +61,17 @@
 
 namespace redi {
 
-// One interface of a column (K1 output): interface T and S (FP64 in both
-// forms: the sweep's density differences are differences of them) and
-// dR/dT, dR/dS in the form's EOS type E (32 bytes in FP64, 24 in FP32).
+// One interface k of a column (K1 output): interface T and S (FP64 in both
+// forms: the sweep's density differences are differences of them), the
+// pressure at the next interface down, Pn = P(min(k+1, nz)) (the surface
+// pressure P(0) is 0), and dR/dT, dR/dS in the form's EOS type E: 40 bytes
+// in FP64, 32 in FP32.  The sweep's window over a column holds the records
+// of two interfaces and the pressures of three, the third one interface
+// deeper than the records; carrying it in the record above, one load of a
+// record brings both.
 template <class E>
-struct alignas(sizeof(E) == 8 ? 16 : 8) ColRecT {
-  double T, S;
+struct alignas(sizeof(E) == 8 ? 8 : 16) ColRecT {
+  double T, S, Pn;
   E dT, dS;
 };
 
@@ -251,7 +257,6 @@ struct ColumnArgs {
   const typename F::R* shtr;
   EosParams eos;
   typename F::Col* rec;
-  double* P;
 };
 
 // Lane = column; `bid` is the workgroup's block index (blockDim.x lanes).
@@ -270,8 +275,10 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
   const double g_rho0 = kGravity * eos.rho0;
   const size_t cb = colstate_base((size_t)c, a.nx, nz), cs = (size_t)a.nx;
   typename F::Col* r = a.rec + cb;
-  double* pc = a.P + cb;
   double P = 0.0;  // Pint(1) = 0
+  // interface k's record is written once P(k+1) is known, at interface k+1
+  C Tp = 0.0, Sp = 0.0;
+  E dTp = E(0), dSp = E(0);
   auto load = [&](int kt, C& hh, C* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
     hh = (C)h[o];
@@ -283,10 +290,11 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
     if (k > 0) P = P + g_rho0 * (double)h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
     E dRdT, dRdS;
     interface_drho<E>(eos, (E)e[0], (E)e[1], (E)P, dRdT, dRdS);
-    r[(size_t)k * cs] = typename F::Col{e[0], e[1], dRdT, dRdS};
-    pc[(size_t)k * cs] = P;
+    if (k > 0) r[(size_t)(k - 1) * cs] = typename F::Col{Tp, Sp, P, dTp, dSp};
+    Tp = e[0]; Sp = e[1]; dTp = dRdT; dSp = dRdS;
   };
   stream_edges<C, 2>(nz, load, emit);
+  r[(size_t)nz * cs] = typename F::Col{Tp, Sp, P, dTp, dSp};  // Pn = P(nz)
 }
 
 template <class F>
@@ -316,15 +324,14 @@ __device__ __forceinline__ double sec_drho(const ColRecT<E>& a, const ColRecT<E>
 template <class Col>
 struct SweepSide {
   const Col* rec;
-  const double* P;
   size_t st;  // elements between consecutive interfaces of the column
   int k, m;
   Col w0, w1;
   double p0, p1, p2;
-  __device__ __forceinline__ void init(const Col* r, const double* p, size_t stride, int nk) {
-    rec = r; P = p; st = stride; k = 1; m = 1;
+  __device__ __forceinline__ void init(const Col* r, size_t stride) {
+    rec = r; st = stride; k = 1; m = 1;
     w0 = r[0]; w1 = r[stride];
-    p0 = p[0]; p1 = p[stride]; p2 = p[(size_t)(nk < 2 ? nk : 2) * stride];
+    p0 = 0.0; p1 = w0.Pn; p2 = w1.Pn;  // P(1) = 0 at the surface, P(2), P(min(3, nk+1))
   }
   __device__ __forceinline__ Col cur() const { return k == 1 ? w0 : w1; }  // record at k
   __device__ __forceinline__ double pk() const { return k == 1 ? p0 : p1; }        // P(k)
@@ -333,7 +340,7 @@ struct SweepSide {
     if (k >= 2) {
       m += 1;
       w0 = w1; w1 = rec[(size_t)m * st];
-      p0 = p1; p1 = p2; p2 = P[(size_t)(m + 1 < nk ? m + 1 : nk) * st];
+      p0 = p1; p1 = p2; p2 = w1.Pn;
     }
     k += 1;
   }
@@ -348,7 +355,6 @@ struct SweepArgs {
   int di, dj;              // left column = (i - di, j - dj)
   const double* wet;
   const typename F::Col* rec;
-  const double* P;
   FaceRecF<F>* out;        // [s][face]
   uint8_t* mono;           // [face]
   double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
@@ -431,8 +437,8 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   SweepSide<typename F::Col> L{}, Rt{};
   if (wet) {
     const size_t bl = colstate_base(cl, a.nx, nk), br = colstate_base(cr, a.nx, nk);
-    L.init(a.rec + bl, a.P + bl, (size_t)a.nx, nk);
-    Rt.init(a.rec + br, a.P + br, (size_t)a.nx, nk);
+    L.init(a.rec + bl, (size_t)a.nx);
+    Rt.init(a.rec + br, (size_t)a.nx);
   }
 
   int lastK_l = 1, lastK_r = 1, contK_l = 0, contK_r = 0;
@@ -441,6 +447,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   SurfVals pend{};
   for (int ks = 1; ks <= ns; ++ks) {
    if (wet) {
+    bool advL = false, advR = false;  // the side whose pointer moves on, after both branches
     if (!reached_bottom) {
       const double dRho = sec_drho(Rt.cur(), L.cur());
       if (dRho < 0.0)
@@ -482,7 +489,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         PoR = 0.0; KoR = Rt.k;
         const double q = Rt.pk();
         posR = q + 0.0 * (Rt.pk1() - q);
-        Rt.advance(nk);
+        advR = true;
       } else {
         PoR = 1.0; KoR = nk;
         posR = Rt.p0 + 1.0 * (Rt.p1 - Rt.p0);
@@ -516,13 +523,18 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         PoL = 0.0; KoL = L.k;
         const double q = L.pk();
         posL = q + 0.0 * (L.pk1() - q);
-        L.advance(nk);
+        advL = true;
       } else {
         PoL = 1.0; KoL = nk;
         posL = L.p0 + 1.0 * (L.p1 - L.p0);
         reached_bottom = true; search_l = true;
       }
     }
+    // Moving a pointer after the two branches, not inside them, keeps its
+    // record load out of the branches' register merges, which otherwise
+    // wait for the load to land before the step ends.
+    if (advL) L.advance(nk);
+    if (advR) Rt.advance(nk);
     lastK_l = KoL; lastP_l = PoL; lastK_r = KoR; lastP_r = PoR;
     if (ks > 1) {  // sublayer between surfaces ks-1 and ks (:452-462)
       const double hL = posL - lastPos_l;

@@ -102,7 +102,7 @@ FormSizes sizes_of(int form) {
 
 // Device scratch of one step; Phase A and Phase B are live at different
 // times and share one buffer.
-//   Phase A: ColRec + P per column interface.
+//   Phase A: a ColRec per column interface.
 //   Phase B, per tracer slot: the reconstruction entries of every column
 //   (one per layer, plus recon_extra) and an FP64 dTr value per layer;
 //   once: a discard layer field that takes the second output of a
@@ -134,7 +134,7 @@ PhaseBScratch carve_phase_b(void* base, int nx, int ny, int nz, int slots, const
 
 size_t workspace_bytes(int nx, int ny, int nz, int nt, int form) {
   const FormSizes z = sizes_of(form);
-  const size_t phase_a = (z.col + sizeof(double)) * (nz + 1) * (size_t)nx * ny;
+  const size_t phase_a = z.col * (nz + 1) * (size_t)nx * ny;
   return std::max(phase_a, carve_phase_b(nullptr, nx, ny, nz, recon_slots(nt), z).bytes);
 }
 
@@ -233,16 +233,15 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
                    hipStream_t st, Timer& tm, const Taper& tp = Taper{}) {
   const int N2 = nx * ny;
   typename F::Col* rec = (typename F::Col*)ws;
-  double* P = (double*)(rec + (size_t)N2 * (nz + 1));
   int slot = tm.begin(REDI_K_COLUMN_STATE, st);
-  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, nx, h, t, s, eos, rec, P});
+  k_column_state<F><<<blocks_for(N2, 128), 128, 0, st>>>(ColumnArgs<F>{N2, nz, nx, h, t, s, eos, rec});
   tm.end(slot, st);
   const double pa_to_h = 1.0 / (kGravity * eos.rho0);
   const int bs = tiles.tx * tiles.ty;
   if (bs == 64 && nx > 1 && ny > 1) {  // both grids in one launch, one wave per grid
-    SweepArgs<F> au{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec, P,
+    SweepArgs<F> au{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec,
                     (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
-    SweepArgs<F> av{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec, P,
+    SweepArgs<F> av{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec,
                     (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
     unsigned nbu, nbv;
     TileMap mu = make_tiles(nx - 1, ny, tiles, &nbu);
@@ -257,7 +256,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
     return REDI_OK;
   }
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
-    SweepArgs<F> a{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec, P,
+    SweepArgs<F> a{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec,
                    (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
@@ -266,7 +265,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
     tm.end(slot, st);
   }
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
-    SweepArgs<F> a{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec, P,
+    SweepArgs<F> a{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec,
                    (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
