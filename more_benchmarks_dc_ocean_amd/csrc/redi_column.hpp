@@ -26,11 +26,15 @@ __device__ __forceinline__ R f_sign(R a, R b) {  // Fortran SIGN(a, b)
   return copysign(fabs(a), b);
 }
 template <class R>
-__device__ __forceinline__ R signum1(R x) {  // redi_signum(1,x) / redi_signum1
-  return x == R(0) ? R(0) : copysign(R(1), x);
-}
-template <class R>
 __device__ __forceinline__ R fmin_sel(R a, R b) { return a < b ? a : b; }
+// The reference's sign test redi_signum(1,a) * redi_signum(1,b) <= 0
+// (redi_signum(1,x) is 0 at x == 0, else 1 with the sign of x), without the
+// products: true when either value is zero or their sign bits differ (a NaN
+// counts with its sign bit, as copysign gives it), so the same decisions.
+template <class R>
+__device__ __forceinline__ bool sgn_prod_le0(R a, R b) {
+  return a == R(0) || b == R(0) || (__builtin_signbit(a) != __builtin_signbit(b));
+}
 template <class R>
 __device__ __forceinline__ R fmax_sel(R a, R b) { return a > b ? a : b; }
 
@@ -76,7 +80,7 @@ __device__ __forceinline__ R plm_slope_w(const PlmW<R>& w, R sm1, R s0, R sp1) {
   const R dc = fv_diff_w<R>(w, sm1, s0, sp1);
   const R dl = R(2) * (s0 - sm1);
   const R dr = R(2) * (sp1 - s0);
-  if (signum1<R>(dl) * signum1<R>(dr) <= R(0)) return R(0);
+  if (sgn_prod_le0<R>(dl, dr)) return R(0);
   const R m = fmin_sel<R>(fmin_sel<R>(fabs(dl), fabs(dc)), fabs(dr));
   return f_sign<R>(m, dc);
 }
@@ -180,7 +184,7 @@ __device__ __forceinline__ void stream_edges(int nz, Load load, Emit emit) {
 // and the top one.
 template <class R>
 __device__ __forceinline__ int ppm_limit(R tl, R& al, R& ar) {
-  if (signum1<R>(ar - tl) * signum1<R>(tl - al) <= R(0)) {
+  if (sgn_prod_le0<R>(ar - tl, tl - al)) {
     al = tl;
     ar = tl;
     return 1;

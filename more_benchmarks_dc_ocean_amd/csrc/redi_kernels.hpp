@@ -827,8 +827,7 @@ __device__ __forceinline__ C edge_value(C tt, C tb, C Po) {
 // agree in sign.
 template <class R>
 __device__ __forceinline__ bool guard_zero(R dT_top, R dT_bot, R dT_ave, R dT_layer) {
-  return signum1<R>(dT_top) * signum1<R>(dT_bot) <= R(0) ||
-         signum1<R>(dT_ave) * signum1<R>(dT_layer) <= R(0);
+  return sgn_prod_le0<R>(dT_top, dT_bot) || sgn_prod_le0<R>(dT_ave, dT_layer);
 }
 
 // redi_sublayer_dT's result from the interface values at the sublayer's
