@@ -223,7 +223,8 @@ struct BatchOps {
       if (nx > 1) {
         SweepItem<F> it{SweepArgs<F>{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0,
                                      d.wet_u, rec,
-                                     (FaceRecF<F>*)d.fs.u, d.fs.mono_u, pa_to_h, d.p.idxCu, d.slope_max},
+                                     (FaceRecF<F>*)d.fs.u, d.fs.mono_u, pa_to_h, d.p.idxCu, d.slope_max,
+                                     d.fs.filled ? 1 : 0},
                         TileMap{}};
         unsigned nb;
         it.tm = make_tiles(nx - 1, ny, d.tiles, &nb);
@@ -232,7 +233,8 @@ struct BatchOps {
       if (ny > 1) {
         SweepItem<F> it{SweepArgs<F>{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1,
                                      d.wet_v, rec,
-                                     (FaceRecF<F>*)d.fs.v, d.fs.mono_v, pa_to_h, d.p.idyCv, d.slope_max},
+                                     (FaceRecF<F>*)d.fs.v, d.fs.mono_v, pa_to_h, d.p.idyCv, d.slope_max,
+                                     d.fs.filled ? 1 : 0},
                         TileMap{}};
         unsigned nb;
         it.tm = make_tiles(nx, ny - 1, d.tiles, &nb);

@@ -196,8 +196,11 @@ inline uint32_t face_fb(int) { return 1u; }
 // Flag bits per face (written by the sweep): Ko of the contributing
 // sublayers (hEff != 0) is non-decreasing in s on the left (bit 0) / right
 // (bit 1) column; bit 2: the face has a contributing sublayer, so the flux
-// kernels skip faces without one (dry faces, whose records are inert).
-constexpr uint8_t kMonoL = 1, kMonoR = 2, kHasFlux = 4;
+// kernels skip faces without one (dry faces, whose records are inert);
+// bit 3: the face's records hold the inert values of a dry face (written by
+// the initial fill or by the last sweep), so a sweep that finds the face dry
+// again need not write them.
+constexpr uint8_t kMonoL = 1, kMonoR = 2, kHasFlux = 4, kInert = 8;
 
 // Thread -> cell/face mapping over TX x TY tiles of a 2-D index space, with
 // tiles handed to the 8 XCDs in contiguous bands: workgroups are dispatched
@@ -360,6 +363,8 @@ struct SweepArgs {
   double pa_to_h;          // 1 / (g rho0): Pa -> m (ocean_redi.F90:688-691)
   const double* inv_dx;    // 1 / face spacing per face (idxCu / idyCv): the slope taper's
   double slope_max;        // opt-in slope taper: 0 = off (the reference has none)
+  int keep_inert;          // 1: records of dry faces flagged kInert are left as they are
+                           // (0 where the face array is not this domain's own since its fill)
 };
 
 // Opt-in neutral-slope taper (no counterpart in the reference): sublayer
@@ -408,6 +413,8 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   const uint32_t fo = inr ? (uint32_t)i + (uint32_t)a.nfa * (uint32_t)j : 0u;
   const int nk = a.nz, ns = 2 * nk + 2;
   const bool wet = inr && a.wet[fo] > 0.0;  // dry face: the inert values of :722-730
+  // a dry face whose records already hold them writes nothing
+  const bool keep = inr && !wet && a.keep_inert && (a.mono[fo] & kInert);
   const SurfVals inert{0.0, 0.0, 0.0, 1, 1};
   auto put = [&](int s, const SurfVals& v) {
     const Face r = Face::put(v);
@@ -424,7 +431,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
     for (int it = 0; it < 64 / (64 / kSweepRunWords); ++it) {
       const int q = it * (64 / kSweepRunWords) + lane / kSweepRunWords;
       const uint32_t foq = (uint32_t)__shfl((int)fo, q, 64);
-      const int okq = __shfl(inr ? 1 : 0, q, 64);
+      const int okq = __shfl(inr && !keep ? 1 : 0, q, 64);
       if (okq && sq < cnt) {
         uint4* dst = reinterpret_cast<uint4*>(a.out + (size_t)foq * a.fa + (size_t)(s0 + sq) * a.fb);
         dst[wd % kW] = stw[q * kSweepRunWords + wd];
@@ -566,7 +573,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   flush(ns - 1 - (ns - 1) % kRun, (ns - 1) % kRun + 1);
   if (inr)
     a.mono[fo] = wet ? (uint8_t)((mono_l ? kMonoL : 0) | (mono_r ? kMonoR : 0) | (has_flux ? kHasFlux : 0))
-                     : (uint8_t)(kMonoL | kMonoR);
+                     : (uint8_t)(kMonoL | kMonoR | kInert);
 }
 
 // Dynamic LDS: kSweepStageBytes per wave of the block.
@@ -1292,7 +1299,7 @@ __global__ void k_fill_faces(size_t nrec, FaceRecF<F>* __restrict__ rec, size_t 
                              uint8_t* __restrict__ mono) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < nrec) rec[t] = FaceRecF<F>::put(SurfVals{0.0, 0.0, 0.0, 1, 1});
-  if (t < nface) mono[t] = kMonoL | kMonoR;
+  if (t < nface) mono[t] = kMonoL | kMonoR | kInert;
 }
 
 }  // namespace redi

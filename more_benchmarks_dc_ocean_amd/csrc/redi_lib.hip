@@ -150,6 +150,7 @@ struct FaceState {
   double* khtr_v = nullptr;
   size_t nu = 0, nv = 0;    // faces
   size_t nup = 0, nvp = 0;  // faces per surface row
+  bool filled = false;      // filled for this domain (fill_face_state), so kInert flags hold
 };
 
 size_t face_state_bytes(int nx, int ny, int nz, int form) {
@@ -183,17 +184,18 @@ void free_face_state(FaceState* fs, bool own_khtr) {
 }
 
 template <class F>
-int fill_face_state(const FaceState& fs, int nz, hipStream_t st) {
+int fill_face_state(FaceState& fs, int nz, hipStream_t st) {
   const size_t ns = 2 * (size_t)nz + 2;
   k_fill_faces<F><<<blocks_for(fs.nup * ns, 256), 256, 0, st>>>(fs.nup * ns, (FaceRecF<F>*)fs.u,
                                                                  fs.nu, fs.mono_u);
   k_fill_faces<F><<<blocks_for(fs.nvp * ns, 256), 256, 0, st>>>(fs.nvp * ns, (FaceRecF<F>*)fs.v,
                                                                  fs.nv, fs.mono_v);
   HIP_TRY(hipGetLastError());
+  fs.filled = true;
   return REDI_OK;
 }
 
-int fill_face_state_form(const FaceState& fs, int nz, int form, hipStream_t st) {
+int fill_face_state_form(FaceState& fs, int nz, int form, hipStream_t st) {
   return form == REDI_FORM_FP32_STORE ? fill_face_state<Fp32>(fs, nz, st)
                                       : fill_face_state<Fp64>(fs, nz, st);
 }
@@ -243,6 +245,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
                     (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
     SweepArgs<F> av{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec,
                     (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
+    au.keep_inert = av.keep_inert = fs.filled ? 1 : 0;
     unsigned nbu, nbv;
     TileMap mu = make_tiles(nx - 1, ny, tiles, &nbu);
     TileMap mv = make_tiles(nx, ny - 1, tiles, &nbv);
@@ -258,6 +261,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   if (nx > 1) {  // u faces i = 2..nx, j = 1..ny (1-based)
     SweepArgs<F> a{nx, nz, nx + 1, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 1, 0, 1, 0, wet_u, rec,
                    (FaceRecF<F>*)fs.u, fs.mono_u, pa_to_h, tp.inv_u, tp.slope_max};
+    a.keep_inert = fs.filled ? 1 : 0;
     unsigned nb;
     const TileMap m = make_tiles(nx - 1, ny, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_U, st);
@@ -267,6 +271,7 @@ int launch_phase_a(int nx, int ny, int nz, const EosParams& eos, const typename 
   if (ny > 1) {  // v faces j = 2..ny, i = 1..nx
     SweepArgs<F> a{nx, nz, nx, face_fa(2 * nz + 2), face_fb(2 * nz + 2), 0, 1, 0, 1, wet_v, rec,
                    (FaceRecF<F>*)fs.v, fs.mono_v, pa_to_h, tp.inv_v, tp.slope_max};
+    a.keep_inert = fs.filled ? 1 : 0;
     unsigned nb;
     const TileMap m = make_tiles(nx, ny - 1, tiles, &nb);
     slot = tm.begin(REDI_K_SWEEP_V, st);
