@@ -38,6 +38,14 @@ __device__ __forceinline__ bool sgn_prod_le0(R a, R b) {
 template <class R>
 __device__ __forceinline__ R fmax_sel(R a, R b) { return a > b ? a : b; }
 
+// 1 / x: the IEEE quotient in FP64 (the plain form rounds as the
+// reference); in FP32 the hardware reciprocal (v_rcp_f32, within 1 ulp),
+// which the FP32-store form's reconstruction weights use.
+template <class R>
+__device__ __forceinline__ R recip(R x) { return R(1) / x; }
+template <>
+__device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // The PLM slope and the PPM edge are linear in the tracer values with
 // weights that depend on the layer thicknesses alone.  Every tracer sharing a
 // thickness column shares those weights, so they are formed once per layer
@@ -56,11 +64,11 @@ struct PlmW {
 template <class R>
 __device__ __forceinline__ PlmW<R> plm_weights(R hkm1, R hk, R hkp1) {
   R h_sum = (hkm1 + hkp1) + hk;
-  if (h_sum != R(0)) h_sum = R(1) / h_sum;
+  if (h_sum != R(0)) h_sum = recip<R>(h_sum);
   R hm = hkm1 + hk;
-  if (hm != R(0)) hm = R(1) / hm;
+  if (hm != R(0)) hm = recip<R>(hm);
   R hp = hkp1 + hk;
-  if (hp != R(0)) hp = R(1) / hp;
+  if (hp != R(0)) hp = recip<R>(hp);
   return PlmW<R>{(R(2) * hkm1 + hk) * hp, (R(2) * hkp1 + hk) * hm, hk * h_sum,
                  !((hkp1 + hk) * (hkm1 + hk) > R(0))};
 }
@@ -107,12 +115,12 @@ __device__ __forceinline__ PpmW<R> ppm_weights(R hkm1, R hk, R hkp1, R hkp2) {
   PpmW<R> w;
   R r = hk + hkp1;
   w.mean = r <= R(0);
-  r = R(1) / r;
+  r = recip<R>(r);
   w.lower = hk < hkp1;
   w.we = w.lower ? hk * r : hkp1 * r;
-  const R r2a = R(1) / ((R(2) * hk + hkp1) + R(kHNeglect));
-  const R r2b = R(1) / ((hk + R(2) * hkp1) + R(kHNeglect));
-  w.f1 = R(1) / ((hk + hkp1) + (hkm1 + hkp2));
+  const R r2a = recip<R>((R(2) * hk + hkp1) + R(kHNeglect));
+  const R r2b = recip<R>((hk + R(2) * hkp1) + R(kHNeglect));
+  w.f1 = recip<R>((hk + hkp1) + (hkm1 + hkp2));
   w.f2 = ((R(2) * (hkp1 * hk)) * r) * (((hkm1 + hk) * r2a) - ((hkp2 + hkp1) * r2b));
   w.f3 = (hk * (hkm1 + hk)) * r2a;
   w.f4 = (hkp1 * (hkp1 + hkp2)) * r2b;
@@ -174,6 +182,54 @@ __device__ __forceinline__ void stream_edges(int nz, Load load, Emit emit) {
   // after the loop sw0 holds layer nz-1 for every nz >= 1
 #pragma unroll
   for (int q = 0; q < NS; ++q) e[q] = sw0[q] + R(0.5) * R(0);  // diff(nk) = 0
+  emit(nz, e, sw0);
+}
+
+// stream_edges for the FP32-store form: FP64 layer values (`load` returns
+// them), PLM slopes and PPM edges computed in FP32 on differences about the
+// FP64 layer means -- a slope about its own layer's mean, an edge about the
+// mean of the layer above it -- and each edge returned as that mean plus
+// its FP32 offset.  Shifting a column's values by a constant shifts its
+// edges by the same constant and leaves its slopes unchanged, so this is the
+// FP32 evaluation of the same formulas (as in the tiled reconstruction).
+template <int NS, class Load, class Emit>
+__device__ __forceinline__ void stream_edges_off(int nz, Load load, Emit emit) {
+  double h0, h1, h2, s0[NS], s1[NS], s2[NS];
+  load(0, h0, s0);
+  load(nz > 1 ? 1 : 0, h1, s1);
+  load(nz > 2 ? 2 : (nz - 1), h2, s2);
+  float hw0 = (float)h0, hw1 = (float)h0, hw2 = (float)h1, hw3 = (float)h2;
+  double sw0[NS], sw1[NS], sw2[NS], e[NS];
+  float dprev[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    sw0[q] = s0[q]; sw1[q] = s1[q]; sw2[q] = s2[q]; dprev[q] = 0.f;
+    e[q] = s0[q];
+  }
+  emit(0, e, sw0);
+  for (int k = 1; k < nz; ++k) {
+    float dcur[NS];
+    const PlmW<float> pw = plm_weights<float>(hw1, hw2, hw3);
+    const PpmW<float> ew = ppm_weights<float>(hw0, hw1, hw2, hw3);
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      dcur[q] = (k <= nz - 2)
+                    ? plm_slope_w<float>(pw, (float)(sw0[q] - sw1[q]), 0.f, (float)(sw2[q] - sw1[q]))
+                    : 0.f;
+      e[q] = sw0[q] + (double)ppm_edge_w<float>(ew, 0.f, (float)(sw1[q] - sw0[q]), dprev[q], dcur[q]);
+    }
+    emit(k, e, sw0);
+    double hn, sn[NS];
+    const int kn = (k + 2 < nz) ? (k + 2) : (nz - 1);
+    load(kn, hn, sn);
+    hw0 = hw1; hw1 = hw2; hw2 = hw3; hw3 = (float)hn;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      sw0[q] = sw1[q]; sw1[q] = sw2[q]; sw2[q] = sn[q]; dprev[q] = dcur[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NS; ++q) e[q] = sw0[q];
   emit(nz, e, sw0);
 }
 

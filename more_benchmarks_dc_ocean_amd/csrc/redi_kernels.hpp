@@ -282,12 +282,19 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
   // interface k's record is written once P(k+1) is known, at interface k+1
   C Tp = 0.0, Sp = 0.0;
   E dTp = E(0), dSp = E(0);
+  constexpr bool kFp32 = std::is_same<F, Fp32>::value;
   auto load = [&](int kt, C& hh, C* s) {
     const size_t o = (size_t)(nz - 1 - kt) * n2 + c;  // top-down -> native
     hh = (C)h[o];
     const C he = hh > C(kHDivEps) ? hh : C(kHDivEps);
-    s[0] = (C)thtr[o] / he;
-    s[1] = (C)shtr[o] / he;
+    if constexpr (kFp32) {  // one FP64 reciprocal per layer
+      const C ih = C(1) / he;
+      s[0] = (C)thtr[o] * ih;
+      s[1] = (C)shtr[o] * ih;
+    } else {
+      s[0] = (C)thtr[o] / he;
+      s[1] = (C)shtr[o] / he;
+    }
   };
   auto emit = [&](int k, const C* e, const C*) {
     if (k > 0) P = P + g_rho0 * (double)h[(size_t)(nz - k) * n2 + c];  // top-down layer k-1
@@ -296,7 +303,10 @@ __device__ __forceinline__ void column_state_body(const ColumnArgs<F>& a, int bi
     if (k > 0) r[(size_t)(k - 1) * cs] = typename F::Col{Tp, Sp, P, dTp, dSp};
     Tp = e[0]; Sp = e[1]; dTp = dRdT; dSp = dRdS;
   };
-  stream_edges<C, 2>(nz, load, emit);
+  if constexpr (kFp32)  // slopes and edges in FP32 about the FP64 layer means
+    stream_edges_off<2>(nz, load, emit);
+  else
+    stream_edges<C, 2>(nz, load, emit);
   r[(size_t)nz * cs] = typename F::Col{Tp, Sp, P, dTp, dSp};  // Pn = P(nz)
 }
 
@@ -722,8 +732,15 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
     const R hh = (R)h[o];
     const R he = hh > R(kHDivEps) ? hh : R(kHDivEps);
     sh[r][cc] = hh;
+    if constexpr (std::is_same<typename F::Recon, ReconRecOff>::value) {
+      // FP32-store form: FP64 means from one FP64 reciprocal per layer
+      const R ih = R(1) / he;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)a.tr.q[t][o] / he;
+      for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)a.tr.q[t][o] * ih;
+    } else {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) sv[t][r][cc] = (R)a.tr.q[t][o] / he;
+    }
   }
   __syncthreads();
   // The FP32-store form (ReconRecOff records) runs stages 2-4 in FP32 on

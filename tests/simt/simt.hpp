@@ -168,6 +168,8 @@ void run_grid(dim3 grid, dim3 block, size_t dyn_lds, const std::function<void()>
 #define __builtin_amdgcn_readfirstlane(v) ::simt::readfirstlane((v), __LINE__)
 #define __builtin_amdgcn_mbcnt_lo(m, a) ::simt::mbcnt_lo((m), (a))
 #define __builtin_amdgcn_mbcnt_hi(m, a) ::simt::mbcnt_hi((m), (a))
+// the hardware reciprocal (within 1 ulp) as the quotient
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 #define __popcll(x) __builtin_popcountll(x)
 // conversion rounding to nearest, ties to even (the default rounding mode)
 inline unsigned __double2uint_rn(double x) { return (unsigned)std::nearbyint(x); }
