@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) kb_face_khtr(BatchView<KhtrArgs> v) {
 
 template <class F, int NT>
 __global__ void __launch_bounds__(kReconLanes) kb_tracer_recon(BatchView<ReconItem<F, NT>> v) {
-  __shared__ double lds[recon_lds_max<NT>()];
+  __shared__ double lds[recon_lds_max<F, NT>()];
   for_each_block<true>(v, [&](const ReconItem<F, NT>& it, int bid, int nb) {
     switch (it.kt) {
       case 8: tracer_recon_tiled_body<F, NT, 8>(it.a, bid, nb, lds); break;

@@ -404,7 +404,11 @@ __device__ __forceinline__ double slope_taper(double hm, double topL, double bot
 // transposed -- 8 lanes per face, 8 faces per store instruction -- so every
 // store fills whole 128-byte runs of records.
 constexpr int kSweepRunWords = 8;                              // 16-byte words per face per run
-constexpr int kSweepStageWords = 64 * kSweepRunWords;          // per wave
+// A lane's run sits kSweepLaneWords apart from the next lane's: one word of
+// padding spreads the lanes' 16-byte stage writes over the LDS banks (with
+// runs 128 bytes apart, every lane's write of a step lands in the same banks).
+constexpr int kSweepLaneWords = kSweepRunWords + 1;
+constexpr int kSweepStageWords = 64 * kSweepLaneWords;         // per wave
 constexpr size_t kSweepStageBytes = kSweepStageWords * sizeof(uint4);
 
 // `tid`: the lane's index within the tile; `stage`: the workgroup's LDS
@@ -431,7 +435,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
     uint4 w[kW];
     __builtin_memcpy(w, &r, sizeof(Face));
 #pragma unroll
-    for (int q = 0; q < kW; ++q) stw[lane * kSweepRunWords + (s % kRun) * kW + q] = w[q];
+    for (int q = 0; q < kW; ++q) stw[lane * kSweepLaneWords + (s % kRun) * kW + q] = w[q];
   };
   // surfaces s0 .. s0+cnt-1 of every face of the wave
   auto flush = [&](int s0, int cnt) {
@@ -444,7 +448,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
       const int okq = __shfl(inr && !keep ? 1 : 0, q, 64);
       if (okq && sq < cnt) {
         uint4* dst = reinterpret_cast<uint4*>(a.out + (size_t)foq * a.fa + (size_t)(s0 + sq) * a.fb);
-        dst[wd % kW] = stw[q * kSweepRunWords + wd];
+        dst[wd % kW] = stw[q * kSweepLaneWords + wd];
       }
     }
     __syncthreads();
@@ -691,16 +695,27 @@ constexpr int kReconLanes = 256;
 constexpr int kReconChunkDefault = 16;
 
 
-// LDS doubles of one tiled-reconstruction workgroup
-template <int NT, int KT>
-constexpr int recon_lds_doubles() {
-  constexpr int CT = kReconLanes / KT;
-  return (KT + 4) * CT * (1 + NT) + NT * (KT + 2) * CT + NT * (KT + 1) * CT;
+// Row stride (doubles) of the tiled reconstruction's LDS arrays of CT
+// columns.  The record stage reads them with lanes along layers, four rows
+// per tracer in the FP32-store form, where a power-of-two stride puts every
+// lane in the same banks: that form pads each row by one double.  The FP64
+// form reads two rows per tracer and keeps the unpadded rows, whose smaller
+// footprint fits one more workgroup per CU.
+template <class F, int CT>
+constexpr int recon_row() {
+  return std::is_same<typename F::Recon, ReconRecOff>::value ? CT + 1 : CT;
 }
-template <int NT>
+
+// LDS doubles of one tiled-reconstruction workgroup
+template <class F, int NT, int KT>
+constexpr int recon_lds_doubles() {
+  constexpr int CTP = recon_row<F, kReconLanes / KT>();
+  return (KT + 4) * CTP * (1 + NT) + NT * (KT + 2) * CTP + NT * (KT + 1) * CTP;
+}
+template <class F, int NT>
 constexpr int recon_lds_max() {
-  constexpr int a = recon_lds_doubles<NT, 8>(), b = recon_lds_doubles<NT, 16>(),
-                c = recon_lds_doubles<NT, 32>();
+  constexpr int a = recon_lds_doubles<F, NT, 8>(), b = recon_lds_doubles<F, NT, 16>(),
+                c = recon_lds_doubles<F, NT, 32>();
   return a > b ? (a > c ? a : c) : (b > c ? b : c);
 }
 
@@ -709,12 +724,13 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
                                                         double* lds) {
   using R = double;
   constexpr int CT = kReconLanes / KT, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
+  constexpr int CTP = recon_row<F, CT>();  // row stride (doubles)
   // h, layers k0-2 .. k0+KT+1 (clamped); layer values hTr / max(h, eps), same
   // layers; PLM slopes, layers k0-1 .. k0+KT; PPM edges, interfaces k0 .. k0+KT
-  R(*sh)[CT] = reinterpret_cast<R(*)[CT]>(lds);
-  R(*sv)[NL][CT] = reinterpret_cast<R(*)[NL][CT]>(lds + NL * CT);
-  R(*sd)[NSL][CT] = reinterpret_cast<R(*)[NSL][CT]>(lds + NL * CT * (1 + NT));
-  R(*se)[NE][CT] = reinterpret_cast<R(*)[NE][CT]>(lds + NL * CT * (1 + NT) + NT * NSL * CT);
+  R(*sh)[CTP] = reinterpret_cast<R(*)[CTP]>(lds);
+  R(*sv)[NL][CTP] = reinterpret_cast<R(*)[NL][CTP]>(lds + NL * CTP);
+  R(*sd)[NSL][CTP] = reinterpret_cast<R(*)[NSL][CTP]>(lds + NL * CTP * (1 + NT));
+  R(*se)[NE][CTP] = reinterpret_cast<R(*)[NE][CTP]>(lds + NL * CTP * (1 + NT) + NT * NSL * CTP);
   const int N2 = a.N2, nz = a.nz;
   const typename F::R* __restrict__ h = a.h;
   const int nkb = (nz + KT - 1) / KT;
@@ -821,7 +837,7 @@ __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& 
 
 template <class F, int NT, int KT>
 __global__ void __launch_bounds__(kReconLanes) k_tracer_recon_tiled(ReconArgs<F, NT> a) {
-  __shared__ double lds[recon_lds_doubles<NT, KT>()];
+  __shared__ double lds[recon_lds_doubles<F, NT, KT>()];
   tracer_recon_tiled_body<F, NT, KT>(a, (int)blockIdx.x, (int)gridDim.x, lds);
 }
 
