@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace redi {
 
 enum EosVariant : int { kEosLinear = 1, kEosWright97 = 2, kEosRoquetSpV = 4 };
@@ -143,114 +145,29 @@ __device__ __forceinline__ float rsqrt_exact<float>(float x) { return sqrtf(x); 
 
 // roquet_spv_point (redi/ocean_eos.F90:282-448): specific volume and its
 // derivatives w.r.t. the model variables (potential temperature, practical
-// salinity) at pressure p (Pa).
+// salinity) at pressure p (Pa).  The body (redi_eos_roquet.inc) is compiled
+// twice: as written and without contraction for the plain FP64 form, which
+// must round as the reference does; and for the FP32-store form with its
+// multiply-adds contracted into FMAs and its divisions by constants taken as
+// products with the constants' reciprocals.
+template <class R>
+__device__ __forceinline__ void roquet_spv_exact(R T_pt, R S_sp, R p, R& sv, R& dsv_dt, R& dsv_ds) {
+  const auto cdiv = [](R x, double c) { return x / R(c); };
+#include "redi_eos_roquet.inc"
+}
+template <class R>
+__device__ __forceinline__ void roquet_spv_fused(R T_pt, R S_sp, R p, R& sv, R& dsv_dt, R& dsv_ds) {
+#pragma clang fp contract(fast)
+  const auto cdiv = [](R x, double c) { return x * R(1.0 / c); };
+#include "redi_eos_roquet.inc"
+}
 template <class R>
 __device__ __forceinline__ void roquet_spv_point(R T_pt, R S_sp, R p, R& sv, R& dsv_dt,
                                                  R& dsv_ds) {
-  using namespace eosc;
-  const R SR = S_sp * R(kSrFactor);
-  const R cx = R(kCtSfac) * SR;
-  const R x2 = cx > R(1.0e-20) ? cx : R(1.0e-20);
-  const R xx = rsqrt_exact<R>(x2);
-  const R yy = T_pt * R(0.025);
-  const R c0 = R(61.01362420681071) +
-               x2 * (R(268.5520265845071) +
-                     xx * (R(937.2099110620707) +
-                           xx * (R(-1687.914374187449) + xx * R(246.9598888781377))));
-  const R c1 = R(168776.46138048015) +
-               x2 * (R(-12019.028203559312) +
-                     xx * (R(588.1802812170108) +
-                           xx * (R(936.3206544460336) + xx * R(123.59576582457964))));
-  const R c2 = R(-2735.2785605119625) +
-               x2 * (R(3734.858026725145) +
-                     xx * (R(248.39476522971285) +
-                           xx * (R(-942.7827304544439) + xx * R(-48.5891069025409))));
-  const R c3 = R(2574.2164453821433) +
-               x2 * (R(-2046.7671145057618) +
-                     xx * (R(-3.871557904936333) + xx * R(369.4389437509002)));
-  const R c4 = R(-1536.6644434977543) +
-               x2 * (R(465.28655623126450) +
-                     xx * (R(-2.6268019854268356) + xx * R(-33.83664947895248)));
-  const R c5 = R(545.7340497931629) + x2 * (R(-0.6370820302831379) + xx * R(-9.987880382780322));
-  const R c6 = R(-50.91091728474331) + x2 * R(-10.650848542359153);
-  const R c7 = R(-18.30489878927802);
-  const R d0 = xx * (R(2.0 * 268.5520265845071) +
-                     xx * (R(3.0 * 937.2099110620707) +
-                           xx * (R(4.0 * (-1687.914374187449)) + (xx * R(5.0)) * R(246.9598888781377))));
-  const R d1 = xx * (R(2.0 * (-12019.028203559312)) +
-                     xx * (R(3.0 * 588.1802812170108) +
-                           xx * (R(4.0 * 936.3206544460336) + (xx * R(5.0)) * R(123.59576582457964))));
-  const R d2 = xx * (R(2.0 * 3734.858026725145) +
-                     xx * (R(3.0 * 248.39476522971285) +
-                           xx * (R(4.0 * (-942.7827304544439)) + (xx * R(5.0)) * R(-48.5891069025409))));
-  const R d3 = xx * (R(2.0 * (-2046.7671145057618)) +
-                     xx * (R(3.0 * (-3.871557904936333)) + (xx * R(4.0)) * R(369.4389437509002)));
-  const R d4 = xx * (R(2.0 * 465.28655623126450) +
-                     xx * (R(3.0 * (-2.6268019854268356)) + (xx * R(4.0)) * R(-33.83664947895248)));
-  const R d5 = xx * (R(2.0 * (-0.6370820302831379)) + (xx * R(3.0)) * R(-9.987880382780322));
-  const R d6 = (xx * R(2.0)) * R(-10.650848542359153);
-  const R d7 = R(0.0);
-  const R hh = c0 + yy * (c1 + yy * (c2 + yy * (c3 + yy * (c4 + yy * (c5 + yy * (c6 + yy * c7))))));
-  const R dh_dy =
-      c1 + yy * ((R(2.0) * c2) +
-                 yy * ((R(3.0) * c3) +
-                       yy * ((R(4.0) * c4) +
-                             yy * ((R(5.0) * c5) + yy * ((R(6.0) * c6) + (yy * R(7.0)) * c7)))));
-  const R dh_dx = d0 + yy * (d1 + yy * (d2 + yy * (d3 + yy * (d4 + yy * (d5 + yy * (d6 + yy * d7))))));
-  const R zt = hh / R(kCp0);
-  const R dct_dpt = (R(0.025) * dh_dy) / R(kCp0);
-  const R dct_dsr = (dh_dx * (R(kCtSfac) / (R(2.0) * xx))) / R(kCp0);
-  const R zs = rsqrt_exact<R>(fabs(S_sp * R(kSrFactor) + R(kRdeltaS)) * R(kR1S0));
-  const R zp = p;
-
-  const R ts3 = R(S003) + (zs * R(S103) + zt * R(S013));
-  const R ts2 = R(S002) + (zs * (R(S102) + zs * R(S202)) + zt * (R(S012) + (zs * R(S112) + zt * R(S022))));
-  const R ts1 =
-      R(S001) + (zs * (R(S101) + zs * (R(S201) + zs * (R(S301) + zs * R(S401)))) +
-                 zt * (R(S011) + (zs * (R(S111) + zs * (R(S211) + zs * R(S311))) +
-                                  zt * (R(S021) + (zs * (R(S121) + zs * R(S221)) +
-                                                   zt * (R(S031) + (zs * R(S131) + zt * R(S041))))))));
-  const R ts0 =
-      zt * (R(S010) +
-            (zs * (R(S110) + zs * (R(S210) + zs * (R(S310) + zs * (R(S410) + zs * R(S510))))) +
-             zt * (R(S020) +
-                   (zs * (R(S120) + zs * (R(S220) + zs * (R(S320) + zs * R(S420)))) +
-                    zt * (R(S030) +
-                          (zs * (R(S130) + zs * (R(S230) + zs * R(S330))) +
-                           zt * (R(S040) + (zs * (R(S140) + zs * R(S240)) +
-                                            zt * (R(S050) + (zs * R(S150) + zt * R(S060)))))))))));
-  const R s0s0 = R(S000) + zs * (R(S100) + zs * (R(S200) + zs * (R(S300) + zs * (R(S400) + zs * (R(S500) + zs * R(S600))))));
-  const R s00p = zp * (R(V00) + zp * (R(V01) + zp * (R(V02) + zp * (R(V03) + zp * (R(V04) + zp * R(V05))))));
-  sv = ((ts0 + s0s0) + zp * (ts1 + zp * (ts2 + zp * ts3))) + s00p;
-
-  const R t3 = R(A003);
-  const R t2 = R(A002) + (zs * R(A102) + zt * R(A012));
-  const R t1 = R(A001) + (zs * (R(A101) + zs * (R(A201) + zs * R(A301))) +
-                          zt * (R(A011) + (zs * (R(A111) + zs * R(A211)) +
-                                           zt * (R(A021) + (zs * R(A121) + zt * R(A031))))));
-  const R t0 =
-      R(A000) + (zs * (R(A100) + zs * (R(A200) + zs * (R(A300) + zs * (R(A400) + zs * R(A500))))) +
-                 zt * (R(A010) + (zs * (R(A110) + zs * (R(A210) + zs * (R(A310) + zs * R(A410)))) +
-                                  zt * (R(A020) + (zs * (R(A120) + zs * (R(A220) + zs * R(A320))) +
-                                                   zt * (R(A030) + (zt * (R(A040) + (zs * R(A140) + zt * R(A050))) +
-                                                                    zs * (R(A130) + zs * R(A230)))))))));
-  const R dsv_dct = t0 + zp * (t1 + zp * (t2 + zp * t3));
-
-  const R u3 = R(B003);
-  const R u2 = R(B002) + (zs * R(B102) + zt * R(B012));
-  const R u1 = R(B001) + (zs * (R(B101) + zs * (R(B201) + zs * R(B301))) +
-                          zt * (R(B011) + (zs * (R(B111) + zs * R(B211)) +
-                                           zt * (R(B021) + (zs * R(B121) + zt * R(B031))))));
-  const R u0 =
-      R(B000) + (zs * (R(B100) + zs * (R(B200) + zs * (R(B300) + zs * (R(B400) + zs * R(B500))))) +
-                 zt * (R(B010) + (zs * (R(B110) + zs * (R(B210) + zs * (R(B310) + zs * R(B410)))) +
-                                  zt * (R(B020) + (zs * (R(B120) + zs * (R(B220) + zs * R(B320))) +
-                                                   zt * (R(B030) + (zt * (R(B040) + (zs * R(B140) + zt * R(B050))) +
-                                                                    zs * (R(B130) + zs * R(B230)))))))));
-  const R dsv_dsa = (u0 + zp * (u1 + zp * (u2 + zp * u3))) / zs;
-
-  dsv_dt = dsv_dct * dct_dpt;
-  dsv_ds = (dsv_dsa + dsv_dct * dct_dsr) * R(kSrFactor);
+  if constexpr (std::is_same<R, float>::value)
+    roquet_spv_fused<R>(T_pt, S_sp, p, sv, dsv_dt, dsv_ds);
+  else
+    roquet_spv_exact<R>(T_pt, S_sp, p, sv, dsv_dt, dsv_ds);
 }
 
 // Interface density and its T/S sensitivities in the form the neutral sweep

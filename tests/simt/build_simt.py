@@ -36,7 +36,7 @@ CSRC = os.path.join(ROOT, "more_benchmarks_dc_ocean_amd", "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB_SOURCES = ("redi_lib.hip", "redi_kernels.hpp", "redi_batch_host.hpp",
-               "redi_batch_kernels.hpp", "redi_column.hpp", "redi_eos.hpp")
+               "redi_batch_kernels.hpp", "redi_column.hpp", "redi_eos.hpp", "redi_eos_roquet.inc")
 # asan: g++, AddressSanitizer + UndefinedBehaviorSanitizer.  msan: clang++
 # (ROCm's host compiler), MemorySanitizer: a branch, address or output that
 # depends on an uninitialized value -- which a GPU register would hold as
@@ -95,7 +95,8 @@ def prepare(variant: str, dst: str) -> int:
 
 def flags(opt: int, sanitizer: str = "asan") -> list:
     return ["-std=c++17", f"-O{opt}", "-g", "-fno-omit-frame-pointer", "-ffp-contract=off",
-            "-fno-fast-math", "-Wno-attributes", "-Wno-unknown-attributes", "-U_FORTIFY_SOURCE",
+            "-fno-fast-math", "-Wno-attributes", "-Wno-unknown-attributes", "-Wno-unknown-pragmas",
+            "-U_FORTIFY_SOURCE",
             *SANITIZERS[sanitizer][1]]
 
 
