@@ -934,7 +934,7 @@ __device__ __forceinline__ Global<T>* global_ptr(T* p) {
 // (cell, K, t) entry at cell*nz*NT + (K-1)*NT + t), one contiguous block per
 // cell, so each lane's loads and stores are coalesced.
 constexpr int kChunk = 63;      // sublayers per pass of a wavefront (64 surfaces)
-constexpr int kFluxSegDefault = 32;  // cells per run
+constexpr int kFluxSegDefault = 16;  // cells per run
 
 // Layer-capacity classes: a domain of nz layers runs the kernels of the
 // smallest class with nzc >= nz (the LDS lists and the register
