@@ -2,10 +2,12 @@
 //
 //   k_column_state        Phase A, once per column: top-down interface
 //                         pressure, PPM T/S edges and dR/dT, dR/dS (EOS).
-//   k_neutral_sweep       Phase A, once per wet face: the continuous neutral
-//                         surface sweep over the two adjacent columns.
-//   k_tracer_recon_tiled  Phase B, per tracer group: layer averages, PPM edges
-//                         and the PPM limiter branch of every column.
+//   k_neutral_sweep_uv    Phase A, once per wet face of both grids (a u wave
+//                         and a v wave per workgroup; k_neutral_sweep for
+//                         one grid): the continuous neutral surface sweep
+//                         over the two adjacent columns.
+//   k_tracer_recon_tiled  Phase B, per tracer group: layer averages and PPM
+//                         interface values of every column.
 //   k_flux (x, y)         Phase B, per tracer group and face grid: the flux of
 //                         every contributing sublayer of every face, added
 //                         into the bounding cells' dTr in the reference's
@@ -18,9 +20,11 @@
 //         reference's rounding order, so results match the reference bits;
 //   Fp32  the FP32-store form: the fields h / hTr and the face records are
 //         stored in 32-bit words (FP32 values, int16 layer indices, positions
-//         as 31-bit fractions) and each sublayer flux is rounded to FP32; the
-//         column state, the reconstruction, the density differences and the
-//         tendency accumulation stay FP64.
+//         as 31-bit fractions); the EOS, the PPM slopes and edges and the
+//         sublayer fluxes are computed in FP32; FP64 stays in the layer
+//         means and T/S interface values (the slopes and edges in FP32
+//         about the means), the pressures and positions, the density and
+//         guard differences, and the tendency accumulation.
 // Both forms come from one source: records convert through get() / put(),
 // which are identities for Fp64.
 //
@@ -34,11 +38,12 @@
 //   face coefficients (K2->K4)  FaceRec per face and neutral surface s,
 //                               face-major: (face, s) at face*ns + s; one flag
 //                               byte per face
-//   tracer recon (K3 -> K4)     ReconRec (layer mean, interface values at the
-//                               layer's top and bottom, limiter branch; FP64
-//                               in both forms) per layer, column and tracer
-//                               of a group; each tracer's column contiguous
-//                               by default
+//   tracer recon (K3 -> K4)     per column and tracer of a group, each
+//                               tracer's column contiguous by default: FP64
+//                               (layer mean, bottom interface value) pairs,
+//                               nz+1 per column with the top interface value
+//                               first; FP32-store: per layer the FP64 mean and
+//                               its top and bottom values as FP32 offsets
 //   dTr (K4 -> K5)              FP64, one block of nz*NT values per cell,
 //                               (K, t) at (K-1)*NT + t
 // The sweep reads each of its two columns once, front to back; its lanes
@@ -618,10 +623,10 @@ __global__ void __launch_bounds__(128) k_neutral_sweep_uv(SweepArgs<F> au, TileM
 
 // ---------------------------------------------------------------------------
 // K3: tracer column reconstruction (redi_tracer_column, :852-891), the NT
-// tracers of a group per launch.  ReconRec (k, col, t) at col*ca + k*cb +
-// t*ct (ReconLayout):
-//   per column (ca = NT*nz, cb = 1, ct = nz): each tracer's column is one
-//     run of records, which a face's lanes, walking down the column, read in
+// tracers of a group per launch.  Entry k (0 .. ne-1, ne = nz + the form's
+// kExtra) of column col and tracer t at col*ca + k*cb + t*ct (ReconLayout):
+//   per column (ca = NT*ne, cb = 1, ct = ne): each tracer's column is one
+//     run of entries, which a face's lanes, walking down the column, read in
 //     contiguous stretches;
 //   layer-major (ca = NT, cb = N2*NT, ct = 1): the lanes of the streaming
 //     kernel, adjacent columns, write contiguous rows.
@@ -646,11 +651,10 @@ struct ReconArgs {
   ReconLayout y;
 };
 
-// The reconstruction is computed in FP64 in both forms from the stored
-// fields (the FP32-store form keeps its edge values as FP32 offsets from
-// the FP64 layer means).
-// Streaming form: one lane per column walks it top-down with a register
-// window (stream_edges).
+// Streaming form (the layer-major layout): one lane per column walks it
+// top-down with a register window (stream_edges), in FP64 in both forms
+// (the FP32-store form stores its edge values as FP32 offsets from the FP64
+// layer means).
 template <class F, int NT>
 __global__ void __launch_bounds__(128) k_tracer_recon(ReconArgs<F, NT> a) {
   using C = double;
