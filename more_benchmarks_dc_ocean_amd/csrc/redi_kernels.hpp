@@ -710,11 +710,20 @@ constexpr int recon_row() {
   return std::is_same<typename F::Recon, ReconRecOff>::value ? CT + 1 : CT;
 }
 
-// LDS doubles of one tiled-reconstruction workgroup
+// Element types of the tiled reconstruction's LDS stages: the layer means
+// are FP64 in both forms; the thicknesses, slopes and edge offsets of the
+// FP32-store form are FP32 values and are staged as such.
+template <class F>
+using ReconLdsSmall =
+    std::conditional_t<std::is_same<typename F::Recon, ReconRecOff>::value, float, double>;
+
+// LDS of one tiled-reconstruction workgroup, in doubles: the means, then the
+// thicknesses, slopes and edges
 template <class F, int NT, int KT>
 constexpr int recon_lds_doubles() {
   constexpr int CTP = recon_row<F, kReconLanes / KT>();
-  return (KT + 4) * CTP * (1 + NT) + NT * (KT + 2) * CTP + NT * (KT + 1) * CTP;
+  constexpr int small = (KT + 4) * CTP + NT * (KT + 2) * CTP + NT * (KT + 1) * CTP;
+  return (KT + 4) * CTP * NT + (small * (int)sizeof(ReconLdsSmall<F>) + 7) / 8;
 }
 template <class F, int NT>
 constexpr int recon_lds_max() {
@@ -727,14 +736,16 @@ template <class F, int NT, int KT>
 __device__ __forceinline__ void tracer_recon_tiled_body(const ReconArgs<F, NT>& a, int bid, int nb,
                                                         double* lds) {
   using R = double;
+  using Sm = ReconLdsSmall<F>;  // staged thicknesses, slopes and edges
   constexpr int CT = kReconLanes / KT, NL = KT + 4, NSL = KT + 2, NE = KT + 1;
-  constexpr int CTP = recon_row<F, CT>();  // row stride (doubles)
-  // h, layers k0-2 .. k0+KT+1 (clamped); layer values hTr / max(h, eps), same
+  constexpr int CTP = recon_row<F, CT>();  // row stride (elements)
+  // layer values hTr / max(h, eps), layers k0-2 .. k0+KT+1 (clamped); h, same
   // layers; PLM slopes, layers k0-1 .. k0+KT; PPM edges, interfaces k0 .. k0+KT
-  R(*sh)[CTP] = reinterpret_cast<R(*)[CTP]>(lds);
-  R(*sv)[NL][CTP] = reinterpret_cast<R(*)[NL][CTP]>(lds + NL * CTP);
-  R(*sd)[NSL][CTP] = reinterpret_cast<R(*)[NSL][CTP]>(lds + NL * CTP * (1 + NT));
-  R(*se)[NE][CTP] = reinterpret_cast<R(*)[NE][CTP]>(lds + NL * CTP * (1 + NT) + NT * NSL * CTP);
+  R(*sv)[NL][CTP] = reinterpret_cast<R(*)[NL][CTP]>(lds);
+  Sm* small = reinterpret_cast<Sm*>(lds + NT * NL * CTP);
+  Sm(*sh)[CTP] = reinterpret_cast<Sm(*)[CTP]>(small);
+  Sm(*sd)[NSL][CTP] = reinterpret_cast<Sm(*)[NSL][CTP]>(small + NL * CTP);
+  Sm(*se)[NE][CTP] = reinterpret_cast<Sm(*)[NE][CTP]>(small + NL * CTP + NT * NSL * CTP);
   const int N2 = a.N2, nz = a.nz;
   const typename F::R* __restrict__ h = a.h;
   const int nkb = (nz + KT - 1) / KT;
