@@ -55,12 +55,13 @@ def test_library_loads_and_host_only_calls(libpath):
         assert hasattr(L, n)
     assert L.redi_version() >= 100
     # 10x8x4 domain (80 columns), four tracer slots, every array rounded up
-    # to 256 B: 32-B reconstruction records, the discard field, the FP64 dTr
-    # per layer, column and tracer (Phase A's 40 B per column interface,
-    # 16000 B, is smaller and shares the buffer)
+    # to 256 B: 16-B reconstruction entries (nz + 1 per column and tracer in
+    # the FP64 form), the discard field, the FP64 dTr per layer, column and
+    # tracer (Phase A's 40 B per column interface, 16000 B, is smaller and
+    # shares the buffer)
     up = lambda b: -(-b // 256) * 256
-    parts = [32 * 80 * 4 * 4, 8 * 80 * 4, 8 * 80 * 4 * 4]
-    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 53760
+    parts = [16 * 80 * 5 * 4, 8 * 80 * 4, 8 * 80 * 4 * 4]
+    assert L.redi_workspace_bytes(10, 8, 4) == sum(up(b) for b in parts) == 38400
     assert L.redi_workspace_bytes(0, 8, 4) == 0
     assert L.redi_state_bytes(10, 8, 4) > L.redi_workspace_bytes(10, 8, 4)
     assert isinstance(redi.last_error(), str)
