@@ -990,7 +990,11 @@ struct FluxLds {
 
 // The contributing sublayers of face `fo` between columns cl (left) and cr
 // (right), compacted into L in s order; returns their count (wave-uniform).
-template <class F, int NT, int NZC>
+// kAhead (FP64 form): each tracer's reconstruction entries are gathered
+// while the previous tracer is computed, which costs 16 VGPRs; the u-face
+// kernel has them to spare at four waves per SIMD, the v-face kernel
+// (which also holds the cell's W + E partial) has not.
+template <class F, int NT, int NZC, bool kAhead>
 __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t fo, uint32_t cl,
                                                   uint32_t cr, FluxLds<typename F::C, NT, NZC>& L) {
   using C = typename F::C;
@@ -1019,10 +1023,25 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
     C x[NT];
     // The FP32-store form gathers every tracer's two 16-byte records before
     // computing (one round trip per chunk, eight records in flight); the FP64
-    // form gathers one tracer's 32-byte records at a time, which keeps its
-    // registers, and so its waves per SIMD, at the faster setting.
+    // form gathers one tracer's four 16-byte entries at a time (kAhead: the
+    // next tracer's while this one is computed), which keeps its registers,
+    // and so its waves per SIMD, at the faster setting.
     constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
     ReconRecOff gl[kOff ? NT : 1], gr[kOff ? NT : 1];
+    // FP64: entries K-1 and K of both columns (with kAhead, one tracer ahead)
+    ReconPair64 pla[2], plk[2], pra[2], prk[2];
+    auto gather64 = [&](int t, int b) {
+      if constexpr (!kOff) {
+        pla[b] = plk[b] = pra[b] = prk[b] = ReconPair64{0.0, 0.0};
+        if (surf) {
+          pla[b] = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
+          plk[b] = recL[(uint32_t)KoL * cb + (uint32_t)t * ct];
+          pra[b] = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
+          prk[b] = recR[(uint32_t)KoR * cb + (uint32_t)t * ct];
+        }
+      }
+    };
+    if constexpr (kAhead) gather64(0, 0);
     if constexpr (kOff) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -1063,13 +1082,12 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
       } else {
         // entries K-1 and K of each column (layer K's top value, mean and
         // bottom value); the limited edges only where a flux is formed
-        ReconPair64 la{0.0, 0.0}, lk = la, ra = la, rk = la;
-        if (surf) {
-          la = recL[(uint32_t)(KoL - 1) * cb + (uint32_t)t * ct];
-          lk = recL[(uint32_t)KoL * cb + (uint32_t)t * ct];
-          ra = recR[(uint32_t)(KoR - 1) * cb + (uint32_t)t * ct];
-          rk = recR[(uint32_t)KoR * cb + (uint32_t)t * ct];
+        if constexpr (kAhead) {
+          if (t + 1 < NT) gather64(t + 1, (t + 1) & 1);
+        } else {
+          gather64(t, t & 1);
         }
+        const ReconPair64 la = pla[t & 1], lk = plk[t & 1], ra = pra[t & 1], rk = prk[t & 1];
         const C eL = edge_value<C>(la.Tb, lk.Tb, PoL), eR = edge_value<C>(ra.Tb, rk.Tb, PoR);
         const C eLb = __shfl_down(eL, 1, 64), eRb = __shfl_down(eR, 1, 64);
         if (contrib) {
@@ -1168,7 +1186,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
     int nc = 0;
     bool monoL = true, monoR = true;
     if (live) {
-      nc = face_contributions<F, NT, NZC>(a, fo, cl, cr, L);
+      nc = face_contributions<F, NT, NZC, !kY>(a, fo, cl, cr, L);
       __syncthreads();
       // a side is monotone when no entry's target is below the previous one's
       bool dL = false, dR = false;
