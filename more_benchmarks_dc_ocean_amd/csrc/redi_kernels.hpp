@@ -407,7 +407,11 @@ __device__ __forceinline__ double slope_taper(double hm, double topL, double bot
 // lane puts its surface's record in its wave's stage, and every run of
 // kSweepRunWords / (record words) surfaces the wave writes the stage out
 // transposed -- 8 lanes per face, 8 faces per store instruction -- so every
-// store fills whole 128-byte runs of records.
+// store fills whole 128-byte runs of records.  A stage belongs to one wave,
+// and a wave's LDS instructions execute in order, so the stage needs only
+// wavefront-scope fences (which order the compiler's accesses and cost no
+// instruction), not workgroup barriers: the u and v waves of a workgroup
+// run on without waiting for each other.
 constexpr int kSweepRunWords = 8;                              // 16-byte words per face per run
 // A lane's run sits kSweepLaneWords apart from the next lane's: one word of
 // padding spreads the lanes' 16-byte stage writes over the LDS banks (with
@@ -444,7 +448,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
   };
   // surfaces s0 .. s0+cnt-1 of every face of the wave
   auto flush = [&](int s0, int cnt) {
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     const int wd = lane % kSweepRunWords, sq = wd / kW;
 #pragma unroll
     for (int it = 0; it < 64 / (64 / kSweepRunWords); ++it) {
@@ -456,7 +460,7 @@ __device__ __forceinline__ void neutral_sweep_body(const SweepArgs<F>& a, const 
         dst[wd % kW] = stw[q * kSweepLaneWords + wd];
       }
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   };
   const size_t cl = (size_t)(i - a.di) + (size_t)a.nx * (j - a.dj);
   const size_t cr = (size_t)i + (size_t)a.nx * j;

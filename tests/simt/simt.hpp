@@ -170,6 +170,10 @@ void run_grid(dim3 grid, dim3 block, size_t dyn_lds, const std::function<void()>
 #define __builtin_amdgcn_mbcnt_hi(m, a) ::simt::mbcnt_hi((m), (a))
 // the hardware reciprocal (within 1 ulp) as the quotient
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
+// a wavefront-scope fence: the lanes of a wave exchange data through LDS
+// across it, which here needs the fibers to meet (a barrier of the
+// workgroup, stronger than the wave's in-order execution it stands for)
+#define __builtin_amdgcn_fence(order, scope) ::simt::barrier(__LINE__)
 #define __popcll(x) __builtin_popcountll(x)
 // conversion rounding to nearest, ties to even (the default rounding mode)
 inline unsigned __double2uint_rn(double x) { return (unsigned)std::nearbyint(x); }
