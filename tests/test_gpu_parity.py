@@ -326,3 +326,34 @@ def test_hip_nan_input_same_as_oracle(gpu_device):
         m = away_from(got[k].shape, where, r, face)
         assert same_with_nan(got[k][m], ref[k][m]), k
         assert not np.isnan(got[k][m]).any(), k
+
+
+def test_hip_wet_masks_changing_between_steps(gpu_device):
+    """A dry face's records hold the inert values (Po = hEff = 0, Ko = 1) and
+    the sweep leaves them alone while the face stays dry.  Faces that turn dry
+    between two Phase-A calls get the inert values again, and faces that turn
+    wet get computed ones: the coefficients of each call equal the oracle's
+    for that call's masks, bit for bit."""
+    import torch
+    from more_benchmarks_dc_ocean_amd.redi import make_redi
+    base = inputs.case_fractal(41, 36, 28, 25, 2, 1, seed=41, land=0.3,
+                               fronts=[("x", 0.5, 1.0, 80_000.0)], noise_sigma=2.0, noise_T=0.5,
+                               noise_S=0.05)
+    drier = base.copy()
+    drier.wet_u[:, ::3] = 0.0  # every third column of u faces turns dry
+    drier.wet_v[::4, :] = 0.0  # every fourth row of v faces turns dry
+    assert (drier.wet_u < base.wet_u).any() and (drier.wet_v < base.wet_v).any()
+    r, dom = make_redi(base, gpu_device)
+    print(f"[gpu] wet masks {base.nx}x{base.ny}x{base.nz}: base, drier, base", flush=True)
+    h = np.ascontiguousarray(base.h)
+    hTr = np.ascontiguousarray(base.hTr)
+    for label, c in (("base", base), ("drier", drier), ("base again", base)):
+        dom.wet_u.copy_(torch.from_numpy(np.ascontiguousarray(c.wet_u)))
+        dom.wet_v.copy_(torch.from_numpy(np.ascontiguousarray(c.wet_v)))
+        assert r.calc_coeffs(dom) == 0
+        got = {k: v.cpu().numpy() for k, v in r.coeffs().items()}
+        o = redi_oracle.OracleRedi(c)
+        o.calc_coeffs(h, hTr)
+        for k, ref in o.coeffs().items():
+            assert np.array_equal(got[k], ref), f"{label}: {k} differs from the oracle"
+    r.close()
