@@ -994,15 +994,23 @@ struct FluxLds {
 
 // The contributing sublayers of face `fo` between columns cl (left) and cr
 // (right), compacted into L in s order; returns their count (wave-uniform).
-// kAhead (FP64 form): each tracer's reconstruction entries are gathered
-// while the previous tracer is computed, which costs 16 VGPRs; the u-face
-// kernel has them to spare at four waves per SIMD, the v-face kernel
-// (which also holds the cell's W + E partial) has not.
-template <class F, int NT, int NZC, bool kAhead>
+// Two ways of keeping loads in flight, each where the registers allow four
+// waves per SIMD (kY: the v-face kernel, which also holds the cell's W + E
+// partial):
+//   kAhead (FP64 form, u faces): each tracer's reconstruction entries are
+//     gathered while the previous tracer is computed (16 VGPRs);
+//   kPre (FP64 form; FP32-store form, v faces): the next chunk's surface
+//     records are loaded while this chunk is computed (8 VGPRs; the
+//     FP32-store u-face kernel would drop from five waves per SIMD to four,
+//     which costs more than it hides).
+template <class F, int NT, int NZC, bool kY>
 __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t fo, uint32_t cl,
                                                   uint32_t cr, FluxLds<typename F::C, NT, NZC>& L) {
   using C = typename F::C;
   using Face = FaceRecF<F>;
+  constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
+  constexpr bool kAhead = !kOff && !kY;
+  constexpr bool kPre = !kOff || kY;
   const int lane = (int)(threadIdx.x & 63);
   const C coef = (C)(((a.dt * a.khtr[fo]) * a.len[fo]) * a.inv[fo]);
   const typename F::Recon* recL = a.recon + cl * a.ry.ca;
@@ -1012,11 +1020,19 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
   const int ns = 2 * a.nz + 2, nsub = ns - 1;
   const Face* r = a.rec + fo * a.fa;
   int nc = 0;
+  Face rn{};  // kPre: the surface records of the chunk at hand
+  if (kPre && lane < ns) rn = r[(size_t)lane * fb];
   for (int c0 = 0; c0 < nsub; c0 += kChunk) {
     const int s = c0 + lane;
     const bool surf = s < ns;
     SurfVals Fs{0.0, 0.0, 0.0, 1, 1};
-    if (surf) Fs = r[(size_t)s * fb].get();
+    if constexpr (kPre) {
+      const Face rc = rn;
+      if (s + kChunk < ns) rn = r[(size_t)(s + kChunk) * fb];  // the next chunk's
+      if (surf) Fs = rc.get();
+    } else {
+      if (surf) Fs = r[(size_t)s * fb].get();
+    }
     const int KoL = Fs.KoL, KoR = Fs.KoR;
     const C PoL = (C)Fs.PoL, PoR = (C)Fs.PoR;
     // surface s+1, the bottom of sublayer s (lane 63's value is never used)
@@ -1030,7 +1046,6 @@ __device__ __forceinline__ int face_contributions(const FluxArgs<F>& a, size_t f
     // form gathers one tracer's four 16-byte entries at a time (kAhead: the
     // next tracer's while this one is computed), which keeps its registers,
     // and so its waves per SIMD, at the faster setting.
-    constexpr bool kOff = std::is_same<typename F::Recon, ReconRecOff>::value;
     ReconRecOff gl[kOff ? NT : 1], gr[kOff ? NT : 1];
     // FP64: entries K-1 and K of both columns (with kAhead, one tracer ahead)
     ReconPair64 pla[2], plk[2], pra[2], prk[2];
@@ -1190,7 +1205,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs<F>& a, int bid, int nb,
     int nc = 0;
     bool monoL = true, monoR = true;
     if (live) {
-      nc = face_contributions<F, NT, NZC, !kY>(a, fo, cl, cr, L);
+      nc = face_contributions<F, NT, NZC, kY>(a, fo, cl, cr, L);
       __syncthreads();
       // a side is monotone when no entry's target is below the previous one's
       bool dL = false, dR = false;
