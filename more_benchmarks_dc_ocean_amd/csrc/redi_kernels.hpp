@@ -613,8 +613,9 @@ __global__ void __launch_bounds__(256) k_neutral_sweep(SweepArgs<F> a, TileMap t
 // time, from the same caches: the column state is fetched about once for
 // both grids where two launches fetch it once each.  Both maps must number
 // the same tile grid (tiles0, tiles1); tiles outside a grid's faces have no
-// lanes in range.  Each wave runs the same sequence of barriers (ns steps of
-// one domain), so the workgroup barriers of the record staging pair up.
+// lanes in range.  Each wave stages its records in its own part of the LDS
+// and orders them with wavefront-scope fences, so neither wave waits for
+// the other.
 template <class F>
 __global__ void __launch_bounds__(128) k_neutral_sweep_uv(SweepArgs<F> au, TileMap mu, SweepArgs<F> av,
                                                           TileMap mv) {
