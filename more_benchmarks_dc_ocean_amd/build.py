@@ -3,7 +3,8 @@
 ``build()`` compiles ``csrc/redi_lib.hip`` (which includes every kernel
 header) into ``more_benchmarks_dc_ocean_amd/libredi_hip.so`` with hipcc.
 FP contraction is off so the FP64 kernels round like the reference build;
-packed-FP32 instructions are disabled.
+packed-FP32 instructions are disabled.  The library links ROCm's roctx
+library for the named ranges around its public calls.
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ def hipcc_flags() -> list:
 def build(verbose: bool = False) -> str:
     src = os.path.join(CSRC, "redi_lib.hip")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *hipcc_flags(), src, "-o", tmp]
+    cmd = [HIPCC, *hipcc_flags(), src, "-lrocprofiler-sdk-roctx", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True)

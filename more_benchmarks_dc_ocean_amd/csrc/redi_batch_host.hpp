@@ -600,6 +600,7 @@ int redi_batch_remove(redi_batch_t* b, int slot) {
 
 int redi_batch_step(redi_batch_t* b, int nsteps, void* stream) {
   if (!b) return fail(REDI_ERR_ARG, "redi_batch_step: null handle");
+  const Range range("redi_batch_step");
   if (nsteps < 0) return fail(REDI_ERR_ARG, "redi_batch_step: nsteps %d < 0", nsteps);
   hipStream_t st = (hipStream_t)stream;
   if (b->dirty) {
@@ -617,6 +618,7 @@ int redi_batch_step(redi_batch_t* b, int nsteps, void* stream) {
 
 int redi_batch_step_serial(redi_batch_t* b, int nsteps, void* stream) {
   if (!b) return fail(REDI_ERR_ARG, "redi_batch_step_serial: null handle");
+  const Range range("redi_batch_step_serial");
   if (nsteps < 0) return fail(REDI_ERR_ARG, "redi_batch_step_serial: nsteps %d < 0", nsteps);
   for (int n = 0; n < nsteps; ++n) {
     const int rc = with_batch_ops(b->form, b->group,

@@ -5,6 +5,7 @@
 // redi_apply_flux (:927-1002) with the same silent no-op guards, plus the
 // flat launchers with the argument list of redi/opt_kernel.cu:534-574.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -35,6 +36,16 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
+
+// A named host-side range around a public call, from its entry to its
+// return (rocprofv3 --marker-trace shows it beside the kernels it
+// enqueued; without a profiler attached the calls do nothing).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
 
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
@@ -557,6 +568,7 @@ int launch_flat(int nx, int ny, int nz, double dt, int nghost, int nxp, int nyp,
                 const double* wet_v, const redi_coeffs_t& cf, const double* kue, const double* kve,
                 const double* dy_cu, const double* dx_cv, const double* idxCu, const double* idyCv,
                 const double* areaT, FaceState fs, void* ws, hipStream_t st) {
+  const Range range("redi_launch (flat)");
   fs.khtr_u = cf.khtr_u;
   fs.khtr_v = cf.khtr_v;
   const Tiles tiles;
@@ -641,6 +653,7 @@ template <class F>
 int calc_coeffs_impl(const char* who, const redi_hgrid_t* grid, const redi_metrics_t* m,
                      const redi_eos_t* eos, redi_t* r, const StateView<typename F::R>& ms,
                      void* stream) {
+  const Range range(who);
   // silent guards (:585-593)
   if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
   if (ms.idx_t <= 0 || ms.idx_s <= 0) return REDI_SKIPPED;
@@ -667,6 +680,7 @@ template <class F>
 int apply_flux_impl(const char* who, const redi_hgrid_t* grid, const redi_metrics_t* m, redi_t* r,
                     const StateView<typename F::R>& ms, double dt, const double* kue,
                     const double* kve, const redi_bc_t* bc, void* stream) {
+  const Range range(who);
   // silent guards (:951-959, :978)
   if (!r || !r->is_init || !r->enable) return REDI_SKIPPED;
   if (!ms.h || !ms.tracers || ms.nt < 1) return REDI_SKIPPED;
